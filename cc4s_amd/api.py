@@ -1,0 +1,291 @@
+"""ctypes binding of the C ABI in include/cc4s_ccsd.h.
+
+Arrays cross the boundary in the cc4s layout (first index fastest), i.e.
+numpy arrays in Fortran order indexed in the reference's index order
+(T2[a,b,i,j], Gamma[G,p,q]).  The library must be built
+(``python -m cc4s_amd.build``); there is no fallback path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import pathlib
+
+import numpy as np
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libcc4s_ccsd.so"
+
+FIELD_REAL = 0
+FIELD_COMPLEX = 1
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+# every symbol declared in include/cc4s_ccsd.h
+EXPORTS = (
+    "cc4s_ccsd_default_params", "cc4s_ccsd_create", "cc4s_ccsd_set_inputs",
+    "cc4s_ccsd_set_inputs_device", "cc4s_ccsd_set_amplitudes",
+    "cc4s_ccsd_iterate", "cc4s_ccsd_run", "cc4s_ccsd_get_amplitudes",
+    "cc4s_ccsd_residuum", "cc4s_ccsd_energy", "cc4s_ccsd_set_profiling",
+    "cc4s_ccsd_ladder_stats", "cc4s_ccsd_kernel_stats",
+    "cc4s_ccsd_last_error", "cc4s_ccsd_destroy",
+    "cc4s_contract",
+)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("field", C.c_int),
+        ("No", C.c_int64), ("Nv", C.c_int64), ("NG", C.c_int64),
+        ("max_iterations", C.c_int64),
+        ("energy_convergence", C.c_double),
+        ("amplitudes_convergence", C.c_double),
+        ("level_shift", C.c_double),
+        ("max_residua", C.c_int64),
+        ("ppl", C.c_int),
+        ("integrals_slice_size", C.c_int64),
+        ("precision", C.c_int),
+        ("mixed_switch", C.c_double),
+        ("device", C.c_int),
+    ]
+
+
+class Iteration(C.Structure):
+    _fields_ = [
+        ("iteration", C.c_int64),
+        ("energy", C.c_double), ("delta_energy", C.c_double),
+        ("residuum_norm", C.c_double),
+        ("direct", C.c_double), ("exchange", C.c_double),
+        ("seconds", C.c_double), ("flops", C.c_double),
+        ("converged", C.c_int), ("precision_used", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("correlation", C.c_double), ("direct", C.c_double),
+        ("exchange", C.c_double), ("second_order", C.c_double),
+        ("unit", C.c_double), ("convergence_reached", C.c_int),
+        ("iterations", C.c_int64),
+    ]
+
+
+class CcsdError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: pathlib.Path | str | None = None):
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = pathlib.Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise CcsdError(f"{p} not built: run python -m cc4s_amd.build")
+    lib = C.CDLL(str(p))
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    ctxp = C.c_void_p
+    lib.cc4s_ccsd_default_params.argtypes = [C.POINTER(Params)]
+    lib.cc4s_ccsd_default_params.restype = None
+    lib.cc4s_ccsd_create.argtypes = [C.POINTER(Params), C.POINTER(ctxp)]
+    lib.cc4s_ccsd_set_inputs.argtypes = [ctxp, vp, vp]
+    lib.cc4s_ccsd_set_inputs_device.argtypes = [ctxp, vp, vp]
+    lib.cc4s_ccsd_set_amplitudes.argtypes = [ctxp, vp, vp]
+    lib.cc4s_ccsd_iterate.argtypes = [ctxp, C.POINTER(Iteration)]
+    lib.cc4s_ccsd_run.argtypes = [ctxp, C.POINTER(Result), C.POINTER(Iteration),
+                                  C.c_int64]
+    lib.cc4s_ccsd_get_amplitudes.argtypes = [ctxp, vp, vp]
+    lib.cc4s_ccsd_residuum.argtypes = [ctxp, vp, vp, vp, vp]
+    lib.cc4s_ccsd_energy.argtypes = [ctxp, vp, vp, dp]
+    lib.cc4s_ccsd_set_profiling.argtypes = [ctxp, C.c_int]
+    lib.cc4s_ccsd_ladder_stats.argtypes = [ctxp, dp, dp, C.POINTER(C.c_int64)]
+    lib.cc4s_ccsd_kernel_stats.argtypes = [ctxp, C.c_char_p, C.POINTER(C.c_int64),
+                                           dp, dp]
+    lib.cc4s_ccsd_last_error.argtypes = [ctxp]
+    lib.cc4s_ccsd_last_error.restype = C.c_char_p
+    lib.cc4s_ccsd_destroy.argtypes = [ctxp]
+    lib.cc4s_ccsd_destroy.restype = None
+    i64p = C.POINTER(C.c_int64)
+    lib.cc4s_contract.argtypes = [C.c_int, C.c_double, vp, C.c_char_p, i64p, C.c_int,
+                                  vp, C.c_char_p, i64p, C.c_int, C.c_double, vp,
+                                  C.c_char_p, i64p, C.c_int]
+    for name in ("cc4s_ccsd_create", "cc4s_ccsd_set_inputs",
+                 "cc4s_ccsd_set_inputs_device", "cc4s_ccsd_set_amplitudes",
+                 "cc4s_ccsd_iterate", "cc4s_ccsd_run", "cc4s_ccsd_get_amplitudes",
+                 "cc4s_ccsd_residuum", "cc4s_ccsd_energy",
+                 "cc4s_ccsd_set_profiling", "cc4s_ccsd_ladder_stats",
+                 "cc4s_ccsd_kernel_stats", "cc4s_contract"):
+        getattr(lib, name).restype = C.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def default_params() -> Params:
+    p = Params()
+    load_library().cc4s_ccsd_default_params(C.byref(p))
+    return p
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _fortran(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+class CoupledCluster:
+    """One CCSD run on the GPU: the CoupledCluster step with method Ccsd and
+    DiisMixer, arguments named as in a cc4s operation file."""
+
+    def __init__(self, No, Nv, NG, field="complex", maxIterations=16,
+                 energyConvergence=1e-7, amplitudesConvergence=1e-6,
+                 levelShift=0.0, maxResidua=4, ppl=True,
+                 integralsSliceSize=0, device=0, precision=PRECISION_FP64):
+        self.lib = load_library()
+        p = default_params()
+        p.field = FIELD_COMPLEX if field == "complex" else FIELD_REAL
+        p.No, p.Nv, p.NG = No, Nv, NG
+        p.max_iterations = maxIterations
+        p.energy_convergence = energyConvergence
+        p.amplitudes_convergence = amplitudesConvergence
+        p.level_shift = levelShift
+        p.max_residua = maxResidua
+        p.ppl = 1 if ppl else 0
+        p.integrals_slice_size = integralsSliceSize
+        p.device = device
+        p.precision = precision
+        self.params = p
+        self.field = field
+        self.No, self.Nv, self.NG = No, Nv, NG
+        self.dtype = np.complex128 if field == "complex" else np.float64
+        self._ctx = C.c_void_p()
+        rc = self.lib.cc4s_ccsd_create(C.byref(p), C.byref(self._ctx))
+        if rc != 0:
+            raise CcsdError(self.lib.cc4s_ccsd_last_error(None).decode())
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CcsdError(self.lib.cc4s_ccsd_last_error(self._ctx).decode())
+
+    def close(self):
+        if self._ctx:
+            self.lib.cc4s_ccsd_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_inputs(self, gamma, eps):
+        g = _fortran(gamma, np.complex128)
+        Np = self.No + self.Nv
+        if g.shape != (self.NG, Np, Np):
+            raise ValueError(f"gamma shape {g.shape} != {(self.NG, Np, Np)}")
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        self._check(self.lib.cc4s_ccsd_set_inputs(self._ctx, _ptr(g), _ptr(e)))
+
+    def set_inputs_device(self, gamma_device_ptr: int, eps):
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        self._check(self.lib.cc4s_ccsd_set_inputs_device(
+            self._ctx, C.c_void_p(gamma_device_ptr), _ptr(e)))
+
+    def set_amplitudes(self, T1, T2):
+        t1, t2 = _fortran(T1, self.dtype), _fortran(T2, self.dtype)
+        self._check(self.lib.cc4s_ccsd_set_amplitudes(self._ctx, _ptr(t1), _ptr(t2)))
+
+    def iterate(self):
+        it = Iteration()
+        self._check(self.lib.cc4s_ccsd_iterate(self._ctx, C.byref(it)))
+        return it.as_dict()
+
+    def run(self):
+        res = Result()
+        cap = int(self.params.max_iterations) + 1
+        trace = (Iteration * cap)()
+        self._check(self.lib.cc4s_ccsd_run(self._ctx, C.byref(res), trace, cap))
+        n = res.iterations
+        energy = {"correlation": res.correlation, "direct": res.direct,
+                  "exchange": res.exchange, "unit": res.unit}
+        if not math.isnan(res.second_order):
+            energy["secondOrder"] = res.second_order
+        return {"energy": energy,
+                "convergenceReached": bool(res.convergence_reached),
+                "iterations": n,
+                "trace": [trace[k].as_dict() for k in range(n)]}
+
+    def amplitudes(self):
+        T1 = np.zeros((self.Nv, self.No), dtype=self.dtype, order="F")
+        T2 = np.zeros((self.Nv, self.Nv, self.No, self.No), dtype=self.dtype, order="F")
+        self._check(self.lib.cc4s_ccsd_get_amplitudes(self._ctx, _ptr(T1), _ptr(T2)))
+        return T1, T2
+
+    def residuum(self, amplitudes=None):
+        R1 = np.zeros((self.Nv, self.No), dtype=self.dtype, order="F")
+        R2 = np.zeros((self.Nv, self.Nv, self.No, self.No), dtype=self.dtype, order="F")
+        if amplitudes is None:
+            t1 = t2 = None
+        else:
+            t1 = _fortran(amplitudes[0], self.dtype)
+            t2 = _fortran(amplitudes[1], self.dtype)
+        self._check(self.lib.cc4s_ccsd_residuum(self._ctx, _ptr(t1), _ptr(t2),
+                                                _ptr(R1), _ptr(R2)))
+        return R1, R2
+
+    def energy(self, amplitudes):
+        t1 = _fortran(amplitudes[0], self.dtype)
+        t2 = _fortran(amplitudes[1], self.dtype)
+        out = (C.c_double * 6)()
+        self._check(self.lib.cc4s_ccsd_energy(self._ctx, _ptr(t1), _ptr(t2), out))
+        return complex(out[0], out[1]), complex(out[2], out[3]), complex(out[4], out[5])
+
+    def set_profiling(self, enabled=True):
+        self._check(self.lib.cc4s_ccsd_set_profiling(self._ctx, 1 if enabled else 0))
+
+    def kernel_stats(self, variant):
+        n, ms, fl = C.c_int64(), C.c_double(), C.c_double()
+        self._check(self.lib.cc4s_ccsd_kernel_stats(
+            self._ctx, variant.encode(), C.byref(n), C.byref(ms), C.byref(fl)))
+        return {"launches": n.value, "ms": ms.value, "flops": fl.value}
+
+    def ladder_stats(self):
+        ms, fl, n = C.c_double(), C.c_double(), C.c_int64()
+        self._check(self.lib.cc4s_ccsd_ladder_stats(self._ctx, C.byref(ms),
+                                                    C.byref(fl), C.byref(n)))
+        return ms.value, fl.value, n.value
+
+
+def contract(alpha, A, idx_a, B, idx_b, beta, C_, idx_c, conj_a=False,
+             conj_b=False, device=0):
+    """GPU contraction with host arrays (indexed in index-string order)."""
+    lib = load_library()
+    cplx = any(np.iscomplexobj(x) for x in (A, B, C_))
+    dt = np.complex128 if cplx else np.float64
+    a, b = _fortran(A, dt), _fortran(B, dt)
+    c = _fortran(C_, dt).copy(order="F")
+    la = (C.c_int64 * max(1, a.ndim))(*a.shape)
+    lb = (C.c_int64 * max(1, b.ndim))(*b.shape)
+    lc = (C.c_int64 * max(1, c.ndim))(*c.shape)
+    rc = lib.cc4s_contract(FIELD_COMPLEX if cplx else FIELD_REAL, alpha,
+                           _ptr(a), idx_a.encode(), la, int(conj_a),
+                           _ptr(b), idx_b.encode(), lb, int(conj_b),
+                           beta, _ptr(c), idx_c.encode(), lc, device)
+    if rc != 0:
+        raise CcsdError(lib.cc4s_ccsd_last_error(None).decode())
+    return c

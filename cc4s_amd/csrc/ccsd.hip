@@ -1,0 +1,401 @@
+// CCSD residuum on the GPU.  Each statement below is one tcc statement of
+// Ccsd.cxx (cited by line); three-factor products are split into two
+// pairwise contractions in the cheaper order.  Conjugate transposes and
+// slices of Gamma are strided views (no copies); the particle-particle
+// ladder forms V_xycd slabs from the dressed vertices and never holds the
+// full v^4 tensor.
+#include "ccsd.hpp"
+
+#include "elementwise.hpp"
+
+#include <algorithm>
+#include <chrono>
+
+namespace cc4s_amd {
+
+namespace {
+
+inline void *offset_ptr(const void *p, int64_t elements, Field f) {
+  return static_cast<char *>(const_cast<void *>(p)) + elements * scalar_bytes(f);
+}
+
+}  // namespace
+
+struct Ccsd::Work {
+  // Coulomb integral slices (complex: 7, real: 3)
+  Tensor Vpphh, Vphhp, Vhhpp, Vphph, Vhhhh, Vhhhp, Vphhh;
+  // intermediates
+  Tensor X, Y, Kac, Lac, Kki, Lki, Kck, Xakic, Xakci, Xklij;
+  Tensor DGph, DGpp, DGhh, Dpp;  // dressed vertices
+  Tensor wG, Zgvo, Wgvo, Wvooo, Woo;
+  Tensor slab, Rxy;
+  int64_t NGe = 0;  // length of the G index of the real-path vertex views
+};
+
+Ccsd::Ccsd(ContractionEngine &E, const CcsdShape &shape, const CcsdOptions &opt,
+           void *gamma, bool gamma_real)
+    : E_(E), s_(shape), opt_(opt), gamma_(gamma), gamma_real_(gamma_real),
+      w_(new Work) {
+  if (opt_.integrals_slice_size <= 0) opt_.integrals_slice_size = s_.No;
+  const Field f = s_.field;
+  const int64_t o = s_.No, v = s_.Nv;
+  // the real path sees Re and Im of Gamma as one real index of length 2 NG
+  // (Ccsd.cxx:122-136 splits them; every product there is Re*Re + Im*Im)
+  w_->NGe = f == Field::Real ? (gamma_real_ ? s_.NG : 2 * s_.NG) : s_.NG;
+  const int64_t G = w_->NGe;
+  auto &w = *w_;
+  w.X.reset(f, "abij", {v, v, o, o});
+  w.Kac.reset(f, "ac", {v, v});
+  w.Lac.reset(f, "ac", {v, v});
+  w.Kki.reset(f, "ki", {o, o});
+  w.Lki.reset(f, "ki", {o, o});
+  w.Kck.reset(f, "ck", {v, o});
+  w.Xakic.reset(f, "akic", {v, o, o, v});
+  w.Xakci.reset(f, "akci", {v, o, v, o});
+  w.Xklij.reset(f, "klij", {o, o, o, o});
+  w.DGph.reset(f, "Gai", {G, v, o});
+  w.DGpp.reset(f, "Gab", {G, v, v});
+  w.DGhh.reset(f, "Gij", {G, o, o});
+  w.wG.reset(f, "G", {G});
+  w.Zgvo.reset(f, "Gak", {G, v, o});
+  w.Wgvo.reset(f, "Gai", {G, v, o});
+  w.Wvooo.reset(f, "akij", {v, o, o, o});
+  w.Woo.reset(f, "ki", {o, o});
+  if (f == Field::Complex) {
+    w.Y.reset(f, "abij", {v, v, o, o});
+    w.Dpp.reset(f, "Gab", {G, v, v});
+  }
+}
+
+Ccsd::~Ccsd() = default;
+
+const View &Ccsd::Vpphh() const { return w_->Vpphh.v; }
+
+// Gamma slice `part` (two of h/p) with its dimensions named `names`;
+// conjT gives the conjugate transpose cT Gamma_XY[G,x,y] = conj Gamma_YX[G,y,x]
+// (Ccsd.cxx:431-436).  Real path: the real view over (Re, Im) pairs.
+View Ccsd::gam(const char *part, const std::string &names, bool conjT) const {
+  const int64_t No = s_.No, Np = s_.Np(), NG = s_.NG;
+  auto range = [&](char c) {
+    return c == 'h' ? std::make_pair<int64_t, int64_t>(0, int64_t(No))
+                    : std::make_pair<int64_t, int64_t>(int64_t(No), int64_t(Np));
+  };
+  View full;
+  if (s_.field == Field::Complex) {
+    full = dense_view(gamma_, Field::Complex, "GPQ", {NG, Np, Np});
+  } else {
+    full.ptr = gamma_;
+    full.field = Field::Real;
+    full.idx = "GPQ";
+    full.lens = {w_->NGe, Np, Np};
+    full.strides = {gamma_real_ ? 2 : 1, 2 * NG, 2 * NG * Np};
+  }
+  View v;
+  if (!conjT) {
+    auto r1 = range(part[0]), r2 = range(part[1]);
+    v = full.sliced(1, r1.first, r1.second).sliced(2, r2.first, r2.second);
+  } else {
+    auto r1 = range(part[1]), r2 = range(part[0]);
+    v = full.sliced(1, r1.first, r1.second).sliced(2, r2.first, r2.second);
+    v = v.transposed({0, 2, 1});
+    if (s_.field == Field::Complex) v = v.conjugated();
+  }
+  return v.as(names);
+}
+
+void Ccsd::form_integrals() {
+  auto &w = *w_;
+  const Field f = s_.field;
+  const int64_t o = s_.No, v = s_.Nv;
+  auto len = [&](char c) { return c == 'h' ? o : v; };
+  auto make = [&](Tensor &V, const char *name) {
+    V.reset(f, "pqsr", {len(name[0]), len(name[1]), len(name[2]), len(name[3])});
+    const std::string LI{name[0], name[2]}, RI{name[1], name[3]};
+    if (f == Field::Complex) {
+      // V[p,q,s,r] = sum_G conj(Gamma_{LI LO}[G,s,p]) Gamma_{RO RI}[G,q,r]
+      // (VertexCoulombIntegrals.cxx:143-171)
+      E_.contract(1.0, gam(LI.c_str(), "Gps", true), gam(RI.c_str(), "Gqr"), 0.0,
+                  V("pqsr"));
+    } else {
+      // V[p,q,s,r] = sum_G ReGamma_{LO LI}[G,p,s] ReGamma_{RO RI}[G,q,r] + Im Im
+      // (VertexCoulombIntegrals.cxx:87-102)
+      E_.contract(1.0, gam(LI.c_str(), "Gps"), gam(RI.c_str(), "Gqr"), 0.0,
+                  V("pqsr"));
+    }
+  };
+  make(w.Vpphh, "pphh");
+  make(w.Vhhhh, "hhhh");
+  make(w.Vhhhp, "hhhp");
+  if (f == Field::Complex) {
+    make(w.Vphhp, "phhp");
+    make(w.Vhhpp, "hhpp");
+    make(w.Vphph, "phph");
+    make(w.Vphhh, "phhh");
+  }
+  CC4S_HIP(hipStreamSynchronize(E_.stream()));
+}
+
+void Ccsd::get_residuum(const void *T, void *R) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (s_.field == Field::Complex)
+    residuum_complex(T, R);
+  else
+    residuum_real(T, R);
+  if (profile) {
+    CC4S_HIP(hipStreamSynchronize(E_.stream()));
+    timings.total_ms += std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - t0).count();
+  }
+}
+
+// Particle-particle ladder (Ccsd.cxx:549-600 complex, 283-333 real):
+// for slice pairs m <= n of size s, V_xycd = sum_G left[G,x,c] right[G,y,d]
+// (x in slice n, y in slice m), R_xyij = V_xycd tau_cdij added to R2 at
+// (a,b) and, for a > b, transposed at (b,a,j,i).  The x range of a slice is
+// processed in chunks so that a V slab stays within a fixed HBM budget.
+void Ccsd::ladder(const View &left, const View &right, const View &tauv,
+                  const View &R2) {
+  auto &w = *w_;
+  const Field f = s_.field;
+  const int64_t v = s_.Nv, o = s_.No, s = opt_.integrals_slice_size;
+  const int64_t ns = (v + s - 1) / s;
+  const size_t es = scalar_bytes(f);
+  const size_t slab_budget = size_t(2) << 30;
+  const int64_t per_x = v * v * std::min(s, v) * static_cast<int64_t>(es);
+  const int64_t cx_max = std::max<int64_t>(1, std::min<int64_t>(s, slab_budget / per_x));
+  if (w.slab.size() < v * v * cx_max * std::min(s, v)) {
+    w.slab.reset(f, "cxdy", {v, cx_max, v, std::min(s, v)});
+    w.Rxy.reset(f, "xyij", {cx_max, std::min(s, v), o, o});
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (profile) {
+    CC4S_HIP(hipEventCreate(&e0));
+    CC4S_HIP(hipEventCreate(&e1));
+    CC4S_HIP(hipEventRecord(e0, E_.stream()));
+  }
+  const double flops0 = E_.stats.flops;
+  for (int64_t m = 0; m < ns; ++m) {
+    for (int64_t n = m; n < ns; ++n) {
+      const int64_t a = n * s, b = m * s;
+      const int64_t Nx = std::min(s, v - a), Ny = std::min(s, v - b);
+      const View Rv = right.sliced(1, b, b + Ny).as("Gyd");
+      for (int64_t x0 = 0; x0 < Nx; x0 += cx_max) {
+        const int64_t cx = std::min(cx_max, Nx - x0);
+        const View Lv = left.sliced(1, a + x0, a + x0 + cx).as("Gxc");
+        const View slab = dense_view(w.slab.buf.get(), f, "cxdy", {v, cx, v, Ny});
+        E_.contract(1.0, Lv, Rv, 0.0, slab);                       // :588-589
+        const View Rxy = dense_view(w.Rxy.buf.get(), f, "xyij", {cx, Ny, o, o});
+        E_.contract(1.0, slab, tauv.as("cdij"), 0.0, Rxy);         // :590
+        View blk = R2.sliced(0, a + x0, a + x0 + cx).sliced(1, b, b + Ny);
+        E_.sum(1.0, Rxy, 1.0, blk.as("xyij"));                     // :591
+        if (a > b) {                                               // :593-596
+          View mir = R2.sliced(0, b, b + Ny).sliced(1, a + x0, a + x0 + cx);
+          E_.sum(1.0, Rxy, 1.0, mir.as("yxji"));
+        }
+      }
+    }
+  }
+  if (profile) {
+    CC4S_HIP(hipEventRecord(e1, E_.stream()));
+    CC4S_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CC4S_HIP(hipEventElapsedTime(&ms, e0, e1));
+    timings.ladder_ms += ms;
+    timings.ladder_flops += E_.stats.flops - flops0;
+    timings.ladder_calls += 1;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+}
+
+void Ccsd::residuum_complex(const void *Tp, void *Rp) {
+  auto &w = *w_;
+  auto &E = E_;
+  const Field f = Field::Complex;
+  const int64_t o = s_.No, v = s_.Nv;
+  const hipStream_t st = E.stream();
+  const View R1 = dense_view(Rp, f, "ai", {v, o});
+  const View R2 = dense_view(offset_ptr(Rp, s_.n1(), f), f, "abij", {v, v, o, o});
+  const View &Vpphh = w.Vpphh.v;
+  if (!Tp) {                                                      // :404-409
+    E.scale(0.0, R1);
+    E.sum(1.0, Vpphh, 0.0, R2);
+    return;
+  }
+  const View T1 = dense_view(const_cast<void *>(Tp), f, "ai", {v, o});
+  const View T2 = dense_view(offset_ptr(Tp, s_.n1(), f), f, "abij", {v, v, o, o});
+  auto C = [&](double al, const View &A, const View &B, double be, const View &Cv) {
+    E.contract(al, A, B, be, Cv);
+  };
+  auto S = [&](double al, const View &A, double be, const View &Cv) {
+    E.sum(al, A, be, Cv);
+  };
+  auto &X = w.X, &Y = w.Y;
+  auto &Vhhpp = w.Vhhpp, &Vhhhp = w.Vhhhp, &Vphhh = w.Vphhh, &Vphhp = w.Vphhp;
+
+  tau(st, f, X.buf.get(), T2.ptr, T1.ptr, 1.0, o, v);            // :460-461
+  tau(st, f, Y.buf.get(), T2.ptr, T1.ptr, 2.0, o, v);            // :462-463
+  C(-2.0, Vhhpp("klcd"), X("adkl"), 0.0, w.Kac("ac"));          // :465
+  C(1.0, Vhhpp("kldc"), X("adkl"), 1.0, w.Kac("ac"));           // :466
+  S(1.0, w.Kac("ac"), 0.0, w.Lac("ac"));                        // :468
+  C(1.0, gam("hp", "Gkd"), T1.as("dk"), 0.0, w.wG("G"));        // :469
+  C(2.0, gam("pp", "Gac", true), w.wG("G"), 1.0, w.Lac("ac"));
+  C(1.0, gam("pp", "Gad", true), T1.as("dk"), 0.0, w.Zgvo("Gak"));  // :470
+  C(-1.0, w.Zgvo("Gak"), gam("hp", "Gkc"), 1.0, w.Lac("ac"));
+  C(2.0, Vhhpp("klcd"), X("cdil"), 0.0, w.Kki("ki"));           // :472
+  C(-1.0, Vhhpp("kldc"), X("cdil"), 1.0, w.Kki("ki"));          // :473
+  S(1.0, w.Kki("ki"), 0.0, w.Lki("ki"));                        // :476
+  C(2.0, Vhhhp("klic"), T1.as("cl"), 1.0, w.Lki("ki"));         // :477
+  C(-1.0, Vhhhp("lkic"), T1.as("cl"), 1.0, w.Lki("ki"));        // :478
+  C(1.0, w.Lac("ac"), T2.as("cbij"), 0.0, R2);                  // :481
+  C(-1.0, w.Lki("ki"), T2.as("abkj"), 1.0, R2);                 // :484
+  S(1.0, gam("ph", "Gai", true), 0.0, w.DGph("Gai"));           // :487
+  C(-1.0, gam("hh", "Gki", true), T1.as("ak"), 1.0, w.DGph("Gai"));  // :488
+  C(1.0, gam("pp", "Gbc"), T1.as("cj"), 0.0, w.Wgvo("Gbj"));    // :489
+  C(1.0, w.DGph("Gai"), w.Wgvo("Gbj"), 1.0, R2);
+  C(-1.0, Vphhh("akij"), T1.as("bk"), 1.0, R2);                 // :490
+  C(1.0, Vphhp("akic"), T1.as("cj"), 0.0, w.Wvooo("akij"));     // :491
+  C(-1.0, w.Wvooo("akij"), T1.as("bk"), 1.0, R2);
+  C(1.0, gam("pp", "Gad", true), T1.as("di"), 1.0, w.DGph("Gai"));   // :494
+  C(-0.5, gam("hp", "Gld", true), Y("dail"), 1.0, w.DGph("Gai"));   // :495
+  C(1.0, w.DGph("Gai"), gam("hp", "Gkc"), 0.0, w.Xakic("akic"));    // :496
+  S(1.0, Vhhpp("lkdc"), 0.0, Y("dclk"));                        // :497
+  S(-0.5, Vhhpp("lkcd"), 1.0, Y("dclk"));                       // :498
+  C(1.0, Y("dclk"), T2.as("adil"), 1.0, w.Xakic("akic"));       // :499
+  S(2.0, T2.as("cbkj"), 0.0, Y("cbkj"));                        // :500
+  S(-1.0, T2.as("bckj"), 1.0, Y("cbkj"));                       // :501
+  C(1.0, w.Xakic("akic"), Y("cbkj"), 1.0, R2);                  // :502
+  S(1.0, gam("pp", "Gab", true), 0.0, w.DGpp("Gab"));           // :509
+  C(-1.0, gam("hp", "Glc", true), T1.as("al"), 1.0, w.DGpp("Gac"));  // :510
+  S(1.0, gam("hh", "Gij"), 0.0, w.DGhh("Gij"));                 // :511
+  C(1.0, gam("hp", "Gkd"), T1.as("di"), 1.0, w.DGhh("Gki"));    // :512
+  C(1.0, w.DGpp("Gac"), w.DGhh("Gki"), 0.0, w.Xakci("akci"));   // :514
+  C(-0.5, Vhhpp("lkcd"), T2.as("dail"), 1.0, w.Xakci("akci"));  // :515
+  C(-1.0, w.Xakci("akci"), T2.as("cbkj"), 1.0, R2);             // :516
+  C(-1.0, w.Xakci("bkci"), T2.as("ackj"), 1.0, R2);             // :517
+  symmetrize_pairs(st, f, R2.ptr, o, v);                        // :520
+  S(1.0, Vpphh, 1.0, R2);                                       // :529
+  S(1.0, w.Vhhhh("klij"), 0.0, w.Xklij("klij"));                // :534
+  C(1.0, Vhhhp("klic"), T1.as("cj"), 1.0, w.Xklij("klij"));     // :535
+  C(1.0, Vhhhp("lkjc"), T1.as("ci"), 1.0, w.Xklij("klij"));     // :536
+  C(1.0, w.Xklij("klij"), X("abkl"), 1.0, R2);                  // :538
+  C(1.0, Vhhpp("klcd"), X("cdij"), 0.0, w.Xklij("klij"));       // :541
+  C(1.0, w.Xklij("klij"), opt_.ppl ? T2.as("abkl") : X("abkl"), 1.0, R2);  // :542-546
+  if (opt_.ppl) {
+    S(1.0, gam("pp", "Gab"), 0.0, w.Dpp("Gab"));                // :560
+    C(-1.0, gam("hp", "Gkb"), T1.as("ak"), 1.0, w.Dpp("Gab"));  // :561
+    ladder(w.DGpp.v, w.Dpp.v, X.v, R2);                         // :564-600
+  }
+  C(1.0, w.Kac("ac"), T1.as("ci"), 0.0, R1);                    // :609
+  C(-1.0, w.Kki("ki"), T1.as("ak"), 1.0, R1);                   // :610
+  C(2.0, Vhhpp("klcd"), T1.as("dl"), 0.0, w.Kck("ck"));         // :613
+  C(-1.0, Vhhpp("kldc"), T1.as("dl"), 1.0, w.Kck("ck"));        // :614
+  C(2.0, w.Kck("ck"), T2.as("caki"), 1.0, R1);                  // :617
+  C(-1.0, w.Kck("ck"), T2.as("caik"), 1.0, R1);                 // :618
+  C(1.0, w.Kck("ck"), T1.as("ci"), 0.0, w.Woo("ki"));           // :619
+  C(1.0, w.Woo("ki"), T1.as("ak"), 1.0, R1);
+  C(2.0, Vphhp("akic"), T1.as("ck"), 1.0, R1);                  // :620
+  C(-1.0, w.Vphph("akci"), T1.as("ck"), 1.0, R1);               // :621
+  C(1.0, gam("hp", "Gkd"), X("cdik"), 0.0, w.Wgvo("Gci"));      // :622-623
+  C(2.0, gam("pp", "Gac", true), w.Wgvo("Gci"), 1.0, R1);
+  C(1.0, gam("hp", "Gkc"), X("cdik"), 0.0, w.Wgvo("Gdi"));      // :624-625
+  C(-1.0, gam("pp", "Gad", true), w.Wgvo("Gdi"), 1.0, R1);
+  C(-2.0, Vhhhp("klic"), X("ackl"), 1.0, R1);                   // :627
+  C(1.0, Vhhhp("lkic"), X("ackl"), 1.0, R1);                    // :628
+}
+
+void Ccsd::residuum_real(const void *Tp, void *Rp) {
+  auto &w = *w_;
+  auto &E = E_;
+  const Field f = Field::Real;
+  const int64_t o = s_.No, v = s_.Nv;
+  const hipStream_t st = E.stream();
+  const View R1 = dense_view(Rp, f, "ai", {v, o});
+  const View R2 = dense_view(offset_ptr(Rp, s_.n1(), f), f, "abij", {v, v, o, o});
+  const View &Vpphh = w.Vpphh.v;
+  if (!Tp) {                                                      // :97-102
+    E.scale(0.0, R1);
+    E.sum(1.0, Vpphh, 0.0, R2);
+    return;
+  }
+  const View T1 = dense_view(const_cast<void *>(Tp), f, "ai", {v, o});
+  const View T2 = dense_view(offset_ptr(Tp, s_.n1(), f), f, "abij", {v, v, o, o});
+  auto C = [&](double al, const View &A, const View &B, double be, const View &Cv) {
+    E.contract(al, A, B, be, Cv);
+  };
+  auto S = [&](double al, const View &A, double be, const View &Cv) {
+    E.sum(al, A, be, Cv);
+  };
+  auto &X = w.X;
+  auto &Vp = w.Vpphh, &Vhhhp = w.Vhhhp;
+  // gam(...) here is the real (Re, Im) vertex: Re*Re + Im*Im in one sum
+
+  tau(st, f, X.buf.get(), T2.ptr, T1.ptr, 1.0, o, v);            // :169-170
+  C(2.0, Vp("cdkl"), X("cdil"), 0.0, w.Kki("ki"));               // :172
+  C(-1.0, Vp("dckl"), X("cdil"), 1.0, w.Kki("ki"));              // :173
+  S(1.0, w.Kki("ki"), 0.0, w.Lki("ki"));                         // :175
+  C(-2.0, Vp("cdkl"), X("adkl"), 0.0, w.Kac("ac"));              // :177
+  C(1.0, Vp("dckl"), X("adkl"), 1.0, w.Kac("ac"));               // :178
+  S(1.0, w.Kac("ac"), 0.0, w.Lac("ac"));                         // :180
+  C(1.0, gam("ph", "Gdk"), T1.as("dk"), 0.0, w.wG("G"));         // :182-185
+  C(2.0, gam("pp", "Gca"), w.wG("G"), 1.0, w.Lac("ac"));
+  C(1.0, gam("pp", "Gda"), T1.as("dk"), 0.0, w.Zgvo("Gak"));     // :186-189
+  C(-1.0, w.Zgvo("Gak"), gam("ph", "Gck"), 1.0, w.Lac("ac"));
+  C(2.0, Vhhhp("klic"), T1.as("cl"), 1.0, w.Lki("ki"));          // :192
+  C(-1.0, Vhhhp("lkic"), T1.as("cl"), 1.0, w.Lki("ki"));         // :193
+  C(1.0, w.Lac("ac"), T2.as("cbij"), 0.0, R2);                   // :195
+  C(-1.0, w.Lki("ki"), T2.as("abkj"), 1.0, R2);                  // :197
+  S(1.0, gam("ph", "Gai"), 0.0, w.DGph("Gai"));                  // :201,203
+  C(-1.0, gam("hh", "Gki"), T1.as("ak"), 1.0, w.DGph("Gai"));    // :202,204
+  C(1.0, gam("pp", "Gbc"), T1.as("cj"), 0.0, w.Wgvo("Gbj"));     // :205-208
+  C(1.0, w.DGph("Gai"), w.Wgvo("Gbj"), 1.0, R2);
+  C(-1.0, Vhhhp("jika"), T1.as("bk"), 1.0, R2);                  // :209
+  C(1.0, Vp("acik"), T1.as("cj"), 0.0, w.Wvooo("akij"));         // :210
+  C(-1.0, w.Wvooo("akij"), T1.as("bk"), 1.0, R2);
+  tau(st, f, X.buf.get(), T2.ptr, T1.ptr, 2.0, o, v);            // :215
+  C(-0.5, Vp("dclk"), X("dail"), 0.0, w.Xakic("akic"));          // :216
+  C(1.0, Vp("dclk"), T2.as("adil"), 1.0, w.Xakic("akic"));       // :217
+  C(-0.5, Vp("cdlk"), T2.as("adil"), 1.0, w.Xakic("akic"));      // :218
+  C(1.0, gam("pp", "Gad"), T1.as("di"), 1.0, w.DGph("Gai"));     // :220-221
+  C(1.0, w.DGph("Gai"), gam("ph", "Gck"), 1.0, w.Xakic("akic")); // :223-224
+  S(2.0, T2.as("cbkj"), 0.0, X("cbkj"));                         // :227
+  S(-1.0, T2.as("bckj"), 1.0, X("cbkj"));                        // :228
+  C(1.0, w.Xakic("akic"), X("cbkj"), 1.0, R2);                   // :229
+  C(-0.5, Vp("cdlk"), T2.as("dail"), 0.0, w.Xakci("akci"));      // :239
+  S(1.0, gam("pp", "Gab"), 0.0, w.DGpp("Gab"));                  // :240-241
+  C(-1.0, gam("ph", "Gcl"), T1.as("al"), 1.0, w.DGpp("Gac"));    // :242-243
+  S(1.0, gam("hh", "Gij"), 0.0, w.DGhh("Gij"));                  // :244-245
+  C(1.0, gam("ph", "Gdk"), T1.as("di"), 1.0, w.DGhh("Gki"));     // :246-247
+  C(1.0, w.DGpp("Gac"), w.DGhh("Gki"), 1.0, w.Xakci("akci"));    // :248-249
+  C(-1.0, w.Xakci("akci"), T2.as("cbkj"), 1.0, R2);              // :252
+  C(-1.0, w.Xakci("bkci"), T2.as("ackj"), 1.0, R2);              // :253
+  symmetrize_pairs(st, f, R2.ptr, o, v);                         // :256
+  S(1.0, Vpphh, 1.0, R2);                                        // :263
+  tau(st, f, X.buf.get(), T2.ptr, T1.ptr, 1.0, o, v);            // :267-268
+  S(1.0, w.Vhhhh("klij"), 0.0, w.Xklij("klij"));                 // :269
+  C(1.0, Vhhhp("klic"), T1.as("cj"), 1.0, w.Xklij("klij"));      // :270
+  C(1.0, Vhhhp("lkjc"), T1.as("ci"), 1.0, w.Xklij("klij"));      // :271
+  C(1.0, w.Xklij("klij"), X("abkl"), 1.0, R2);                   // :273
+  C(1.0, Vp("cdkl"), X("cdij"), 0.0, w.Xklij("klij"));           // :275
+  C(1.0, w.Xklij("klij"), opt_.ppl ? T2.as("abkl") : X("abkl"), 1.0, R2);  // :276-280
+  if (opt_.ppl) ladder(w.DGpp.v, w.DGpp.v, X.v, R2);             // :283-333
+  C(1.0, w.Kac("ac"), T1.as("ci"), 0.0, R1);                     // :341
+  C(-1.0, w.Kki("ki"), T1.as("ak"), 1.0, R1);                    // :342
+  C(2.0, Vp("cdkl"), T1.as("dl"), 0.0, w.Kck("ck"));             // :345
+  C(-1.0, Vp("dckl"), T1.as("dl"), 1.0, w.Kck("ck"));            // :346
+  C(2.0, w.Kck("ck"), T2.as("caki"), 1.0, R1);                   // :349
+  C(-1.0, w.Kck("ck"), T2.as("caik"), 1.0, R1);                  // :350
+  C(1.0, w.Kck("ck"), T1.as("ci"), 0.0, w.Woo("ki"));            // :351
+  C(1.0, w.Woo("ki"), T1.as("ak"), 1.0, R1);
+  C(2.0, Vp("acik"), T1.as("ck"), 1.0, R1);                      // :352
+  C(1.0, gam("pp", "Gca"), T1.as("ck"), 0.0, w.Zgvo("Gak"));     // :358-361
+  C(-1.0, w.Zgvo("Gak"), gam("hh", "Gik"), 1.0, R1);
+  C(1.0, gam("ph", "Gdk"), X("cdik"), 0.0, w.Wgvo("Gci"));       // :363-366
+  C(2.0, gam("pp", "Gca"), w.Wgvo("Gci"), 1.0, R1);
+  C(1.0, gam("ph", "Gck"), X("cdik"), 0.0, w.Wgvo("Gdi"));       // :367-370
+  C(-1.0, gam("pp", "Gda"), w.Wgvo("Gdi"), 1.0, R1);
+  C(-2.0, Vhhhp("klic"), X("ackl"), 1.0, R1);                    // :372
+  C(1.0, Vhhhp("lkic"), X("ackl"), 1.0, R1);                     // :373
+}
+
+}  // namespace cc4s_amd

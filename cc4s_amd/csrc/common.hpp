@@ -1,0 +1,38 @@
+// Shared definitions for the cc4s CCSD library on MI355X (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace cc4s_amd {
+
+// Errors carry a message like the reference's Exception text; the C ABI
+// turns them into a status code plus cc4s_ccsd_last_error().
+struct Error : std::runtime_error {
+  explicit Error(const std::string &m) : std::runtime_error(m) {}
+};
+
+#define CC4S_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t err_ = (call);                                                 \
+    if (err_ != hipSuccess)                                                   \
+      throw ::cc4s_amd::Error(std::string("HIP error '") +                    \
+                              hipGetErrorString(err_) + "' at " __FILE__ ":" + \
+                              std::to_string(__LINE__) + " in " #call);      \
+  } while (0)
+
+#define CC4S_ASSERT(cond, msg)                                                \
+  do {                                                                        \
+    if (!(cond)) throw ::cc4s_amd::Error(msg);                                \
+  } while (0)
+
+// Scalar field of a run: the reference instantiates Ccsd for Real<64> and
+// Complex<64> (coupledcluster/Ccsd.cxx:635-639).
+enum class Field : int { Real = 0, Complex = 1 };
+
+inline size_t scalar_bytes(Field f) { return f == Field::Complex ? 16 : 8; }
+
+}  // namespace cc4s_amd

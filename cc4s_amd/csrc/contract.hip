@@ -1,0 +1,719 @@
+// GETT kernels (FP64 MFMA, gfx950) and the contraction planner.
+#include "contract.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+
+namespace cc4s_amd {
+
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;  // 4 wave64 per workgroup
+constexpr int kPad = 16;       // LDS row padding (doubles)
+
+struct KArgs {
+  const double *A;
+  const double *B;
+  double *C;
+  double *W;
+  const int64_t *rowA, *colA, *rowB, *colB, *rowC, *colC;
+  int64_t M, N, K, kChunk;
+  double alpha, beta;
+  double signA, signB;  // -1 conjugates the operand's imaginary part
+  int tilesM, tilesN;
+};
+
+// Workgroup tile BM x BN, K staged BK at a time through LDS; 4 waves in a
+// (BM/WM) x (BN/WN) grid, each owning WM x WN of C as 16x16 MFMA blocks.
+// A is read either along K (AKC: consecutive lanes -> consecutive k) or along
+// M; B along K (BKC) or along N.  LDS holds A as [k][m] and B as [k][n] (one
+// plane per real/imaginary part), which is exactly the lane order of the
+// MFMA operands (lane l reads row l%16 at k = l/16).
+template <bool CPLX, int BM, int BN, int BK, int WM, int WN, bool AKC, bool BKC>
+__global__ __launch_bounds__(kThreads, 2) void gett_kernel(KArgs p) {
+  constexpr int NPL = CPLX ? 2 : 1;
+  constexpr int MB = WM / 16, NB = WN / 16;
+  constexpr int WAVES_N = BN / WN;
+  static_assert((BM / WM) * (BN / WN) == 4, "four waves per workgroup");
+  static_assert(BK % 4 == 0, "BK multiple of the MFMA k");
+  constexpr int EA = BM * BK / kThreads;
+  constexpr int EB = BN * BK / kThreads;
+  static_assert(EA >= 1 && EB >= 1, "tile too small");
+  static_assert(AKC ? (kThreads % BK == 0) : (kThreads % BM == 0), "A map");
+  static_assert(BKC ? (kThreads % BK == 0) : (kThreads % BN == 0), "B map");
+  constexpr int NKA = AKC ? 1 : EA;  // distinct k per thread per stage
+  constexpr int NRA = AKC ? EA : 1;  // distinct m per thread
+  constexpr int NKB = BKC ? 1 : EB;
+  constexpr int NRB = BKC ? EB : 1;
+
+  __shared__ double As[NPL][BK][BM + kPad];
+  __shared__ double Bs[NPL][BK][BN + kPad];
+
+  const int t = threadIdx.x;
+  // tile index: spread consecutive workgroups (round-robin over the 8 XCDs)
+  // back into neighbouring tiles per XCD, then walk groups of 8 M-tiles so
+  // that concurrently running tiles share A rows and B columns in L2.
+  const int ntiles = p.tilesM * p.tilesN;
+  int g = blockIdx.x;
+  if ((ntiles & 7) == 0) g = (g & 7) * (ntiles >> 3) + (g >> 3);
+  constexpr int GM = 8;
+  const int per_group = GM * p.tilesN;
+  const int first_m = (g / per_group) * GM;
+  const int gsize = min(p.tilesM - first_m, GM);
+  const int tm = first_m + (g % per_group) % gsize;
+  const int tn = (g % per_group) / gsize;
+  const int64_t m0 = static_cast<int64_t>(tm) * BM;
+  const int64_t n0 = static_cast<int64_t>(tn) * BN;
+  const int64_t kbeg = static_cast<int64_t>(blockIdx.y) * p.kChunk;
+  const int64_t kend = min(p.K, kbeg + p.kChunk);
+  const int nstages = kend > kbeg ? static_cast<int>((kend - kbeg + BK - 1) / BK) : 0;
+
+  // fixed per-thread rows of A and columns of B
+  int64_t aRow[NRA];
+  bool aRowOk[NRA];
+#pragma unroll
+  for (int i = 0; i < NRA; ++i) {
+    const int64_t m = m0 + (AKC ? (t / BK + i * (kThreads / BK)) : (t % BM));
+    aRowOk[i] = m < p.M;
+    aRow[i] = aRowOk[i] ? p.rowA[m] : 0;
+  }
+  int64_t bCol[NRB];
+  bool bColOk[NRB];
+#pragma unroll
+  for (int i = 0; i < NRB; ++i) {
+    const int64_t n = n0 + (BKC ? (t / BK + i * (kThreads / BK)) : (t % BN));
+    bColOk[i] = n < p.N;
+    bCol[i] = bColOk[i] ? p.colB[n] : 0;
+  }
+  auto a_k = [&](int i) { return AKC ? (t % BK) : (t / BM + i * (kThreads / BM)); };
+  auto b_k = [&](int i) { return BKC ? (t % BK) : (t / BN + i * (kThreads / BN)); };
+
+  // K offsets are prefetched one stage ahead of the data they address
+  int64_t aK[NKA], aKn[NKA], bK[NKB], bKn[NKB];
+  bool aKok[NKA], aKokn[NKA], bKok[NKB], bKokn[NKB];
+  auto load_koff = [&](int s, int64_t *ak, bool *akok, int64_t *bk, bool *bkok) {
+    const int64_t kb = kbeg + static_cast<int64_t>(s) * BK;
+#pragma unroll
+    for (int i = 0; i < NKA; ++i) {
+      const int64_t k = kb + a_k(i);
+      akok[i] = k < kend;
+      ak[i] = akok[i] ? p.colA[k] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < NKB; ++i) {
+      const int64_t k = kb + b_k(i);
+      bkok[i] = k < kend;
+      bk[i] = bkok[i] ? p.rowB[k] : 0;
+    }
+  };
+
+  double ra[EA][NPL], rb[EB][NPL];
+  auto load_data = [&]() {
+#pragma unroll
+    for (int i = 0; i < EA; ++i) {
+      const int ir = AKC ? i : 0, ik = AKC ? 0 : i;
+      const bool ok = aRowOk[ir] && aKok[ik];
+      const int64_t off = aRow[ir] + aK[ik];
+      if constexpr (CPLX) {
+        double2 v = ok ? *reinterpret_cast<const double2 *>(p.A + 2 * off)
+                       : make_double2(0.0, 0.0);
+        ra[i][0] = v.x;
+        ra[i][1] = v.y * p.signA;
+      } else {
+        ra[i][0] = ok ? p.A[off] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < EB; ++i) {
+      const int ir = BKC ? i : 0, ik = BKC ? 0 : i;
+      const bool ok = bColOk[ir] && bKok[ik];
+      const int64_t off = bCol[ir] + bK[ik];
+      if constexpr (CPLX) {
+        double2 v = ok ? *reinterpret_cast<const double2 *>(p.B + 2 * off)
+                       : make_double2(0.0, 0.0);
+        rb[i][0] = v.x;
+        rb[i][1] = v.y * p.signB;
+      } else {
+        rb[i][0] = ok ? p.B[off] : 0.0;
+      }
+    }
+  };
+  auto store_lds = [&]() {
+#pragma unroll
+    for (int i = 0; i < EA; ++i) {
+      const int k = a_k(i);
+      const int m = AKC ? (t / BK + i * (kThreads / BK)) : (t % BM);
+#pragma unroll
+      for (int c = 0; c < NPL; ++c) As[c][k][m] = ra[i][c];
+    }
+#pragma unroll
+    for (int i = 0; i < EB; ++i) {
+      const int k = b_k(i);
+      const int n = BKC ? (t / BK + i * (kThreads / BK)) : (t % BN);
+#pragma unroll
+      for (int c = 0; c < NPL; ++c) Bs[c][k][n] = rb[i][c];
+    }
+  };
+
+  const int lane = t & 63;
+  const int w = t >> 6;
+  const int wm = w / WAVES_N, wn = w % WAVES_N;
+  dbl4 accR[MB][NB], accI[MB][NB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      accR[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+      accI[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  if (nstages > 0) {
+    load_koff(0, aK, aKok, bK, bKok);
+    if (nstages > 1) load_koff(1, aKn, aKokn, bKn, bKokn);
+    load_data();
+    store_lds();
+    __syncthreads();
+  }
+  for (int s = 0; s < nstages; ++s) {
+    const bool more = s + 1 < nstages;
+    if (more) {
+#pragma unroll
+      for (int i = 0; i < NKA; ++i) { aK[i] = aKn[i]; aKok[i] = aKokn[i]; }
+#pragma unroll
+      for (int i = 0; i < NKB; ++i) { bK[i] = bKn[i]; bKok[i] = bKokn[i]; }
+      if (s + 2 < nstages) load_koff(s + 2, aKn, aKokn, bKn, bKokn);
+      load_data();
+    }
+#pragma unroll
+    for (int kq = 0; kq < BK / 4; ++kq) {
+      const int kr = kq * 4 + (lane >> 4);
+      double ar[MB], ai[MB], br[NB], bi[NB];
+#pragma unroll
+      for (int i = 0; i < MB; ++i) {
+        ar[i] = As[0][kr][wm * WM + i * 16 + (lane & 15)];
+        if constexpr (CPLX) ai[i] = As[NPL - 1][kr][wm * WM + i * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        br[j] = Bs[0][kr][wn * WN + j * 16 + (lane & 15)];
+        if constexpr (CPLX) bi[j] = Bs[NPL - 1][kr][wn * WN + j * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
+      if constexpr (CPLX) {
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+          const double nai = -ai[i];
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bi[j], accR[i][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+      }
+    }
+    if (more) {
+      __syncthreads();
+      store_lds();
+      __syncthreads();
+    }
+  }
+
+  // epilogue: lane l, register r of a 16x16 block holds row 4*(l/16)+r,
+  // column l%16
+  int64_t cCol[NB];
+  bool cColOk[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int64_t n = n0 + wn * WN + j * 16 + (lane & 15);
+    cColOk[j] = n < p.N;
+    cCol[j] = (cColOk[j] && !p.W) ? p.colC[n] : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < MB; ++i) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t m = m0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+      if (m >= p.M) continue;
+      const int64_t rc = p.W ? 0 : p.rowC[m];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (!cColOk[j]) continue;
+        const int64_t n = n0 + wn * WN + j * 16 + (lane & 15);
+        const double vr = accR[i][j][r];
+        const double vi = accI[i][j][r];
+        if (p.W) {
+          const int64_t o = (static_cast<int64_t>(blockIdx.y) * p.M + m) * p.N + n;
+          if constexpr (CPLX) {
+            reinterpret_cast<double2 *>(p.W)[o] = make_double2(vr, vi);
+          } else {
+            p.W[o] = vr;
+          }
+        } else {
+          const int64_t o = rc + cCol[j];
+          if constexpr (CPLX) {
+            double2 *c = reinterpret_cast<double2 *>(p.C) + o;
+            double2 out = make_double2(p.alpha * vr, p.alpha * vi);
+            if (p.beta != 0.0) {
+              const double2 old = *c;
+              out.x += p.beta * old.x;
+              out.y += p.beta * old.y;
+            }
+            *c = out;
+          } else {
+            double *c = p.C + o;
+            double out = p.alpha * vr;
+            if (p.beta != 0.0) out += p.beta * *c;
+            *c = out;
+          }
+        }
+      }
+    }
+  }
+}
+
+// C[rowC[m] + colC[n]] = alpha * sum_s W[s][m][n] + beta * C, fixed order
+template <bool CPLX>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(
+    const double *W, double *C, const int64_t *rowC, const int64_t *colC,
+    int64_t M, int64_t N, int splits, double alpha, double beta) {
+  const int64_t total = M * N;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total;
+       e += static_cast<int64_t>(gridDim.x) * 256) {
+    const int64_t m = e / N, n = e - m * N;
+    const int64_t o = rowC[m] + colC[n];
+    if constexpr (CPLX) {
+      const double2 *w = reinterpret_cast<const double2 *>(W);
+      double sr = 0.0, si = 0.0;
+      for (int s = 0; s < splits; ++s) {
+        const double2 v = w[s * total + e];
+        sr += v.x;
+        si += v.y;
+      }
+      double2 *c = reinterpret_cast<double2 *>(C) + o;
+      double2 out = make_double2(alpha * sr, alpha * si);
+      if (beta != 0.0) {
+        const double2 old = *c;
+        out.x += beta * old.x;
+        out.y += beta * old.y;
+      }
+      *c = out;
+    } else {
+      double sr = 0.0;
+      for (int s = 0; s < splits; ++s) sr += W[s * total + e];
+      double out = alpha * sr;
+      if (beta != 0.0) out += beta * C[o];
+      C[o] = out;
+    }
+  }
+}
+
+constexpr int kMaxRank = 8;
+struct SumArgs {
+  const double *A;
+  double *C;
+  int rank;
+  int64_t lens[kMaxRank];
+  int64_t sA[kMaxRank];
+  int64_t sC[kMaxRank];
+  int64_t total;
+  double alpha, beta, signA;
+};
+
+// C = alpha * op(A) + beta * C over C's index space (first index fastest)
+template <bool CPLX>
+__global__ __launch_bounds__(256) void sum_kernel(SumArgs p) {
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < p.total;
+       e += static_cast<int64_t>(gridDim.x) * 256) {
+    int64_t r = e, oa = 0, oc = 0;
+    for (int d = 0; d < p.rank; ++d) {
+      const int64_t q = r / p.lens[d];
+      const int64_t i = r - q * p.lens[d];
+      r = q;
+      oa += i * p.sA[d];
+      oc += i * p.sC[d];
+    }
+    if constexpr (CPLX) {
+      double2 a = p.A ? reinterpret_cast<const double2 *>(p.A)[oa]
+                      : make_double2(0.0, 0.0);
+      double2 *c = reinterpret_cast<double2 *>(p.C) + oc;
+      double2 out = make_double2(p.alpha * a.x, p.alpha * p.signA * a.y);
+      if (p.beta != 0.0) {
+        const double2 old = *c;
+        out.x += p.beta * old.x;
+        out.y += p.beta * old.y;
+      }
+      *c = out;
+    } else {
+      double out = p.A ? p.alpha * p.A[oa] : 0.0;
+      if (p.beta != 0.0) out += p.beta * p.C[oc];
+      p.C[oc] = out;
+    }
+  }
+}
+
+template <bool CPLX, int BM, int BN, int BK, int WM, int WN>
+void dispatch_gett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_t s) {
+  if (L.a_kcontig && L.b_kcontig)
+    gett_kernel<CPLX, BM, BN, BK, WM, WN, true, true><<<grid, kThreads, 0, s>>>(args);
+  else if (L.a_kcontig)
+    gett_kernel<CPLX, BM, BN, BK, WM, WN, true, false><<<grid, kThreads, 0, s>>>(args);
+  else if (L.b_kcontig)
+    gett_kernel<CPLX, BM, BN, BK, WM, WN, false, true><<<grid, kThreads, 0, s>>>(args);
+  else
+    gett_kernel<CPLX, BM, BN, BK, WM, WN, false, false><<<grid, kThreads, 0, s>>>(args);
+}
+
+// tile shapes (complex: 128x64 of complex, real: 128x128)
+constexpr int kCBM = 128, kCBN = 64, kCBK = 16, kCWM = 64, kCWN = 32;
+constexpr int kRBM = 128, kRBN = 128, kRBK = 16, kRWM = 64, kRWN = 64;
+
+inline int grid_for(int64_t n) {
+  return static_cast<int>(std::min<int64_t>((n + 255) / 256, 256 * 64));
+}
+
+}  // namespace
+
+std::string gett_variant(const GettLaunch &L) {
+  std::string n = L.field == Field::Complex ? "gett_c64" : "gett_f64";
+  n += L.a_kcontig ? "_ak" : "_am";
+  n += L.b_kcontig ? "_bk" : "_bn";
+  return n;
+}
+
+void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start,
+                 hipEvent_t stop) {
+  const bool cplx = L.field == Field::Complex;
+  const int BM = cplx ? kCBM : kRBM, BN = cplx ? kCBN : kRBN;
+  const int BK = cplx ? kCBK : kRBK;
+  KArgs a;
+  a.A = static_cast<const double *>(L.A);
+  a.B = static_cast<const double *>(L.B);
+  a.C = static_cast<double *>(L.C);
+  a.W = L.splits > 1 ? static_cast<double *>(L.W) : nullptr;
+  a.rowA = L.rowA; a.colA = L.colA; a.rowB = L.rowB; a.colB = L.colB;
+  a.rowC = L.rowC; a.colC = L.colC;
+  a.M = L.M; a.N = L.N; a.K = L.K;
+  const int64_t kstages = (L.K + BK - 1) / BK;
+  a.kChunk = ((kstages + L.splits - 1) / L.splits) * BK;
+  a.alpha = L.alpha; a.beta = L.beta;
+  a.signA = L.conjA ? -1.0 : 1.0;
+  a.signB = L.conjB ? -1.0 : 1.0;
+  a.tilesM = static_cast<int>((L.M + BM - 1) / BM);
+  a.tilesN = static_cast<int>((L.N + BN - 1) / BN);
+  CC4S_ASSERT(static_cast<int64_t>(a.tilesM) * a.tilesN < (int64_t(1) << 31),
+              "contraction too large for one launch");
+  dim3 grid(a.tilesM * a.tilesN, L.splits);
+  if (start) CC4S_HIP(hipEventRecord(start, s));
+  if (cplx)
+    dispatch_gett<true, kCBM, kCBN, kCBK, kCWM, kCWN>(L, a, grid, s);
+  else
+    dispatch_gett<false, kRBM, kRBN, kRBK, kRWM, kRWN>(L, a, grid, s);
+  CC4S_HIP(hipGetLastError());
+  if (stop) CC4S_HIP(hipEventRecord(stop, s));
+  if (L.splits > 1) {
+    const int64_t total = L.M * L.N;
+    if (cplx)
+      splitk_reduce_kernel<true><<<grid_for(total), 256, 0, s>>>(
+          a.W, a.C, L.rowC, L.colC, L.M, L.N, L.splits, L.alpha, L.beta);
+    else
+      splitk_reduce_kernel<false><<<grid_for(total), 256, 0, s>>>(
+          a.W, a.C, L.rowC, L.colC, L.M, L.N, L.splits, L.alpha, L.beta);
+    CC4S_HIP(hipGetLastError());
+  }
+}
+
+// ---------------------------------------------------------------------------
+// planner
+// ---------------------------------------------------------------------------
+
+struct GettPlan {
+  int64_t M = 0, N = 0, K = 0;
+  bool swap = false, a_kc = true, b_kc = true;
+  int splits = 1;
+  DeviceBuffer tables;
+  const int64_t *rowA, *colA, *rowB, *colB, *rowC, *colC;
+};
+
+ContractionEngine::ContractionEngine(hipStream_t s) : stream_(s) {}
+
+ContractionEngine::~ContractionEngine() {
+  for (auto &p : pending_) {
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  for (auto e : event_pool_) (void)hipEventDestroy(e);
+}
+
+void ContractionEngine::set_profiling(bool on) { profiling_ = on; }
+
+void ContractionEngine::reset_timings() {
+  (void)timings();
+  timings_.clear();
+}
+
+const std::unordered_map<std::string, KernelTiming> &ContractionEngine::timings() {
+  if (!pending_.empty()) {
+    CC4S_HIP(hipStreamSynchronize(stream_));
+    for (auto &p : pending_) {
+      float ms = 0;
+      CC4S_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
+      auto &t = timings_[p.name];
+      t.launches += 1;
+      t.ms += ms;
+      t.flops += p.flops;
+      event_pool_.push_back(p.start);
+      event_pool_.push_back(p.stop);
+    }
+    pending_.clear();
+  }
+  return timings_;
+}
+
+namespace {
+
+std::string plan_key(const View &A, const View &B, const View &C) {
+  std::ostringstream k;
+  for (const View *v : {&A, &B, &C}) {
+    k << v->idx << (v->conj ? "*" : "") << ':';
+    for (size_t d = 0; d < v->lens.size(); ++d)
+      k << v->lens[d] << '/' << v->strides[d] << ',';
+    k << '|';
+  }
+  k << static_cast<int>(C.field);
+  return k.str();
+}
+
+// offsets of every multi-index of `group` (first listed index fastest)
+std::vector<int64_t> offsets(const View &v, const std::string &group,
+                             const View &shape) {
+  int64_t n = 1;
+  for (char c : group) n *= shape.len(c);
+  std::vector<int64_t> out(static_cast<size_t>(n), 0);
+  int64_t block = 1;
+  for (char c : group) {
+    const int64_t L = shape.len(c), s = v.stride(c);
+    for (int64_t e = 0; e < n; ++e) out[e] += ((e / block) % L) * s;
+    block *= L;
+  }
+  return out;
+}
+
+}  // namespace
+
+GettPlan &ContractionEngine::plan_for(const View &A0, const View &B0,
+                                      const View &C) {
+  const std::string key = plan_key(A0, B0, C);
+  auto it = plans_.find(key);
+  if (it != plans_.end()) return *it->second;
+
+  auto plan = std::make_unique<GettPlan>();
+  std::string Mi, Ni, Ki;
+  for (char c : C.idx) {
+    const bool inA = A0.find(c) >= 0, inB = B0.find(c) >= 0;
+    CC4S_ASSERT(!(inA && inB), std::string("index '") + c +
+                                   "' in both operands and result is not a contraction");
+    CC4S_ASSERT(inA || inB, std::string("result index '") + c + "' not in any operand");
+    (inA ? Mi : Ni) += c;
+  }
+  for (char c : A0.idx) {
+    if (C.find(c) >= 0) continue;
+    CC4S_ASSERT(B0.find(c) >= 0, std::string("index '") + c + "' only in one operand");
+    Ki += c;
+  }
+  for (char c : B0.idx)
+    CC4S_ASSERT(C.find(c) >= 0 || A0.find(c) >= 0,
+                std::string("index '") + c + "' only in one operand");
+  for (char c : Ki)
+    CC4S_ASSERT(A0.len(c) == B0.len(c), std::string("length mismatch of '") + c + "'");
+  for (char c : Mi) CC4S_ASSERT(A0.len(c) == C.len(c), "length mismatch (A, C)");
+  for (char c : Ni) CC4S_ASSERT(B0.len(c) == C.len(c), "length mismatch (B, C)");
+
+  // make C contiguous along N: if its fastest index belongs to A, exchange
+  // the operands and compute the transposed problem
+  char cfast = 0;
+  int64_t cmin = INT64_MAX;
+  for (int d = 0; d < C.rank(); ++d)
+    if (C.lens[d] > 1 && C.strides[d] < cmin) { cmin = C.strides[d]; cfast = C.idx[d]; }
+  plan->swap = cfast && Mi.find(cfast) != std::string::npos;
+  const View &A = plan->swap ? B0 : A0;
+  const View &B = plan->swap ? A0 : B0;
+  if (plan->swap) std::swap(Mi, Ni);
+
+  auto by = [](const View &v) {
+    return [&v](char x, char y) { return v.stride(x) < v.stride(y); };
+  };
+  std::stable_sort(Mi.begin(), Mi.end(), by(A));
+  std::stable_sort(Ni.begin(), Ni.end(), by(C));
+  bool a_unit_k = false;
+  for (char c : Ki) a_unit_k |= A.stride(c) == 1 && A.len(c) > 1;
+  std::stable_sort(Ki.begin(), Ki.end(), a_unit_k ? by(A) : by(B));
+
+  auto unit = [](const View &v, const std::string &g) {
+    return !g.empty() && v.stride(g[0]) == 1 && v.len(g[0]) > 1;
+  };
+  plan->a_kc = unit(A, Ki) || !unit(A, Mi);
+  plan->b_kc = unit(B, Ki) || !unit(B, Ni);
+
+  auto rowA = offsets(A, Mi, C), colA = offsets(A, Ki, A);
+  auto rowB = offsets(B, Ki, A), colB = offsets(B, Ni, C);
+  auto rowC = offsets(C, Mi, C), colC = offsets(C, Ni, C);
+  plan->M = static_cast<int64_t>(rowA.size());
+  plan->N = static_cast<int64_t>(colB.size());
+  plan->K = static_cast<int64_t>(colA.size());
+  if (Ki.empty()) plan->K = 1;  // outer product: one k with offset 0
+
+  std::vector<int64_t> all;
+  auto append = [&all](const std::vector<int64_t> &v) {
+    all.insert(all.end(), v.begin(), v.end());
+  };
+  append(rowA); append(colA); append(rowB); append(colB); append(rowC); append(colC);
+  plan->tables.allocate(all.size() * sizeof(int64_t));
+  CC4S_HIP(hipMemcpy(plan->tables.get(), all.data(), all.size() * sizeof(int64_t),
+                     hipMemcpyHostToDevice));
+  const int64_t *base = plan->tables.as<int64_t>();
+  plan->rowA = base; base += rowA.size();
+  plan->colA = base; base += colA.size();
+  plan->rowB = base; base += rowB.size();
+  plan->colB = base; base += colB.size();
+  plan->rowC = base; base += rowC.size();
+  plan->colC = base;
+
+  // split K when the tile grid cannot fill 256 CUs x 2 workgroups
+  const bool cplx = C.field == Field::Complex;
+  const int BM = cplx ? kCBM : kRBM, BN = cplx ? kCBN : kRBN, BK = cplx ? kCBK : kRBK;
+  const int64_t tiles = ((plan->M + BM - 1) / BM) * ((plan->N + BN - 1) / BN);
+  const int64_t kstages = (plan->K + BK - 1) / BK;
+  int64_t splits = 1;
+  if (tiles < 512) {
+    splits = std::min<int64_t>((1024 + tiles - 1) / tiles, kstages / 4);
+    const size_t es = scalar_bytes(C.field);
+    while (splits > 1 &&
+           static_cast<size_t>(splits) * plan->M * plan->N * es > max_workspace)
+      --splits;
+    splits = std::max<int64_t>(1, std::min<int64_t>(splits, 256));
+  }
+  plan->splits = static_cast<int>(splits);
+
+  auto &ref = *plan;
+  plans_.emplace(key, std::move(plan));
+  return ref;
+}
+
+void ContractionEngine::contract(double alpha, const View &A, const View &B,
+                                 double beta, const View &C) {
+  CC4S_ASSERT(A.field == C.field && B.field == C.field, "mixed scalar fields");
+  CC4S_ASSERT(!C.conj, "conjugated result view");
+  GettPlan &P = plan_for(A, B, C);
+  if (P.M == 0 || P.N == 0) return;
+  GettLaunch L;
+  L.field = C.field;
+  L.A = P.swap ? B.ptr : A.ptr;
+  L.B = P.swap ? A.ptr : B.ptr;
+  L.conjA = P.swap ? B.conj : A.conj;
+  L.conjB = P.swap ? A.conj : B.conj;
+  L.C = C.ptr;
+  L.rowA = P.rowA; L.colA = P.colA; L.rowB = P.rowB; L.colB = P.colB;
+  L.rowC = P.rowC; L.colC = P.colC;
+  L.M = P.M; L.N = P.N; L.K = P.K;
+  L.splits = P.splits;
+  L.alpha = alpha; L.beta = beta;
+  L.a_kcontig = P.a_kc; L.b_kcontig = P.b_kc;
+  L.W = nullptr;
+  if (P.splits > 1) {
+    workspace_.reserve(static_cast<size_t>(P.splits) * P.M * P.N *
+                       scalar_bytes(C.field));
+    L.W = workspace_.get();
+  }
+  const double flops = (C.field == Field::Complex ? 8.0 : 2.0) * double(P.M) *
+                       double(P.N) * double(P.K);
+  if (profiling_) {
+    auto take = [this]() {
+      hipEvent_t e;
+      if (!event_pool_.empty()) {
+        e = event_pool_.back();
+        event_pool_.pop_back();
+      } else {
+        CC4S_HIP(hipEventCreate(&e));
+      }
+      return e;
+    };
+    Pending p{take(), take(), gett_variant(L), flops};
+    launch_gett(L, stream_, p.start, p.stop);
+    pending_.push_back(p);
+    if (pending_.size() > 4096) (void)timings();
+  } else {
+    launch_gett(L, stream_);
+  }
+  stats.flops += flops;
+  stats.launches += 1;
+}
+
+void ContractionEngine::sum(double alpha, const View &A, double beta,
+                            const View &C) {
+  CC4S_ASSERT(A.field == C.field, "mixed scalar fields");
+  CC4S_ASSERT(C.rank() <= kMaxRank, "rank too large");
+  CC4S_ASSERT(!C.conj, "conjugated result view");
+  SumArgs a;
+  a.A = static_cast<const double *>(A.ptr);
+  a.C = static_cast<double *>(C.ptr);
+  a.rank = C.rank();
+  a.total = C.size();
+  for (int d = 0; d < C.rank(); ++d) {
+    const char c = C.idx[d];
+    CC4S_ASSERT(A.find(c) >= 0 && A.len(c) == C.lens[d],
+                std::string("sum: index '") + c + "' mismatch");
+    a.lens[d] = C.lens[d];
+    a.sC[d] = C.strides[d];
+    a.sA[d] = A.stride(c);
+  }
+  CC4S_ASSERT(A.rank() == C.rank(), "sum: operand has extra indices");
+  a.alpha = alpha;
+  a.beta = beta;
+  a.signA = A.conj ? -1.0 : 1.0;
+  if (a.total == 0) return;
+  if (C.field == Field::Complex)
+    sum_kernel<true><<<grid_for(a.total), 256, 0, stream_>>>(a);
+  else
+    sum_kernel<false><<<grid_for(a.total), 256, 0, stream_>>>(a);
+  CC4S_HIP(hipGetLastError());
+}
+
+void ContractionEngine::scale(double beta, const View &C) {
+  SumArgs a;
+  a.A = nullptr;
+  a.C = static_cast<double *>(C.ptr);
+  a.rank = C.rank();
+  a.total = C.size();
+  for (int d = 0; d < C.rank(); ++d) {
+    a.lens[d] = C.lens[d];
+    a.sC[d] = C.strides[d];
+    a.sA[d] = 0;
+  }
+  a.alpha = 0.0;
+  a.beta = beta;
+  a.signA = 1.0;
+  if (a.total == 0) return;
+  if (C.field == Field::Complex)
+    sum_kernel<true><<<grid_for(a.total), 256, 0, stream_>>>(a);
+  else
+    sum_kernel<false><<<grid_for(a.total), 256, 0, stream_>>>(a);
+  CC4S_HIP(hipGetLastError());
+}
+
+}  // namespace cc4s_amd

@@ -1,0 +1,97 @@
+// Tensor contraction engine: C[c] = alpha * A[a] B[b] + beta * C[c] with
+// Einstein summation over the indices absent from C.  This is the semantics
+// of one tcc contraction statement (tcc/Contraction.hpp:160-477, executed by
+// engines/CtfMachineTensor.hpp:117-133 in the reference): "<<=" is beta = 0,
+// "+=" is beta = 1, a leading scalar is alpha.
+//
+// Every contraction is executed by one hand-written gfx950 kernel family: a
+// generalised tensor-tensor contraction (GETT) on FP64 MFMA
+// (v_mfma_f64_16x16x4_f64), tiles staged through LDS, operands addressed
+// through per-mode offset tables so that any index permutation is read in
+// place (no transpose kernels), optional split-K with a deterministic
+// reduction.  Complex operands use four real MFMA products per complex MAC.
+#pragma once
+
+#include "tensor.hpp"
+
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace cc4s_amd {
+
+struct GettPlan;
+
+struct ContractStats {
+  double flops = 0;       // algorithmic flops (8 per complex MAC, 2 per real)
+  int64_t launches = 0;
+};
+
+// per kernel variant: launches, summed duration (HIP events around the GETT
+// launch on the engine's stream, split-K reduction excluded) and flops
+struct KernelTiming {
+  int64_t launches = 0;
+  double ms = 0, flops = 0;
+};
+
+class ContractionEngine {
+ public:
+  explicit ContractionEngine(hipStream_t s);
+  ~ContractionEngine();
+
+  // C = alpha * op(A) op(B) + beta * C, op = conjugation where the view says
+  void contract(double alpha, const View &A, const View &B, double beta,
+                const View &C);
+  // C = alpha * op(A) + beta * C, A's indices are a permutation of C's
+  void sum(double alpha, const View &A, double beta, const View &C);
+  // C = beta * C
+  void scale(double beta, const View &C);
+
+  hipStream_t stream() const { return stream_; }
+  ContractStats stats;
+
+  // per-launch timing of the GETT kernels (off by default)
+  void set_profiling(bool on);
+  // resolves pending events; returns timings keyed by kernel variant name
+  const std::unordered_map<std::string, KernelTiming> &timings();
+  void reset_timings();
+  // upper bound for split-K workspace (bytes)
+  size_t max_workspace = size_t(1) << 30;
+
+ private:
+  GettPlan &plan_for(const View &A, const View &B, const View &C);
+  struct Pending {
+    hipEvent_t start, stop;
+    std::string name;
+    double flops;
+  };
+  bool profiling_ = false;
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> event_pool_;
+  std::unordered_map<std::string, KernelTiming> timings_;
+  hipStream_t stream_;
+  std::unordered_map<std::string, std::unique_ptr<GettPlan>> plans_;
+  DeviceBuffer workspace_;
+};
+
+// kernel-level entry used by the planner and by tests of the kernel itself
+struct GettLaunch {
+  Field field;
+  const void *A, *B;
+  void *C;
+  void *W;  // split-K partials, splits * M * N elements, or null
+  const int64_t *rowA, *colA, *rowB, *colB, *rowC, *colC;
+  int64_t M, N, K;
+  int splits;
+  double alpha, beta;
+  bool conjA, conjB;
+  bool a_kcontig, b_kcontig;
+};
+// `start`/`stop` (optional) bracket the main kernel, not the reduction
+void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start = nullptr,
+                 hipEvent_t stop = nullptr);
+// name of the kernel variant launch_gett uses for L
+std::string gett_variant(const GettLaunch &L);
+
+}  // namespace cc4s_amd

@@ -1,0 +1,266 @@
+#include "coupled_cluster.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace cc4s_amd {
+
+// ---------------------------------------------------------------------------
+// DiisMixer
+// ---------------------------------------------------------------------------
+
+DiisMixer::DiisMixer(Field f, int64_t nset, int64_t N, hipStream_t s)
+    : f_(f), n_(nset), N_(N), s_(s) {
+  CC4S_ASSERT(N >= 1 && N <= 16, "maxResidua must be in [1, 16]");
+  A_.resize(N);
+  R_.resize(N);
+  for (int64_t i = 0; i < N; ++i) {
+    A_[i].allocate(nset * scalar_bytes(f));
+    R_[i].allocate(nset * scalar_bytes(f));
+  }
+  filled_.assign(N, false);
+  const size_t M = N + 1;
+  B_.assign(M * M, 0.0);                         // DiisMixer.cxx:42-44
+  for (size_t i = 1; i < M; ++i) B_[i] = -1.0;
+  for (size_t i = M; i < M * M; i += M) B_[i] = -1.0;
+  partials_.allocate(std::max(dots_partials_size(static_cast<int>(N)),
+                              dots_partials_size(1)));
+}
+
+// B c = (-1, 0, ..., 0) for the symmetric bordered DIIS matrix (the
+// reference calls LAPACK dsysv/zsysv, DiisMixer.cxx:119-148); Gaussian
+// elimination with partial pivoting in FP64 gives the same solution for a
+// nonsingular system.  An exactly singular system fails like info != 0.
+std::vector<double> DiisMixer::solve(std::vector<double> a, size_t n) const {
+  std::vector<double> x(n, 0.0);
+  x[0] = -1.0;
+  for (size_t k = 0; k < n; ++k) {
+    size_t piv = k;
+    for (size_t r = k + 1; r < n; ++r)
+      if (std::fabs(a[r * n + k]) > std::fabs(a[piv * n + k])) piv = r;
+    if (a[piv * n + k] == 0.0) throw Error("Diagonalization failed");
+    if (piv != k) {
+      for (size_t c = 0; c < n; ++c) std::swap(a[k * n + c], a[piv * n + c]);
+      std::swap(x[k], x[piv]);
+    }
+    for (size_t r = k + 1; r < n; ++r) {
+      const double l = a[r * n + k] / a[k * n + k];
+      if (l == 0.0) continue;
+      for (size_t c = k; c < n; ++c) a[r * n + c] -= l * a[k * n + c];
+      x[r] -= l * x[k];
+    }
+  }
+  for (size_t k = n; k-- > 0;) {
+    double s = x[k];
+    for (size_t c = k + 1; c < n; ++c) s -= a[k * n + c] * x[c];
+    x[k] = s / a[k * n + k];
+  }
+  return x;
+}
+
+void DiisMixer::append(void *out) {
+  const size_t N = static_cast<size_t>(N_);
+  filled_[next_] = true;                          // DiisMixer.cxx:65-66
+  // overlaps 2 Re <R_i|R_new> with every stored residuum (:69-77)
+  std::vector<const void *> xs;
+  std::vector<size_t> which;
+  for (size_t i = 0; i < N; ++i)
+    if (filled_[i]) {
+      xs.push_back(R_[i].get());
+      which.push_back(i);
+    }
+  std::vector<cplx> dots;
+  multi_dot(s_, f_, xs.data(), static_cast<int>(xs.size()), R_[next_].get(), n_,
+            partials_.get(), dots);
+  for (size_t k = 0; k < which.size(); ++k) {
+    const size_t i = which[k];
+    const double overlap = 2.0 * dots[k].real();
+    B_[(i + 1) * (N + 1) + next_ + 1] = overlap;
+    B_[(next_ + 1) * (N + 1) + i + 1] = overlap;
+  }
+  if (count_ < N) ++count_;                       // :80
+  const size_t dim = count_ + 1;
+  std::vector<double> matrix(dim * dim);
+  for (size_t m = 0; m < dim; ++m)
+    for (size_t n = 0; n < dim; ++n) matrix[n + dim * m] = B_[n + (N + 1) * m];
+  const std::vector<double> column = solve(matrix, dim);   // :89
+  // next = sum_j c_j A_(next-j), residuum = sum_j c_j R_(next-j) (:92-104)
+  std::vector<const void *> As, Rs;
+  coeffs_.clear();
+  for (size_t j = 0; j < count_; ++j) {
+    const size_t i = (next_ + N - j) % N;
+    As.push_back(A_[i].get());
+    Rs.push_back(R_[i].get());
+    coeffs_.push_back(column[i + 1]);
+  }
+  const double rr = combine(s_, f_, As.data(), Rs.data(), coeffs_, n_, out,
+                            partials_.get());
+  norm_ = std::sqrt(rr);                          // :106
+  next_ = (next_ + 1) % N;                        // :105
+}
+
+// ---------------------------------------------------------------------------
+// CoupledCluster
+// ---------------------------------------------------------------------------
+
+CoupledCluster::CoupledCluster(const CoupledClusterParams &p, int device)
+    : p_(p), device_(device) {
+  const auto &s = p_.shape;
+  CC4S_ASSERT(s.No > 0 && s.Nv > 0 && s.NG > 0, "empty dimension");
+  CC4S_HIP(hipSetDevice(device_));
+  CC4S_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  engine_.reset(new ContractionEngine(stream_));
+  const size_t es = scalar_bytes(s.field);
+  T_.allocate(s.nset() * es);
+  R_.allocate(s.nset() * es);
+  energy_partials_.allocate(energy_partials_size());
+  mixer_.reset(new DiisMixer(s.field, s.nset(), p_.max_residua, stream_));
+}
+
+CoupledCluster::~CoupledCluster() {
+  ccsd_.reset();
+  mixer_.reset();
+  engine_.reset();
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void CoupledCluster::set_inputs(const double *gamma, bool on_device,
+                                const double *eps) {
+  CC4S_HIP(hipSetDevice(device_));
+  const auto &s = p_.shape;
+  const size_t gbytes = static_cast<size_t>(s.NG) * s.Np() * s.Np() * 16;
+  gamma_.allocate(gbytes);
+  bool gamma_real = false;
+  if (on_device) {
+    CC4S_HIP(hipMemcpyAsync(gamma_.get(), gamma, gbytes, hipMemcpyDeviceToDevice, stream_));
+  } else {
+    CC4S_HIP(hipMemcpyAsync(gamma_.get(), gamma, gbytes, hipMemcpyHostToDevice, stream_));
+    if (s.field == Field::Real) {
+      gamma_real = true;
+      const size_t n = gbytes / 16;
+      for (size_t e = 0; e < n && gamma_real; ++e) gamma_real = gamma[2 * e + 1] == 0.0;
+    }
+  }
+  eps_h_.allocate(s.No * sizeof(double));
+  eps_p_.allocate(s.Nv * sizeof(double));
+  CC4S_HIP(hipMemcpyAsync(eps_h_.get(), eps, s.No * sizeof(double),
+                          hipMemcpyHostToDevice, stream_));
+  CC4S_HIP(hipMemcpyAsync(eps_p_.get(), eps + s.No, s.Nv * sizeof(double),
+                          hipMemcpyHostToDevice, stream_));
+  ccsd_.reset(new Ccsd(*engine_, s, p_.ccsd, gamma_.get(), gamma_real));
+  ccsd_->form_integrals();
+  CC4S_HIP(hipStreamSynchronize(stream_));
+  inputs_set_ = true;
+}
+
+void CoupledCluster::upload_set(void *dst, const double *T1, const double *T2) {
+  const auto &s = p_.shape;
+  const size_t es = scalar_bytes(s.field);
+  CC4S_HIP(hipMemcpyAsync(dst, T1, s.n1() * es, hipMemcpyHostToDevice, stream_));
+  CC4S_HIP(hipMemcpyAsync(static_cast<char *>(dst) + s.n1() * es, T2, s.n2() * es,
+                          hipMemcpyHostToDevice, stream_));
+  CC4S_HIP(hipStreamSynchronize(stream_));
+}
+
+void CoupledCluster::download_set(const void *src, double *T1, double *T2) {
+  const auto &s = p_.shape;
+  const size_t es = scalar_bytes(s.field);
+  if (T1)
+    CC4S_HIP(hipMemcpyAsync(T1, src, s.n1() * es, hipMemcpyDeviceToHost, stream_));
+  if (T2)
+    CC4S_HIP(hipMemcpyAsync(T2, static_cast<const char *>(src) + s.n1() * es,
+                            s.n2() * es, hipMemcpyDeviceToHost, stream_));
+  CC4S_HIP(hipStreamSynchronize(stream_));
+}
+
+void CoupledCluster::set_amplitudes(const double *T1, const double *T2) {
+  CC4S_HIP(hipSetDevice(device_));
+  upload_set(T_.get(), T1, T2);
+  have_amplitudes_ = true;
+}
+
+void CoupledCluster::energy(const void *T, cplx *E, cplx *D, cplx *X) {
+  const auto &s = p_.shape;
+  cc4s_amd::energy(stream_, s.field, T, ccsd_->Vpphh().ptr, s.No, s.Nv,
+                   energy_partials_.get(), D, X);
+  *E = *D + *X;
+}
+
+IterationReport CoupledCluster::iterate() {
+  CC4S_ASSERT(inputs_set_, "inputs not set");
+  CC4S_HIP(hipSetDevice(device_));
+  const auto &s = p_.shape;
+  const auto t0 = std::chrono::steady_clock::now();
+  const double flops0 = engine_->stats.flops;
+  IterationReport rep;
+  rep.iteration = iterations_done + 1;
+  const void *T = have_amplitudes_ ? T_.get() : nullptr;
+  ccsd_->get_residuum(T, R_.get());                     // :136
+  // :137-145: T' = -R/(Delta+s) and dT = T' - T, written into the mixer slot
+  amplitude_update(stream_, s.field, R_.get(), T,
+                   static_cast<const double *>(eps_h_.get()),
+                   static_cast<const double *>(eps_p_.get()), p_.level_shift,
+                   mixer_->next_amplitudes_slot(), mixer_->next_residuum_slot(),
+                   s.No, s.Nv);
+  const bool is_second_order = !have_amplitudes_;
+  mixer_->append(T_.get());                             // :145-147
+  have_amplitudes_ = true;
+  rep.residuum_norm = mixer_->residuum_norm();          // :148
+  energy(T_.get(), &rep.energy, &rep.direct, &rep.exchange);   // :149
+  CC4S_HIP(hipStreamSynchronize(stream_));
+  rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  rep.flops = engine_->stats.flops - flops0;
+  rep.delta_energy = (rep.energy - previous_e_).real();
+  if (is_second_order) second_order = rep.energy.real();      // :164-166
+  rep.converged = std::abs(rep.energy - previous_e_) < p_.energy_convergence &&
+                  rep.residuum_norm < p_.amplitudes_convergence;  // :167-173
+  previous_e_ = rep.energy;
+  iterations_done += 1;
+  return rep;
+}
+
+bool CoupledCluster::run(std::vector<IterationReport> *trace) {
+  int64_t i = 0;
+  for (; i < p_.max_iterations; ++i) {                  // :129
+    IterationReport r = iterate();
+    if (trace) trace->push_back(r);
+    if (r.converged) break;
+  }
+  CC4S_ASSERT(have_amplitudes_, "no amplitudes: maxIterations 0 needs initialAmplitudes");
+  energy(T_.get(), &final_energy, &final_direct, &final_exchange);   // :184
+  return i < p_.max_iterations;                         // :185
+}
+
+void CoupledCluster::get_amplitudes(double *T1, double *T2) {
+  CC4S_ASSERT(have_amplitudes_, "no amplitudes yet");
+  CC4S_HIP(hipSetDevice(device_));
+  download_set(T_.get(), T1, T2);
+}
+
+void CoupledCluster::residuum(const double *T1, const double *T2, double *R1,
+                              double *R2) {
+  CC4S_ASSERT(inputs_set_, "inputs not set");
+  CC4S_HIP(hipSetDevice(device_));
+  const auto &s = p_.shape;
+  const void *T = nullptr;
+  if (T1 && T2) {
+    scratch_.reserve(s.nset() * scalar_bytes(s.field));
+    upload_set(scratch_.get(), T1, T2);
+    T = scratch_.get();
+  }
+  ccsd_->get_residuum(T, R_.get());
+  download_set(R_.get(), R1, R2);
+}
+
+void CoupledCluster::energy_of(const double *T1, const double *T2, cplx *E,
+                               cplx *D, cplx *X) {
+  CC4S_ASSERT(inputs_set_, "inputs not set");
+  CC4S_HIP(hipSetDevice(device_));
+  const auto &s = p_.shape;
+  scratch_.reserve(s.nset() * scalar_bytes(s.field));
+  upload_set(scratch_.get(), T1, T2);
+  energy(scratch_.get(), E, D, X);
+}
+
+}  // namespace cc4s_amd

@@ -1,0 +1,373 @@
+// Bandwidth-bound kernels.  Reductions are two-pass with a fixed grid, so
+// every result is bitwise reproducible run to run.
+#include "elementwise.hpp"
+
+#include <cstring>
+
+namespace cc4s_amd {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kReduceBlocks = 1024;  // fixed: deterministic reduction order
+constexpr int kMaxDiis = 16;
+
+inline int grid_for(int64_t n) {
+  return static_cast<int>(std::min<int64_t>((n + kBlock - 1) / kBlock, 256 * 32));
+}
+
+template <bool CPLX>
+struct Sc;
+template <>
+struct Sc<false> {
+  using T = double;
+  __device__ static T mul(T a, T b) { return a * b; }
+  __device__ static T add(T a, T b) { return a + b; }
+  __device__ static T scale(T a, double s) { return a * s; }
+  __device__ static T zero() { return 0.0; }
+  // conj(a) * b
+  __device__ static double2 cdot(T a, T b) { return make_double2(a * b, 0.0); }
+};
+template <>
+struct Sc<true> {
+  using T = double2;
+  __device__ static T mul(T a, T b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  }
+  __device__ static T add(T a, T b) { return make_double2(a.x + b.x, a.y + b.y); }
+  __device__ static T scale(T a, double s) { return make_double2(a.x * s, a.y * s); }
+  __device__ static T zero() { return make_double2(0.0, 0.0); }
+  __device__ static double2 cdot(T a, T b) {
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+  }
+};
+
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void tau_kernel(typename Sc<CPLX>::T *X,
+                                                     const typename Sc<CPLX>::T *T2,
+                                                     const typename Sc<CPLX>::T *T1,
+                                                     double f, int64_t No, int64_t Nv) {
+  using S = Sc<CPLX>;
+  const int64_t n = Nv * Nv * No * No;
+  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kBlock) {
+    int64_t r = e;
+    const int64_t a = r % Nv; r /= Nv;
+    const int64_t b = r % Nv; r /= Nv;
+    const int64_t i = r % No;
+    const int64_t j = r / No;
+    X[e] = S::add(T2[e], S::scale(S::mul(T1[a + Nv * i], T1[b + Nv * j]), f));
+  }
+}
+
+// one thread per unordered pair {(a,i),(b,j)}: both elements read first
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void symmetrize_kernel(typename Sc<CPLX>::T *R,
+                                                            int64_t No, int64_t Nv) {
+  using S = Sc<CPLX>;
+  const int64_t n = Nv * Nv * No * No;
+  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kBlock) {
+    int64_t r = e;
+    const int64_t a = r % Nv; r /= Nv;
+    const int64_t b = r % Nv; r /= Nv;
+    const int64_t i = r % No;
+    const int64_t j = r / No;
+    const int64_t p = a + Nv * i, q = b + Nv * j;  // pair indices (a,i), (b,j)
+    if (p > q) continue;
+    const int64_t t = b + Nv * (a + Nv * (j + No * i));
+    const typename S::T x = R[e];
+    if (p == q) {
+      R[e] = S::add(x, x);
+    } else {
+      const typename S::T y = R[t];
+      const typename S::T s2 = S::add(x, y);
+      R[e] = s2;
+      R[t] = S::add(y, x);
+    }
+  }
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void update_kernel(
+    const typename Sc<CPLX>::T *R, const typename Sc<CPLX>::T *T,
+    const double *eh, const double *ep, double shift, typename Sc<CPLX>::T *Tn,
+    typename Sc<CPLX>::T *dT, int64_t No, int64_t Nv) {
+  using S = Sc<CPLX>;
+  const int64_t n1 = Nv * No;
+  const int64_t n = n1 + Nv * Nv * No * No;
+  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kBlock) {
+    double delta;
+    if (e < n1) {
+      const int64_t a = e % Nv, i = e / Nv;
+      delta = (0.0 + ep[a]) - eh[i];
+    } else {
+      int64_t r = e - n1;
+      const int64_t a = r % Nv; r /= Nv;
+      const int64_t b = r % Nv; r /= Nv;
+      const int64_t i = r % No;
+      const int64_t j = r / No;
+      delta = (((0.0 + ep[a]) - eh[i]) + ep[b]) - eh[j];
+    }
+    typename S::T rv = R[e];
+    if (T && shift != 0.0) rv = S::add(rv, S::scale(T[e], -shift));
+    const double d = -1.0 / (delta + shift);
+    const typename S::T tn = S::scale(rv, d);
+    Tn[e] = tn;
+    dT[e] = T ? S::add(tn, S::scale(T[e], -1.0)) : tn;
+  }
+}
+
+// block-level sum of double2 values held by every thread
+__device__ double2 block_sum(double2 v, double2 *sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      sh[t].x += sh[t + w].x;
+      sh[t].y += sh[t + w].y;
+    }
+    __syncthreads();
+  }
+  const double2 r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// energy partials: per block (D, X) as two double2
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void energy_kernel(const typename Sc<CPLX>::T *T,
+                                                        const typename Sc<CPLX>::T *V,
+                                                        int64_t No, int64_t Nv,
+                                                        double2 *partials) {
+  using S = Sc<CPLX>;
+  __shared__ double2 sh[kBlock];
+  const typename S::T *T1 = T;
+  const typename S::T *T2 = T + Nv * No;
+  const int64_t n = Nv * Nv * No * No;
+  double2 d = make_double2(0.0, 0.0), x = make_double2(0.0, 0.0);
+  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kBlock) {
+    int64_t r = e;
+    const int64_t a = r % Nv; r /= Nv;
+    const int64_t b = r % Nv; r /= Nv;
+    const int64_t i = r % No;
+    const int64_t j = r / No;
+    const typename S::T tau = S::add(T2[e], S::mul(T1[a + Nv * i], T1[b + Nv * j]));
+    const int64_t eba = b + Nv * (a + Nv * (i + No * j));
+    // tau * conj(V) = conj(conj(tau) * V)
+    const double2 dv = S::cdot(V[e], tau);
+    const double2 xv = S::cdot(V[eba], tau);
+    d.x += dv.x; d.y += dv.y;
+    x.x += xv.x; x.y += xv.y;
+  }
+  const double2 ds = block_sum(d, sh);
+  const double2 xs = block_sum(x, sh);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = ds;
+    partials[2 * blockIdx.x + 1] = xs;
+  }
+}
+
+// sums `rows` interleaved series of `nblocks` double2 partials, in order
+__global__ __launch_bounds__(kBlock) void finish_kernel(const double2 *partials,
+                                                        int rows, int nblocks,
+                                                        double2 *out) {
+  __shared__ double2 sh[kBlock];
+  for (int r = 0; r < rows; ++r) {
+    double2 v = make_double2(0.0, 0.0);
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+      const double2 p = partials[b * rows + r];
+      v.x += p.x;
+      v.y += p.y;
+    }
+    const double2 s = block_sum(v, sh);
+    if (threadIdx.x == 0) out[r] = s;
+  }
+}
+
+struct DotArgs {
+  const void *x[kMaxDiis];
+  const void *A[kMaxDiis];
+  const void *R[kMaxDiis];
+  double c[kMaxDiis];
+};
+
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void dots_kernel(DotArgs p, int count,
+                                                      const typename Sc<CPLX>::T *y,
+                                                      int64_t n, double2 *partials) {
+  using S = Sc<CPLX>;
+  __shared__ double2 sh[kBlock];
+  double2 acc[kMaxDiis];
+#pragma unroll
+  for (int k = 0; k < kMaxDiis; ++k) acc[k] = make_double2(0.0, 0.0);
+  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kBlock) {
+    const typename S::T yv = y[e];
+#pragma unroll
+    for (int k = 0; k < kMaxDiis; ++k) {
+      if (k < count) {
+        const double2 v = S::cdot(static_cast<const typename S::T *>(p.x[k])[e], yv);
+        acc[k].x += v.x;
+        acc[k].y += v.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kMaxDiis; ++k) {
+    if (k < count) {
+      const double2 s = block_sum(acc[k], sh);
+      if (threadIdx.x == 0) partials[blockIdx.x * count + k] = s;
+    }
+  }
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void combine_kernel(DotArgs p, int count,
+                                                         typename Sc<CPLX>::T *out,
+                                                         int64_t n, double2 *partials) {
+  using S = Sc<CPLX>;
+  __shared__ double2 sh[kBlock];
+  double2 nrm = make_double2(0.0, 0.0);
+  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kBlock) {
+    typename S::T a = S::zero(), r = S::zero();
+    for (int k = 0; k < count; ++k) {
+      a = S::add(a, S::scale(static_cast<const typename S::T *>(p.A[k])[e], p.c[k]));
+      r = S::add(r, S::scale(static_cast<const typename S::T *>(p.R[k])[e], p.c[k]));
+    }
+    out[e] = a;
+    const double2 v = S::cdot(r, r);
+    nrm.x += v.x;
+  }
+  const double2 s = block_sum(nrm, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+}  // namespace
+
+void tau(hipStream_t s, Field f, void *X, const void *T2, const void *T1,
+         double factor, int64_t No, int64_t Nv) {
+  const int64_t n = Nv * Nv * No * No;
+  if (f == Field::Complex)
+    tau_kernel<true><<<grid_for(n), kBlock, 0, s>>>(
+        static_cast<double2 *>(X), static_cast<const double2 *>(T2),
+        static_cast<const double2 *>(T1), factor, No, Nv);
+  else
+    tau_kernel<false><<<grid_for(n), kBlock, 0, s>>>(
+        static_cast<double *>(X), static_cast<const double *>(T2),
+        static_cast<const double *>(T1), factor, No, Nv);
+  CC4S_HIP(hipGetLastError());
+}
+
+void symmetrize_pairs(hipStream_t s, Field f, void *R2, int64_t No, int64_t Nv) {
+  const int64_t n = Nv * Nv * No * No;
+  if (f == Field::Complex)
+    symmetrize_kernel<true><<<grid_for(n), kBlock, 0, s>>>(static_cast<double2 *>(R2), No, Nv);
+  else
+    symmetrize_kernel<false><<<grid_for(n), kBlock, 0, s>>>(static_cast<double *>(R2), No, Nv);
+  CC4S_HIP(hipGetLastError());
+}
+
+void amplitude_update(hipStream_t s, Field f, const void *R, const void *T,
+                      const double *eps_h, const double *eps_p, double shift,
+                      void *Tnew, void *dT, int64_t No, int64_t Nv) {
+  const int64_t n = Nv * No + Nv * Nv * No * No;
+  if (f == Field::Complex)
+    update_kernel<true><<<grid_for(n), kBlock, 0, s>>>(
+        static_cast<const double2 *>(R), static_cast<const double2 *>(T), eps_h,
+        eps_p, shift, static_cast<double2 *>(Tnew), static_cast<double2 *>(dT), No, Nv);
+  else
+    update_kernel<false><<<grid_for(n), kBlock, 0, s>>>(
+        static_cast<const double *>(R), static_cast<const double *>(T), eps_h,
+        eps_p, shift, static_cast<double *>(Tnew), static_cast<double *>(dT), No, Nv);
+  CC4S_HIP(hipGetLastError());
+}
+
+size_t energy_partials_size() { return (2 * kReduceBlocks + 2) * sizeof(double2); }
+
+void energy(hipStream_t s, Field f, const void *T, const void *V, int64_t No,
+            int64_t Nv, void *partials, cplx *D, cplx *X) {
+  double2 *part = static_cast<double2 *>(partials);
+  double2 *out = part + 2 * kReduceBlocks;
+  if (f == Field::Complex)
+    energy_kernel<true><<<kReduceBlocks, kBlock, 0, s>>>(
+        static_cast<const double2 *>(T), static_cast<const double2 *>(V), No, Nv, part);
+  else
+    energy_kernel<false><<<kReduceBlocks, kBlock, 0, s>>>(
+        static_cast<const double *>(T), static_cast<const double *>(V), No, Nv, part);
+  CC4S_HIP(hipGetLastError());
+  finish_kernel<<<1, kBlock, 0, s>>>(part, 2, kReduceBlocks, out);
+  CC4S_HIP(hipGetLastError());
+  double2 h[2];
+  CC4S_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+  CC4S_HIP(hipStreamSynchronize(s));
+  // partial sums hold sum conj(V) tau; the degeneracy factors of
+  // CoupledCluster.cxx:224-231 are 0.5*2*2 = 2 (direct) and -0.5*2 = -1
+  *D = 2.0 * cplx(h[0].x, h[0].y);
+  *X = -1.0 * cplx(h[1].x, h[1].y);
+}
+
+size_t dots_partials_size(int count) {
+  return (static_cast<size_t>(kReduceBlocks) * count + count) * sizeof(double2);
+}
+
+void multi_dot(hipStream_t s, Field f, const void *const *x, int count,
+               const void *y, int64_t n, void *partials, std::vector<cplx> &out) {
+  CC4S_ASSERT(count >= 1 && count <= kMaxDiis, "too many DIIS vectors");
+  DotArgs a;
+  for (int k = 0; k < count; ++k) a.x[k] = x[k];
+  double2 *part = static_cast<double2 *>(partials);
+  double2 *res = part + static_cast<size_t>(kReduceBlocks) * count;
+  if (f == Field::Complex)
+    dots_kernel<true><<<kReduceBlocks, kBlock, 0, s>>>(
+        a, count, static_cast<const double2 *>(y), n, part);
+  else
+    dots_kernel<false><<<kReduceBlocks, kBlock, 0, s>>>(
+        a, count, static_cast<const double *>(y), n, part);
+  CC4S_HIP(hipGetLastError());
+  finish_kernel<<<1, kBlock, 0, s>>>(part, count, kReduceBlocks, res);
+  CC4S_HIP(hipGetLastError());
+  std::vector<double2> h(count);
+  CC4S_HIP(hipMemcpyAsync(h.data(), res, count * sizeof(double2),
+                          hipMemcpyDeviceToHost, s));
+  CC4S_HIP(hipStreamSynchronize(s));
+  out.resize(count);
+  for (int k = 0; k < count; ++k) out[k] = cplx(h[k].x, h[k].y);
+}
+
+double combine(hipStream_t s, Field f, const void *const *A, const void *const *R,
+               const std::vector<double> &c, int64_t n, void *out, void *partials) {
+  const int count = static_cast<int>(c.size());
+  CC4S_ASSERT(count >= 1 && count <= kMaxDiis, "too many DIIS vectors");
+  DotArgs a;
+  for (int k = 0; k < count; ++k) {
+    a.A[k] = A[k];
+    a.R[k] = R[k];
+    a.c[k] = c[k];
+  }
+  double2 *part = static_cast<double2 *>(partials);
+  double2 *res = part + kReduceBlocks;
+  if (f == Field::Complex)
+    combine_kernel<true><<<kReduceBlocks, kBlock, 0, s>>>(a, count,
+                                                          static_cast<double2 *>(out), n, part);
+  else
+    combine_kernel<false><<<kReduceBlocks, kBlock, 0, s>>>(a, count,
+                                                           static_cast<double *>(out), n, part);
+  CC4S_HIP(hipGetLastError());
+  finish_kernel<<<1, kBlock, 0, s>>>(part, 1, kReduceBlocks, res);
+  CC4S_HIP(hipGetLastError());
+  double2 h;
+  CC4S_HIP(hipMemcpyAsync(&h, res, sizeof(h), hipMemcpyDeviceToHost, s));
+  CC4S_HIP(hipStreamSynchronize(s));
+  return h.x;
+}
+
+void copy(hipStream_t s, void *y, const void *x, size_t bytes) {
+  CC4S_HIP(hipMemcpyAsync(y, x, bytes, hipMemcpyDeviceToDevice, s));
+}
+
+}  // namespace cc4s_amd

@@ -1,0 +1,235 @@
+"""Seeded, portable synthetic inputs for the CCSD step.
+
+Produces the two inputs the ``CoupledCluster``/``Ccsd`` step reads in cc4s:
+
+* the Coulomb vertex Gamma^G_pq, shape [NG, Np, Np], first index fastest
+  (cc4s tensor layout, reference/binary_tensor_file_format.txt:86-95), complex
+  interleaved (re, im) float64, as written by cc4s for ``CoulombVertex``;
+* the eigen energies eps_p, shape [Np], with ``fermiEnergy`` between the
+  occupied and virtual blocks (DefineHolesAndParticles.cxx:47-59 derives
+  No from it).
+
+Random numbers come from a counter-based SplitMix64 stream (Steele, Lea and
+Flood, OOPSLA 2014) and an explicit Box-Muller transform, so the draws do not
+depend on any library's distribution code.  Every draw is addressed by the
+element's cc4s global index, so any chunking of the work gives the same bytes.
+
+Gamma construction (per auxiliary index G, orbital pair p <= q):
+  complex field: Re, Im ~ N(0, 1/2) off the diagonal, diagonal real N(0, 1),
+                 Gamma^G_qp = conj(Gamma^G_pq)  (Hermitian in p, q);
+  real field:    N(0, 1) on and above the diagonal, Gamma^G_qp = Gamma^G_pq,
+                 stored with zero imaginary part and ``halfGrid: 1``.
+Each G-slice is scaled by w_G = c / (1 + G/16), with c chosen such that
+max_abij |V_abij| = 0.05 Eh, where V_abij is the ``pphh`` Coulomb integral
+slice the step forms from Gamma (VertexCoulombIntegrals.cxx:68-102 real,
+143-171 complex).  V is quadratic in Gamma, hence c = sqrt(0.05 / max|V|_{c=1}).
+Occupied energies: No sorted uniform draws in [-1.2, -0.4]; virtual energies:
+Nv sorted uniform draws in [0.2, 4.0]; fermiEnergy = -0.1.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+_STREAM_MUL = np.uint64(0xD1B54A32D192ED03)
+
+STREAM_GAMMA = 1
+STREAM_EPS_OCC = 2
+STREAM_EPS_VIR = 3
+STREAM_AMPLITUDES = 4
+
+TARGET_MAX_V = 0.05
+FERMI_ENERGY = -0.1
+
+
+def _mix(z):
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def _stream_base(seed: int, stream: int) -> np.uint64:
+    with np.errstate(over="ignore"):
+        key = np.array([seed], dtype=np.uint64) ^ (
+            np.array([stream], dtype=np.uint64) * _STREAM_MUL)
+        return _mix(key + _GOLDEN)[0]
+
+
+def uniforms(seed: int, stream: int, counters) -> np.ndarray:
+    """Uniform doubles in (0, 1): the SplitMix64 output number ``counter``
+    of the stream, top 53 bits, offset by half an ulp."""
+    base = _stream_base(seed, stream)
+    n = np.asarray(counters, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = _mix(base + (n + np.uint64(1)) * _GOLDEN)
+    return ((x >> np.uint64(11)).astype(np.float64) + 0.5) * (2.0 ** -53)
+
+
+def normals(seed: int, stream: int, counters) -> np.ndarray:
+    """Standard normal draw number ``counter``: Box-Muller (cosine branch)
+    on uniforms 2*counter and 2*counter+1."""
+    n = np.asarray(counters, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        u1 = uniforms(seed, stream, n * np.uint64(2))
+        u2 = uniforms(seed, stream, n * np.uint64(2) + np.uint64(1))
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+@dataclasses.dataclass(frozen=True)
+class CcsdConfig:
+    """One CCSD workload: shapes, field and solver settings."""
+    name: str
+    No: int
+    Nv: int
+    NG: int
+    field: str                       # "real" (halfGrid: 1) or "complex"
+    seed: int = 1
+    max_residua: int = 8
+    energy_convergence: float = 1e-10
+    amplitudes_convergence: float = 1e-8
+    max_iterations: int = 50
+    level_shift: float = 0.0
+
+    @property
+    def Np(self) -> int:
+        return self.No + self.Nv
+
+
+CONFIGS = {
+    0: CcsdConfig("real No=8 Nv=40 NG=200", 8, 40, 200, "real", seed=100),
+    1: CcsdConfig("complex No=32 Nv=256 NG=1500", 32, 256, 1500, "complex",
+                  seed=101),
+    2: CcsdConfig("complex No=64 Nv=512 NG=3000", 64, 512, 3000, "complex",
+                  seed=102),
+    3: CcsdConfig("complex No=32 Nv=256 NG=1500 (adjoint)", 32, 256, 1500,
+                  "complex", seed=101),
+    4: CcsdConfig("complex No=64 Nv=640 NG=4000", 64, 640, 4000, "complex",
+                  seed=104, max_residua=6),
+}
+
+
+def raw_vertex(NG: int, Np: int, field: str, seed: int,
+               q_chunk: int = 16) -> np.ndarray:
+    """Unscaled vertex (w_G = 1/(1+G/16), c = 1), shape (NG, Np, Np),
+    Fortran order so that memory is first-index-fastest like cc4s."""
+    gamma = np.zeros((NG, Np, Np), dtype=np.complex128, order="F")
+    g = np.arange(NG, dtype=np.uint64)
+    w = 1.0 / (1.0 + np.arange(NG, dtype=np.float64) / 16.0)
+    p_all = np.arange(Np, dtype=np.uint64)
+    for q0 in range(0, Np, q_chunk):
+        q1 = min(Np, q0 + q_chunk)
+        q = np.arange(q0, q1, dtype=np.uint64)
+        # global element index I = G + NG*(p + Np*q), shape (NG, Np, nq)
+        idx = (g[:, None, None] + np.uint64(NG) * (
+            p_all[None, :, None] + np.uint64(Np) * q[None, None, :]))
+        re = normals(seed, STREAM_GAMMA, np.uint64(2) * idx)
+        if field == "complex":
+            im = normals(seed, STREAM_GAMMA, np.uint64(2) * idx + np.uint64(1))
+            block = np.sqrt(0.5) * (re + 1j * im)
+        else:
+            block = re.astype(np.complex128)
+        gamma[:, :, q0:q1] = block
+    # symmetrise from the upper triangle (p <= q)
+    pp, qq = np.tril_indices(Np, -1)          # p > q entries get mirrored
+    if field == "complex":
+        gamma[:, pp, qq] = np.conj(gamma[:, qq, pp])
+        d = np.arange(Np)
+        gamma[:, d, d] = np.sqrt(2.0) * gamma[:, d, d].real
+    else:
+        gamma[:, pp, qq] = gamma[:, qq, pp]
+    gamma *= w[:, None, None]
+    return gamma
+
+
+def max_abs_vpphh(gamma: np.ndarray, No: int, field: str) -> float:
+    """max_abij |V_abij| of the pphh slice formed as the step forms it:
+    complex V_abij = sum_G conj(Gamma^G_ia) Gamma^G_bj,
+    real    V_abij = sum_G Re Gamma^G_ai Re Gamma^G_bj + Im Im."""
+    NG, Np, _ = gamma.shape
+    Nv = Np - No
+    if field == "complex":
+        left = np.conj(gamma[:, :No, No:]).transpose(0, 2, 1)   # [G,a,i]
+        right = gamma[:, No:, :No]                               # [G,b,j]
+        L = left.reshape(NG, Nv * No, order="F")
+        R = right.reshape(NG, Nv * No, order="F")
+        V = L.T @ R
+        return float(np.max(np.abs(V)))
+    ph = gamma[:, No:, :No]
+    L = np.ascontiguousarray(ph.real.reshape(NG, Nv * No, order="F"))
+    Li = np.ascontiguousarray(ph.imag.reshape(NG, Nv * No, order="F"))
+    V = L.T @ L + Li.T @ Li
+    return float(np.max(np.abs(V)))
+
+
+def eigen_energies(No: int, Nv: int, seed: int) -> np.ndarray:
+    occ = np.sort(-1.2 + 0.8 * uniforms(seed, STREAM_EPS_OCC, np.arange(No)))
+    vir = np.sort(0.2 + 3.8 * uniforms(seed, STREAM_EPS_VIR, np.arange(Nv)))
+    return np.concatenate([occ, vir])
+
+
+@dataclasses.dataclass
+class CcsdInputs:
+    """Inputs of the step in cc4s layout."""
+    gamma: np.ndarray        # complex128 (NG, Np, Np), Fortran order
+    eps: np.ndarray          # float64 (Np,)
+    No: int
+    field: str               # "real" -> halfGrid 1
+    scale: float             # c
+    fermi_energy: float = FERMI_ENERGY
+
+    @property
+    def NG(self) -> int:
+        return self.gamma.shape[0]
+
+    @property
+    def Np(self) -> int:
+        return self.gamma.shape[1]
+
+    @property
+    def Nv(self) -> int:
+        return self.Np - self.No
+
+    @property
+    def half_grid(self) -> int:
+        return 1 if self.field == "real" else 0
+
+
+def generate(No: int, Nv: int, NG: int, field: str, seed: int,
+             target_max_v: float = TARGET_MAX_V) -> CcsdInputs:
+    """Seeded inputs for one workload (see module docstring)."""
+    if field not in ("real", "complex"):
+        raise ValueError("field must be 'real' or 'complex'")
+    Np = No + Nv
+    gamma = raw_vertex(NG, Np, field, seed)
+    vmax = max_abs_vpphh(gamma, No, field)
+    c = float(np.sqrt(target_max_v / vmax))
+    gamma *= c
+    eps = eigen_energies(No, Nv, seed)
+    return CcsdInputs(gamma=gamma, eps=eps, No=No, field=field, scale=c)
+
+
+def generate_config(cfg: CcsdConfig) -> CcsdInputs:
+    return generate(cfg.No, cfg.Nv, cfg.NG, cfg.field, cfg.seed)
+
+
+def random_amplitudes(No: int, Nv: int, field: str, seed: int,
+                      t1_scale: float = 0.02, t2_scale: float = 0.01):
+    """Seeded amplitudes (T1[a,i], T2[a,b,i,j]) with T2_abij = T2_baji, used
+    to evaluate one residuum at a non-trivial point."""
+    n1 = Nv * No
+    n2 = Nv * Nv * No * No
+    z = normals(seed, STREAM_AMPLITUDES, np.arange(2 * (n1 + n2)))
+    if field == "complex":
+        t1 = (z[0:n1] + 1j * z[n1:2 * n1]) * t1_scale
+        t2 = (z[2 * n1:2 * n1 + n2] + 1j * z[2 * n1 + n2:]) * t2_scale
+    else:
+        t1 = z[0:n1] * t1_scale
+        t2 = z[2 * n1:2 * n1 + n2] * t2_scale
+    T1 = t1.reshape(Nv, No, order="F")
+    T2 = t2.reshape(Nv, Nv, No, No, order="F")
+    T2 = 0.5 * (T2 + T2.transpose(1, 0, 3, 2))
+    return np.asfortranarray(T1), np.asfortranarray(T2)

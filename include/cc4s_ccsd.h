@@ -1,0 +1,171 @@
+/*
+ * C ABI of the MI355X CCSD library: a drop-in for the CoupledCluster step
+ * with method Ccsd and mixer DiisMixer of a cc4s operation file.
+ *
+ * Reference interfaces replaced (paths relative to the cc4s source tree):
+ *   CoupledCluster::run            src/main/algorithms/CoupledCluster.cxx:36-192
+ *   CoupledClusterMethod::getResiduum (Ccsd)
+ *                                  src/main/algorithms/coupledcluster/CoupledClusterMethod.hpp:49-51,
+ *                                  src/main/algorithms/coupledcluster/Ccsd.cxx:76-633
+ *   Mixer::append/get/getResiduumNorm (DiisMixer)
+ *                                  src/main/mixers/Mixer.hpp:49-69, src/main/mixers/DiisMixer.cxx:58-117
+ *   VertexCoulombIntegrals::run    src/main/algorithms/VertexCoulombIntegrals.cxx:37-197
+ *   DefineHolesAndParticles / SliceOperator
+ *                                  src/main/algorithms/DefineHolesAndParticles.cxx:38-96,
+ *                                  src/main/algorithms/SliceOperator.cxx:54-131
+ *
+ * Conventions
+ *   - Tensors cross the boundary in the cc4s layout: dense, first index
+ *     fastest (reference/binary_tensor_file_format.txt:86-95).  Complex values
+ *     are interleaved (re, im) doubles.  Gamma is [NG, Np, Np] (the
+ *     CoulombVertex tensor), T1/R1 are [Nv, No] ("ph"), T2/R2 are
+ *     [Nv, Nv, No, No] ("pphh").  Real-field amplitudes are plain doubles.
+ *   - Every call returns 0 on success and a nonzero status otherwise;
+ *     cc4s_ccsd_last_error() gives the message (the reference throws an
+ *     exception with the same text, Cc4s.cxx:356-389).
+ *   - The caller owns host buffers (copied in and out); the library owns
+ *     device memory.  One HIP stream per context; calls are blocking and not
+ *     re-entrant on one context (the reference's execute() is synchronous).
+ */
+#ifndef CC4S_CCSD_H
+#define CC4S_CCSD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cc4s_ccsd_context cc4s_ccsd_context;
+
+/* Scalar field: the reference runs Ccsd<Real> when the vertex metaData has
+ * halfGrid: 1 and Ccsd<Complex> otherwise (CoupledCluster.cxx:36-52,
+ * VertexCoulombIntegrals.cxx:45-52). */
+enum { CC4S_FIELD_REAL = 0, CC4S_FIELD_COMPLEX = 1 };
+
+/* Arithmetic of the ladder and ring contractions.  FP64 reproduces the
+ * reference; MIXED is reserved for the split 16-bit MFMA mode and is refused
+ * by this build. */
+enum { CC4S_PRECISION_FP64 = 0, CC4S_PRECISION_MIXED = 1 };
+
+/* Step arguments of CoupledCluster (CoupledCluster.cxx:59-117,260-262,
+ * CoupledCluster.hpp:45-49, Ccsd.cxx:95,287-289, DiisMixer.cxx:33). */
+typedef struct {
+  int field;                      /* CC4S_FIELD_* */
+  int64_t No, Nv, NG;             /* holes, particles, auxiliary field */
+  int64_t max_iterations;         /* maxIterations (16) */
+  double energy_convergence;      /* energyConvergence (1e-7) */
+  double amplitudes_convergence;  /* amplitudesConvergence (1e-6) */
+  double level_shift;             /* levelShift (0) */
+  int64_t max_residua;            /* mixer.maxResidua (4), at most 16 */
+  int ppl;                        /* ppl (1) */
+  int64_t integrals_slice_size;   /* integralsSliceSize (0 = No) */
+  int precision;                  /* CC4S_PRECISION_* */
+  double mixed_switch;            /* reserved */
+  int device;                     /* HIP device ordinal */
+} cc4s_ccsd_params;
+
+/* One pass of the iteration loop (CoupledCluster.cxx:129-175). */
+typedef struct {
+  int64_t iteration;       /* 1-based, the reference's "Iter" column */
+  double energy;           /* Re E after mixing */
+  double delta_energy;     /* Re (E - E_previous) */
+  double residuum_norm;    /* dR of the mixer (DiisMixer.cxx:106) */
+  double direct, exchange; /* Re D, Re X */
+  double seconds;          /* wall time of the pass */
+  double flops;            /* algorithmic flops of the pass (8 per complex MAC) */
+  int converged;           /* |dE| < energyConvergence and dR < amplitudesConvergence */
+  int precision_used;      /* CC4S_PRECISION_* */
+} cc4s_ccsd_iteration;
+
+/* Result map of CoupledCluster::run (CoupledCluster.cxx:184-191,245-248). */
+typedef struct {
+  double correlation, direct, exchange; /* energy.{correlation,direct,exchange} */
+  double second_order;                  /* energy.secondOrder (NaN if absent) */
+  double unit;                          /* energy.unit */
+  int convergence_reached;              /* convergenceReached */
+  int64_t iterations;                   /* passes done */
+} cc4s_ccsd_result;
+
+/* Reference defaults: CoupledCluster.hpp:45-49, DiisMixer.cxx:33, Ccsd.cxx:95. */
+void cc4s_ccsd_default_params(cc4s_ccsd_params *params);
+
+/* Creates a context (stream, contraction plans) on params->device. */
+int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **ctx);
+
+/* Inputs of the step: the CoulombVertex Gamma [NG, Np, Np] complex
+ * interleaved, and the eigen energies eps [Np] with holes first.  Replaces the
+ * DefineHolesAndParticles, SliceOperator and VertexCoulombIntegrals steps
+ * (the V slices are formed on the GPU here, once per run). */
+int cc4s_ccsd_set_inputs(cc4s_ccsd_context *ctx, const double *gamma,
+                         const double *eps);
+
+/* Same, with Gamma already resident in device memory (copied device to
+ * device; the caller keeps ownership of gamma_device). */
+int cc4s_ccsd_set_inputs_device(cc4s_ccsd_context *ctx, const void *gamma_device,
+                                const double *eps);
+
+/* initialAmplitudes (CoupledCluster.cxx:70-75); T1 [Nv,No], T2 [Nv,Nv,No,No]. */
+int cc4s_ccsd_set_amplitudes(cc4s_ccsd_context *ctx, const double *T1,
+                             const double *T2);
+
+/* One pass of the loop: residuum, denominator update, DIIS, energy. */
+int cc4s_ccsd_iterate(cc4s_ccsd_context *ctx, cc4s_ccsd_iteration *out);
+
+/* The whole loop until convergence or maxIterations, then the final energy
+ * (CoupledCluster.cxx:129-191).  `trace` receives up to `trace_capacity`
+ * passes (may be NULL). */
+int cc4s_ccsd_run(cc4s_ccsd_context *ctx, cc4s_ccsd_result *out,
+                  cc4s_ccsd_iteration *trace, int64_t trace_capacity);
+
+/* Current amplitudes (the mixer's output) in cc4s layout. */
+int cc4s_ccsd_get_amplitudes(cc4s_ccsd_context *ctx, double *T1, double *T2);
+
+/* Ccsd::getResiduum at the given amplitudes (Ccsd.cxx:76-633); T1 == NULL
+ * and T2 == NULL evaluate the first-iteration residuum (R1 = 0, R2 = V). */
+int cc4s_ccsd_residuum(cc4s_ccsd_context *ctx, const double *T1,
+                       const double *T2, double *R1, double *R2);
+
+/* CoupledCluster::getEnergy of the given amplitudes (CoupledCluster.cxx:
+ * 194-253): out[0..5] = Re E, Im E, Re D, Im D, Re X, Im X. */
+int cc4s_ccsd_energy(cc4s_ccsd_context *ctx, const double *T1,
+                     const double *T2, double *out);
+
+/* Profiling (no counterpart in the reference, whose Timer/OperationsCounter
+ * report only per-step totals, Cc4s.cxx:160-173): when enabled, every
+ * contraction-kernel launch is bracketed by HIP events on the context's
+ * stream, and the particle-particle ladder section is timed as a whole.
+ * Enabling resets the counters. */
+int cc4s_ccsd_set_profiling(cc4s_ccsd_context *ctx, int enabled);
+int cc4s_ccsd_ladder_stats(cc4s_ccsd_context *ctx, double *milliseconds,
+                           double *flops, int64_t *calls);
+/* Launches, summed duration and algorithmic flops of one contraction-kernel
+ * variant ("gett_c64_ak_bk", "gett_f64_am_bn", ...; a = A read along k (k)
+ * or m (m), b = B read along k (k) or n (n)). */
+int cc4s_ccsd_kernel_stats(cc4s_ccsd_context *ctx, const char *variant,
+                           int64_t *launches, double *milliseconds,
+                           double *flops);
+
+/* Message of the last failed call on ctx (ctx == NULL: of the last failed
+ * cc4s_ccsd_create or cc4s_contract). */
+const char *cc4s_ccsd_last_error(const cc4s_ccsd_context *ctx);
+
+void cc4s_ccsd_destroy(cc4s_ccsd_context *ctx);
+
+/* One tensor contraction C[c] = alpha A[a] B[b] + beta C[c] on the GPU with
+ * host buffers, Einstein summation over indices absent from c: the semantics
+ * of one tcc statement (src/main/tcc/Contraction.hpp:160-477), exposed for
+ * testing the contraction kernels.  lens_* give each index's length in the
+ * order of the index string; field is CC4S_FIELD_*. */
+int cc4s_contract(int field, double alpha, const double *A, const char *idx_a,
+                  const int64_t *lens_a, int conj_a, const double *B,
+                  const char *idx_b, const int64_t *lens_b, int conj_b,
+                  double beta, double *C, const char *idx_c,
+                  const int64_t *lens_c, int device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CC4S_CCSD_H */

@@ -1,0 +1,100 @@
+"""Writes the golden fixtures of tests/golden from the FP64 oracle.
+
+config0.json  config-0 workload (real, No=8, Nv=40, NG=200): SHA-256 of the
+              generated inputs, the oracle's per-iteration (E, dE, dR) trace,
+              final energies and checksums of the converged amplitudes.
+small_complex.json  a small complex workload (No=6, Nv=24, NG=60) run the
+              same way, plus the residuum at seeded amplitudes.
+
+Usage: python tests/golden/make_golden.py [config0|small_complex|config1 ...]
+"""
+
+from __future__ import annotations
+
+import hashlib
+import json
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from cc4s_amd import synthetic as syn  # noqa: E402
+from oracle import ccsd_oracle as O  # noqa: E402
+
+HERE = pathlib.Path(__file__).resolve().parent
+
+
+def checksums(T):
+    """Size-independent summaries of an amplitude set."""
+    T1, T2 = T
+    out = {}
+    for name, t in (("T1", T1), ("T2", T2)):
+        f = np.asfortranarray(t).ravel(order="F")
+        w = np.cos(0.37 * np.arange(f.size))
+        out[name] = {
+            "norm": float(np.linalg.norm(f)),
+            "sum_re": float(np.sum(f.real)), "sum_im": float(np.sum(f.imag)),
+            "wsum_re": float(np.sum(w * f.real)),
+            "wsum_im": float(np.sum(w * f.imag)),
+        }
+    return out
+
+
+def run_config(cfg, extra=None):
+    inp = syn.generate_config(cfg)
+    P = O.CcsdProblem(inp.gamma, inp.eps, inp.No, cfg.field)
+    t = time.time()
+    r = O.coupled_cluster(P, max_iterations=cfg.max_iterations,
+                          energy_convergence=cfg.energy_convergence,
+                          amplitudes_convergence=cfg.amplitudes_convergence,
+                          level_shift=cfg.level_shift,
+                          max_residua=cfg.max_residua)
+    dt = time.time() - t
+    g = np.asfortranarray(inp.gamma).tobytes(order="F")
+    out = {
+        "config": cfg.__dict__,
+        "scale": inp.scale,
+        "gamma_sha256": hashlib.sha256(g).hexdigest(),
+        "eps_sha256": hashlib.sha256(inp.eps.tobytes()).hexdigest(),
+        "energy": r["energy"],
+        "convergenceReached": r["convergenceReached"],
+        "iterations": r["iterations"],
+        "trace": r["trace"],
+        "amplitudes": checksums(r["amplitudes"]),
+        "oracle_seconds": dt,
+    }
+    if extra:
+        out.update(extra(P, inp))
+    return out
+
+
+def residuum_fixture(P, inp):
+    T = syn.random_amplitudes(P.No, P.Nv, P.field, seed=77)
+    R1, R2 = P.residuum(T)
+    E, D, X = O.energy(P, T)
+    return {"residuum_at_seeded_amplitudes": {
+        "amplitude_seed": 77,
+        "R1": checksums((R1, R1))["T1"], "R2": checksums((R2, R2))["T2"],
+        "energy": [E.real, E.imag, D.real, D.imag, X.real, X.imag]}}
+
+
+def main(which):
+    if "config0" in which:
+        data = run_config(syn.CONFIGS[0], residuum_fixture)
+        (HERE / "config0.json").write_text(json.dumps(data, indent=1))
+        print("config0", data["energy"], data["iterations"], data["oracle_seconds"])
+    if "small_complex" in which:
+        cfg = syn.CcsdConfig("complex No=6 Nv=24 NG=60", 6, 24, 60, "complex",
+                             seed=42)
+        data = run_config(cfg, residuum_fixture)
+        (HERE / "small_complex.json").write_text(json.dumps(data, indent=1))
+        print("small_complex", data["energy"], data["iterations"],
+              data["oracle_seconds"])
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or ["config0", "small_complex"])

@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every entry point of
+include/cc4s_ccsd.h (no GPU needed: nothing is computed)."""
+
+import ctypes
+import pathlib
+import re
+
+import pytest
+
+from cc4s_amd import api
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    text = (ROOT / "include" / "cc4s_ccsd.h").read_text()
+    return sorted(set(re.findall(r"\b(cc4s_\w+)\s*\(", text)))
+
+
+def test_library_built_and_loadable():
+    assert api.LIB_PATH.exists(), "run python -m cc4s_amd.build"
+    api.load_library()
+
+
+def test_every_declared_symbol_is_exported():
+    lib = ctypes.CDLL(str(api.LIB_PATH))
+    declared = _declared()
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), f"{name} missing from the library"
+    assert sorted(api.EXPORTS) == declared
+
+
+def test_default_params_are_the_reference_defaults():
+    p = api.default_params()
+    # CoupledCluster.hpp:45-49, DiisMixer.cxx:33, Ccsd.cxx:95
+    assert p.max_iterations == 16
+    assert p.energy_convergence == 1e-7
+    assert p.amplitudes_convergence == 1e-6
+    assert p.level_shift == 0.0
+    assert p.max_residua == 4
+    assert p.ppl == 1
+    assert p.integrals_slice_size == 0
+    assert p.field == api.FIELD_COMPLEX
+
+
+def test_unsupported_precision_is_refused_before_touching_a_device():
+    with pytest.raises(api.CcsdError, match="precision"):
+        api.CoupledCluster(2, 4, 3, precision=api.PRECISION_MIXED)
