@@ -216,7 +216,7 @@ void Ccsd::residuum_complex(const void *Tp, void *Rp) {
   const hipStream_t st = E.stream();
   const View R1 = dense_view(Rp, f, "ai", {v, o});
   const View R2 = dense_view(offset_ptr(Rp, s_.n1(), f), f, "abij", {v, v, o, o});
-  const View &Vpphh = w.Vpphh.v;
+  const View Vpphh = w.Vpphh("abij");
   if (!Tp) {                                                      // :404-409
     E.scale(0.0, R1);
     E.sum(1.0, Vpphh, 0.0, R2);
@@ -312,7 +312,7 @@ void Ccsd::residuum_real(const void *Tp, void *Rp) {
   const hipStream_t st = E.stream();
   const View R1 = dense_view(Rp, f, "ai", {v, o});
   const View R2 = dense_view(offset_ptr(Rp, s_.n1(), f), f, "abij", {v, v, o, o});
-  const View &Vpphh = w.Vpphh.v;
+  const View Vpphh = w.Vpphh("abij");
   if (!Tp) {                                                      // :97-102
     E.scale(0.0, R1);
     E.sum(1.0, Vpphh, 0.0, R2);

@@ -234,8 +234,8 @@ __global__ __launch_bounds__(kThreads, 2) void gett_kernel(KArgs p) {
     }
   }
 
-  // epilogue: lane l, register r of a 16x16 block holds row 4*(l/16)+r,
-  // column l%16
+  // epilogue: lane l, register r of a 16x16 f64 MFMA block holds row
+  // 4*r + l/16, column l%16
   int64_t cCol[NB];
   bool cColOk[NB];
 #pragma unroll
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kThreads, 2) void gett_kernel(KArgs p) {
   for (int i = 0; i < MB; ++i) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int64_t m = m0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+      const int64_t m = m0 + wm * WM + i * 16 + 4 * r + (lane >> 4);
       if (m >= p.M) continue;
       const int64_t rc = p.W ? 0 : p.rowC[m];
 #pragma unroll
