@@ -189,4 +189,4 @@ def test_config1_properties():
     Dpphh = dph[:, None, :, None] + dph[None, :, None, :]
     Tm = np.conj(V) / Dpphh
     mp2 = (-2.0 * np.sum(V * Tm) + np.sum(V.transpose(0, 1, 3, 2) * Tm)).real
-    assert abs(it["energy"] - mp2) < 1e-12
+    assert abs(it["energy"] - mp2) < 1e-13 * abs(mp2)
