@@ -287,6 +287,230 @@ __global__ __launch_bounds__(kThreads, 2) void gett_kernel(KArgs p) {
   }
 }
 
+// Complex GETT, second generation: 8 waves (2 x 4) on a BM x BN complex tile,
+// one workgroup per CU.  LDS holds two K-stages (double buffer) with (re, im)
+// interleaved, so an MFMA operand pair is one 128-bit LDS read and a K-stage
+// needs one barrier.  Global loads for stage s+2 are issued while stage s is
+// multiplied; their K offsets were fetched a stage earlier still.
+constexpr int kZThreads = 512;
+
+template <int BM, int BN, int BK, int WM, int WN, bool AKC, bool BKC>
+__global__ __launch_bounds__(kZThreads, 1) void zgett_kernel(KArgs p) {
+  constexpr int NT = kZThreads;
+  constexpr int MB = WM / 16, NB = WN / 16;
+  constexpr int WAVES_N = BN / WN;
+  static_assert((BM / WM) * (BN / WN) * 64 == NT, "eight waves per workgroup");
+  static_assert(BK % 4 == 0, "BK multiple of the MFMA k");
+  constexpr int EA = BM * BK / NT;
+  constexpr int EB = BN * BK / NT;
+  static_assert(EA >= 1 && EB >= 1, "tile too small");
+  static_assert(AKC ? (NT % BK == 0) : (NT % BM == 0), "A map");
+  static_assert(BKC ? (NT % BK == 0) : (NT % BN == 0), "B map");
+  constexpr int NKA = AKC ? 1 : EA;
+  constexpr int NRA = AKC ? EA : 1;
+  constexpr int NKB = BKC ? 1 : EB;
+  constexpr int NRB = BKC ? EB : 1;
+  constexpr int LDA = BM + 8, LDB = BN + 8;  // complex elements per LDS row
+
+  __shared__ double2 As[2][BK][LDA];
+  __shared__ double2 Bs[2][BK][LDB];
+
+  const int t = threadIdx.x;
+  const int ntiles = p.tilesM * p.tilesN;
+  int g = blockIdx.x;
+  if ((ntiles & 7) == 0) g = (g & 7) * (ntiles >> 3) + (g >> 3);
+  constexpr int GM = 8;
+  const int per_group = GM * p.tilesN;
+  const int first_m = (g / per_group) * GM;
+  const int gsize = min(p.tilesM - first_m, GM);
+  const int tm = first_m + (g % per_group) % gsize;
+  const int tn = (g % per_group) / gsize;
+  const int64_t m0 = static_cast<int64_t>(tm) * BM;
+  const int64_t n0 = static_cast<int64_t>(tn) * BN;
+  const int64_t kbeg = static_cast<int64_t>(blockIdx.y) * p.kChunk;
+  const int64_t kend = min(p.K, kbeg + p.kChunk);
+  const int nstages = kend > kbeg ? static_cast<int>((kend - kbeg + BK - 1) / BK) : 0;
+
+  auto a_m = [&](int i) { return AKC ? (t / BK + i * (NT / BK)) : (t % BM); };
+  auto a_k = [&](int i) { return AKC ? (t % BK) : (t / BM + i * (NT / BM)); };
+  auto b_n = [&](int i) { return BKC ? (t / BK + i * (NT / BK)) : (t % BN); };
+  auto b_k = [&](int i) { return BKC ? (t % BK) : (t / BN + i * (NT / BN)); };
+
+  int64_t aRow[NRA];
+  bool aRowOk[NRA];
+#pragma unroll
+  for (int i = 0; i < NRA; ++i) {
+    const int64_t m = m0 + a_m(i);
+    aRowOk[i] = m < p.M;
+    aRow[i] = aRowOk[i] ? p.rowA[m] : 0;
+  }
+  int64_t bCol[NRB];
+  bool bColOk[NRB];
+#pragma unroll
+  for (int i = 0; i < NRB; ++i) {
+    const int64_t n = n0 + b_n(i);
+    bColOk[i] = n < p.N;
+    bCol[i] = bColOk[i] ? p.colB[n] : 0;
+  }
+
+  int64_t aK[NKA], aKn[NKA], bK[NKB], bKn[NKB];
+  bool aKok[NKA], aKokn[NKA], bKok[NKB], bKokn[NKB];
+  auto load_koff = [&](int s, int64_t *ak, bool *akok, int64_t *bk, bool *bkok) {
+    const int64_t kb = kbeg + static_cast<int64_t>(s) * BK;
+#pragma unroll
+    for (int i = 0; i < NKA; ++i) {
+      const int64_t k = kb + a_k(i);
+      akok[i] = k < kend;
+      ak[i] = akok[i] ? p.colA[k] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < NKB; ++i) {
+      const int64_t k = kb + b_k(i);
+      bkok[i] = k < kend;
+      bk[i] = bkok[i] ? p.rowB[k] : 0;
+    }
+  };
+  auto advance_koff = [&]() {
+#pragma unroll
+    for (int i = 0; i < NKA; ++i) { aK[i] = aKn[i]; aKok[i] = aKokn[i]; }
+#pragma unroll
+    for (int i = 0; i < NKB; ++i) { bK[i] = bKn[i]; bKok[i] = bKokn[i]; }
+  };
+
+  double2 ra[EA], rb[EB];
+  auto load_data = [&]() {
+    const double2 *A = reinterpret_cast<const double2 *>(p.A);
+    const double2 *B = reinterpret_cast<const double2 *>(p.B);
+#pragma unroll
+    for (int i = 0; i < EA; ++i) {
+      const int ir = AKC ? i : 0, ik = AKC ? 0 : i;
+      const bool ok = aRowOk[ir] && aKok[ik];
+      ra[i] = ok ? A[aRow[ir] + aK[ik]] : make_double2(0.0, 0.0);
+      ra[i].y *= p.signA;
+    }
+#pragma unroll
+    for (int i = 0; i < EB; ++i) {
+      const int ir = BKC ? i : 0, ik = BKC ? 0 : i;
+      const bool ok = bColOk[ir] && bKok[ik];
+      rb[i] = ok ? B[bCol[ir] + bK[ik]] : make_double2(0.0, 0.0);
+      rb[i].y *= p.signB;
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < EA; ++i) As[buf][a_k(i)][a_m(i)] = ra[i];
+#pragma unroll
+    for (int i = 0; i < EB; ++i) Bs[buf][b_k(i)][b_n(i)] = rb[i];
+  };
+
+  const int lane = t & 63;
+  const int w = t >> 6;
+  const int wm = w / WAVES_N, wn = w % WAVES_N;
+  dbl4 accR[MB][NB], accI[MB][NB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      accR[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+      accI[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  // prologue: stage 0 in LDS buffer 0, stage 1 in registers
+  if (nstages > 0) {
+    load_koff(0, aK, aKok, bK, bKok);
+    if (nstages > 1) load_koff(1, aKn, aKokn, bKn, bKokn);
+    load_data();
+    store_lds(0);
+    if (nstages > 1) {
+      advance_koff();
+      if (nstages > 2) load_koff(2, aKn, aKokn, bKn, bKokn);
+      load_data();
+    }
+    __syncthreads();
+  }
+  for (int s = 0; s < nstages; ++s) {
+    const int buf = s & 1;
+#pragma unroll
+    for (int kq = 0; kq < BK / 4; ++kq) {
+      const int kr = kq * 4 + (lane >> 4);
+      double2 a[MB], b[NB];
+#pragma unroll
+      for (int i = 0; i < MB; ++i) a[i] = As[buf][kr][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) b[j] = Bs[buf][kr][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, accR[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].y, accI[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < MB; ++i) {
+        const double nai = -a[i].y;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[j].y, accR[i][j], 0, 0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].x, accI[i][j], 0, 0, 0);
+    }
+    if (s + 1 < nstages) {
+      store_lds(buf ^ 1);  // stage s+1, loaded during stage s-1
+      if (s + 2 < nstages) {
+        advance_koff();
+        if (s + 3 < nstages) load_koff(s + 3, aKn, aKokn, bKn, bKokn);
+        load_data();
+      }
+    }
+    __syncthreads();
+  }
+
+  int64_t cCol[NB];
+  bool cColOk[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int64_t n = n0 + wn * WN + j * 16 + (lane & 15);
+    cColOk[j] = n < p.N;
+    cCol[j] = (cColOk[j] && !p.W) ? p.colC[n] : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < MB; ++i) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t m = m0 + wm * WM + i * 16 + 4 * r + (lane >> 4);
+      if (m >= p.M) continue;
+      const int64_t rc = p.W ? 0 : p.rowC[m];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (!cColOk[j]) continue;
+        const int64_t n = n0 + wn * WN + j * 16 + (lane & 15);
+        const double vr = accR[i][j][r];
+        const double vi = accI[i][j][r];
+        if (p.W) {
+          const int64_t o = (static_cast<int64_t>(blockIdx.y) * p.M + m) * p.N + n;
+          reinterpret_cast<double2 *>(p.W)[o] = make_double2(vr, vi);
+        } else {
+          double2 *c = reinterpret_cast<double2 *>(p.C) + rc + cCol[j];
+          double2 out = make_double2(p.alpha * vr, p.alpha * vi);
+          if (p.beta != 0.0) {
+            const double2 old = *c;
+            out.x += p.beta * old.x;
+            out.y += p.beta * old.y;
+          }
+          *c = out;
+        }
+      }
+    }
+  }
+}
+
 // C[rowC[m] + colC[n]] = alpha * sum_s W[s][m][n] + beta * C, fixed order
 template <bool CPLX>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(
@@ -379,15 +603,43 @@ void dispatch_gett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_
     gett_kernel<CPLX, BM, BN, BK, WM, WN, false, false><<<grid, kThreads, 0, s>>>(args);
 }
 
-// tile shapes (complex: 128x64 of complex, real: 128x128)
+template <int BM, int BN, int BK, int WM, int WN>
+void dispatch_zgett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_t s) {
+  if (L.a_kcontig && L.b_kcontig)
+    zgett_kernel<BM, BN, BK, WM, WN, true, true><<<grid, kZThreads, 0, s>>>(args);
+  else if (L.a_kcontig)
+    zgett_kernel<BM, BN, BK, WM, WN, true, false><<<grid, kZThreads, 0, s>>>(args);
+  else if (L.b_kcontig)
+    zgett_kernel<BM, BN, BK, WM, WN, false, true><<<grid, kZThreads, 0, s>>>(args);
+  else
+    zgett_kernel<BM, BN, BK, WM, WN, false, false><<<grid, kZThreads, 0, s>>>(args);
+}
+
+// tile shapes: complex first generation 128x64, second generation 128x128
+// (complex elements); real 128x128
 constexpr int kCBM = 128, kCBN = 64, kCBK = 16, kCWM = 64, kCWN = 32;
+constexpr int kZBM = 128, kZBN = 128, kZBK = 16, kZWM = 64, kZWN = 32;
 constexpr int kRBM = 128, kRBN = 128, kRBK = 16, kRWM = 64, kRWN = 64;
+
+int g_complex_generation = 2;
+
+void tile_of(Field f, int &BM, int &BN, int &BK) {
+  if (f == Field::Real) {
+    BM = kRBM; BN = kRBN; BK = kRBK;
+  } else if (g_complex_generation == 1) {
+    BM = kCBM; BN = kCBN; BK = kCBK;
+  } else {
+    BM = kZBM; BN = kZBN; BK = kZBK;
+  }
+}
 
 inline int grid_for(int64_t n) {
   return static_cast<int>(std::min<int64_t>((n + 255) / 256, 256 * 64));
 }
 
 }  // namespace
+
+void set_complex_gett_generation(int g) { g_complex_generation = g == 1 ? 1 : 2; }
 
 std::string gett_variant(const GettLaunch &L) {
   std::string n = L.field == Field::Complex ? "gett_c64" : "gett_f64";
@@ -399,8 +651,8 @@ std::string gett_variant(const GettLaunch &L) {
 void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start,
                  hipEvent_t stop) {
   const bool cplx = L.field == Field::Complex;
-  const int BM = cplx ? kCBM : kRBM, BN = cplx ? kCBN : kRBN;
-  const int BK = cplx ? kCBK : kRBK;
+  int BM, BN, BK;
+  tile_of(L.field, BM, BN, BK);
   KArgs a;
   a.A = static_cast<const double *>(L.A);
   a.B = static_cast<const double *>(L.B);
@@ -420,7 +672,9 @@ void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start,
               "contraction too large for one launch");
   dim3 grid(a.tilesM * a.tilesN, L.splits);
   if (start) CC4S_HIP(hipEventRecord(start, s));
-  if (cplx)
+  if (cplx && g_complex_generation == 2)
+    dispatch_zgett<kZBM, kZBN, kZBK, kZWM, kZWN>(L, a, grid, s);
+  else if (cplx)
     dispatch_gett<true, kCBM, kCBN, kCBK, kCWM, kCWN>(L, a, grid, s);
   else
     dispatch_gett<false, kRBM, kRBN, kRBK, kRWM, kRWN>(L, a, grid, s);
@@ -594,14 +848,15 @@ GettPlan &ContractionEngine::plan_for(const View &A0, const View &B0,
   plan->rowC = base; base += rowC.size();
   plan->colC = base;
 
-  // split K when the tile grid cannot fill 256 CUs x 2 workgroups
-  const bool cplx = C.field == Field::Complex;
-  const int BM = cplx ? kCBM : kRBM, BN = cplx ? kCBN : kRBN, BK = cplx ? kCBK : kRBK;
+  // split K when the tile grid cannot fill the 256 CUs twice over
+  int BM, BN, BK;
+  tile_of(C.field, BM, BN, BK);
+  const int slots = 256 * ((C.field == Field::Complex && g_complex_generation == 2) ? 1 : 2);
   const int64_t tiles = ((plan->M + BM - 1) / BM) * ((plan->N + BN - 1) / BN);
   const int64_t kstages = (plan->K + BK - 1) / BK;
   int64_t splits = 1;
-  if (tiles < 512) {
-    splits = std::min<int64_t>((1024 + tiles - 1) / tiles, kstages / 4);
+  if (tiles < 2 * slots) {
+    splits = std::min<int64_t>((4 * slots + tiles - 1) / tiles, kstages / 4);
     const size_t es = scalar_bytes(C.field);
     while (splits > 1 &&
            static_cast<size_t>(splits) * plan->M * plan->N * es > max_workspace)

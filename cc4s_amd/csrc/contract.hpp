@@ -93,5 +93,9 @@ void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start = nullptr,
                  hipEvent_t stop = nullptr);
 // name of the kernel variant launch_gett uses for L
 std::string gett_variant(const GettLaunch &L);
+// selects the complex kernel generation (1: 4 waves, 128x64 tile, single
+// LDS buffer; 2: 8 waves, 128x128 tile, double-buffered LDS); plans created
+// afterwards use its tile shape
+void set_complex_gett_generation(int generation);
 
 }  // namespace cc4s_amd
