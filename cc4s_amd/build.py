@@ -17,6 +17,7 @@ ROOT = HERE.parent
 CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libcc4s_ccsd.so"
+BENCH = HERE / "gett_bench"
 SOURCES = ["contract.hip", "elementwise.hip", "ccsd.hip", "coupled_cluster.hip",
            "abi.hip"]
 
@@ -57,6 +58,15 @@ def build(jobs: int = 8, verbose: bool = True) -> pathlib.Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    # contraction-kernel benchmark, linked against the library
+    cmd = [HIPCC, *COMMON, "-x", "hip", str(CSRC / "gett_bench.cpp"),
+           "-o", str(BENCH), f"-L{HERE}", "-lcc4s_ccsd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"benchmark build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {BENCH}")
     return LIB
 
 

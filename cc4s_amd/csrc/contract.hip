@@ -310,7 +310,11 @@ __global__ __launch_bounds__(kZThreads, 1) void zgett_kernel(KArgs p) {
   constexpr int NRA = AKC ? EA : 1;
   constexpr int NKB = BKC ? 1 : EB;
   constexpr int NRB = BKC ? EB : 1;
-  constexpr int LDA = BM + 8, LDB = BN + 8;  // complex elements per LDS row
+  // complex elements per LDS row: BM+1 shifts consecutive k-rows by one
+  // 16-byte bank group, so the column-wise stores of K-contiguous operands
+  // and the row-wise 128-bit operand reads are both conflict-free per
+  // 16-lane pass
+  constexpr int LDA = BM + 1, LDB = BN + 1;
 
   __shared__ double2 As[2][BK][LDA];
   __shared__ double2 Bs[2][BK][LDB];
