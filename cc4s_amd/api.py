@@ -115,7 +115,7 @@ def load_library(path: pathlib.Path | str | None = None):
     i64p = C.POINTER(C.c_int64)
     lib.cc4s_contract.argtypes = [C.c_int, C.c_double, vp, C.c_char_p, i64p, C.c_int,
                                   vp, C.c_char_p, i64p, C.c_int, C.c_double, vp,
-                                  C.c_char_p, i64p, C.c_int]
+                                  C.c_char_p, i64p, C.c_int, C.c_int]
     for name in ("cc4s_ccsd_create", "cc4s_ccsd_set_inputs",
                  "cc4s_ccsd_set_inputs_device", "cc4s_ccsd_set_amplitudes",
                  "cc4s_ccsd_iterate", "cc4s_ccsd_run", "cc4s_ccsd_get_amplitudes",
@@ -149,7 +149,8 @@ class CoupledCluster:
     def __init__(self, No, Nv, NG, field="complex", maxIterations=16,
                  energyConvergence=1e-7, amplitudesConvergence=1e-6,
                  levelShift=0.0, maxResidua=4, ppl=True,
-                 integralsSliceSize=0, device=0, precision=PRECISION_FP64):
+                 integralsSliceSize=0, device=0, precision=PRECISION_FP64,
+                 mixedSwitch=1e-5):
         self.lib = load_library()
         p = default_params()
         p.field = FIELD_COMPLEX if field == "complex" else FIELD_REAL
@@ -163,6 +164,7 @@ class CoupledCluster:
         p.integrals_slice_size = integralsSliceSize
         p.device = device
         p.precision = precision
+        p.mixed_switch = mixedSwitch
         self.params = p
         self.field = field
         self.No, self.Nv, self.NG = No, Nv, NG
@@ -272,7 +274,7 @@ class CoupledCluster:
 
 
 def contract(alpha, A, idx_a, B, idx_b, beta, C_, idx_c, conj_a=False,
-             conj_b=False, device=0):
+             conj_b=False, device=0, precision=PRECISION_FP64):
     """GPU contraction with host arrays (indexed in index-string order)."""
     lib = load_library()
     cplx = any(np.iscomplexobj(x) for x in (A, B, C_))
@@ -285,7 +287,7 @@ def contract(alpha, A, idx_a, B, idx_b, beta, C_, idx_c, conj_a=False,
     rc = lib.cc4s_contract(FIELD_COMPLEX if cplx else FIELD_REAL, alpha,
                            _ptr(a), idx_a.encode(), la, int(conj_a),
                            _ptr(b), idx_b.encode(), lb, int(conj_b),
-                           beta, _ptr(c), idx_c.encode(), lc, device)
+                           beta, _ptr(c), idx_c.encode(), lc, precision, device)
     if rc != 0:
         raise CcsdError(lib.cc4s_ccsd_last_error(None).decode())
     return c

@@ -18,8 +18,8 @@ CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libcc4s_ccsd.so"
 BENCH = HERE / "gett_bench"
-SOURCES = ["contract.hip", "elementwise.hip", "ccsd.hip", "coupled_cluster.hip",
-           "abi.hip"]
+SOURCES = ["contract.hip", "mixed.hip", "elementwise.hip", "ccsd.hip",
+           "coupled_cluster.hip", "abi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CC4S_OFFLOAD_ARCH", "gfx950")

@@ -44,7 +44,7 @@ void fill(cc4s_ccsd_iteration *out, const IterationReport &r) {
   out->seconds = r.seconds;
   out->flops = r.flops;
   out->converged = r.converged ? 1 : 0;
-  out->precision_used = CC4S_PRECISION_FP64;
+  out->precision_used = r.mixed ? CC4S_PRECISION_MIXED : CC4S_PRECISION_FP64;
 }
 
 }  // namespace
@@ -73,9 +73,15 @@ int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
     *out = nullptr;
     CC4S_ASSERT(params->field == CC4S_FIELD_REAL || params->field == CC4S_FIELD_COMPLEX,
                 "unsupported tensor type as 'operator'");
-    CC4S_ASSERT(params->precision == CC4S_PRECISION_FP64,
-                "precision mode not available in this build");
+    CC4S_ASSERT(params->precision == CC4S_PRECISION_FP64 ||
+                    params->precision == CC4S_PRECISION_MIXED,
+                "unknown precision mode");
+    CC4S_ASSERT(params->precision == CC4S_PRECISION_FP64 ||
+                    params->field == CC4S_FIELD_COMPLEX,
+                "mixed precision needs the complex field");
     CoupledClusterParams p;
+    p.mixed = params->precision == CC4S_PRECISION_MIXED;
+    p.mixed_switch = params->mixed_switch;
     p.shape.field = params->field == CC4S_FIELD_REAL ? Field::Real : Field::Complex;
     p.shape.No = params->No;
     p.shape.Nv = params->Nv;
@@ -226,7 +232,7 @@ int cc4s_contract(int field, double alpha, const double *A, const char *idx_a,
                   const int64_t *lens_a, int conj_a, const double *B,
                   const char *idx_b, const int64_t *lens_b, int conj_b,
                   double beta, double *C, const char *idx_c,
-                  const int64_t *lens_c, int device) {
+                  const int64_t *lens_c, int precision, int device) {
   return guarded(nullptr, [&] {
     CC4S_ASSERT(A && B && C && idx_a && idx_b && idx_c, "null argument");
     const Field f = field == CC4S_FIELD_REAL ? Field::Real : Field::Complex;
@@ -249,6 +255,11 @@ int cc4s_contract(int field, double alpha, const double *A, const char *idx_a,
       View a = tA.v, b = tB.v;
       a.conj = conj_a != 0;
       b.conj = conj_b != 0;
+      CC4S_ASSERT(precision == CC4S_PRECISION_FP64 ||
+                      (precision == CC4S_PRECISION_MIXED && f == Field::Complex),
+                  "mixed precision needs the complex field");
+      E.set_mixed(precision == CC4S_PRECISION_MIXED);
+      E.mixed_min_flops = 0.0;
       E.contract(alpha, a, b, beta, tC.v);
       CC4S_HIP(hipStreamSynchronize(s));
     }

@@ -684,16 +684,23 @@ void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start,
     dispatch_gett<false, kRBM, kRBN, kRBK, kRWM, kRWN>(L, a, grid, s);
   CC4S_HIP(hipGetLastError());
   if (stop) CC4S_HIP(hipEventRecord(stop, s));
-  if (L.splits > 1) {
-    const int64_t total = L.M * L.N;
-    if (cplx)
-      splitk_reduce_kernel<true><<<grid_for(total), 256, 0, s>>>(
-          a.W, a.C, L.rowC, L.colC, L.M, L.N, L.splits, L.alpha, L.beta);
-    else
-      splitk_reduce_kernel<false><<<grid_for(total), 256, 0, s>>>(
-          a.W, a.C, L.rowC, L.colC, L.M, L.N, L.splits, L.alpha, L.beta);
-    CC4S_HIP(hipGetLastError());
-  }
+  if (L.splits > 1)
+    launch_splitk_reduce(L.field, a.W, a.C, L.rowC, L.colC, L.M, L.N, L.splits,
+                         L.alpha, L.beta, s);
+}
+
+void launch_splitk_reduce(Field f, const double *W, double *C,
+                          const int64_t *rowC, const int64_t *colC, int64_t M,
+                          int64_t N, int splits, double alpha, double beta,
+                          hipStream_t s) {
+  const int64_t total = M * N;
+  if (f == Field::Complex)
+    splitk_reduce_kernel<true><<<grid_for(total), 256, 0, s>>>(
+        W, C, rowC, colC, M, N, splits, alpha, beta);
+  else
+    splitk_reduce_kernel<false><<<grid_for(total), 256, 0, s>>>(
+        W, C, rowC, colC, M, N, splits, alpha, beta);
+  CC4S_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -901,6 +908,12 @@ void ContractionEngine::contract(double alpha, const View &A, const View &B,
   }
   const double flops = (C.field == Field::Complex ? 8.0 : 2.0) * double(P.M) *
                        double(P.N) * double(P.K);
+  if (mixed_ && C.field == Field::Complex && flops >= mixed_min_flops) {
+    contract_mixed(P, alpha, A, B, beta, C, flops);
+    stats.flops += flops;
+    stats.launches += 1;
+    return;
+  }
   if (profiling_) {
     auto take = [this]() {
       hipEvent_t e;
@@ -921,6 +934,61 @@ void ContractionEngine::contract(double alpha, const View &A, const View &B,
   }
   stats.flops += flops;
   stats.launches += 1;
+}
+
+void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
+                                       const View &B, double beta, const View &C,
+                                       double flops) {
+  const MixedTile T = mixed_tile();
+  const int64_t Mp = (P.M + T.BM - 1) / T.BM * T.BM;
+  const int64_t Np = (P.N + T.BN - 1) / T.BN * T.BN;
+  const int64_t Kfull = (P.K + T.BK - 1) / T.BK * T.BK;
+  int64_t Kc = static_cast<int64_t>(pack_budget / (static_cast<size_t>(Mp + Np) * 8));
+  Kc = std::max<int64_t>(T.BK, Kc / T.BK * T.BK);
+  Kc = std::min(Kc, Kfull);
+  const int64_t chunks = (P.K + Kc - 1) / Kc;
+  // split K inside a chunk when the tile grid cannot fill the CUs twice
+  const int64_t tiles = (Mp / T.BM) * (Np / T.BN);
+  int splits = 1;
+  if (tiles < 512) {
+    int64_t sp = std::min<int64_t>((1024 + tiles - 1) / tiles, Kc / (4 * T.BK));
+    while (sp > 1 && static_cast<size_t>(sp) * P.M * P.N * 16 > max_workspace) --sp;
+    splits = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(sp, 256)));
+  }
+  pack_.reserve(static_cast<size_t>(Mp + Np) * Kc * 8);
+  char *Apk = pack_.as<char>();
+  char *Bpk = Apk + static_cast<size_t>(Mp) * Kc * 8;
+  double *W = nullptr;
+  if (splits > 1) {
+    workspace_.reserve(static_cast<size_t>(splits) * P.M * P.N * 16);
+    W = workspace_.as<double>();
+  }
+  const View &Aop = P.swap ? B : A;
+  const View &Bop = P.swap ? A : B;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (profiling_) {
+    CC4S_HIP(hipEventCreate(&e0));
+    CC4S_HIP(hipEventCreate(&e1));
+    CC4S_HIP(hipEventRecord(e0, stream_));
+  }
+  for (int64_t c = 0; c < chunks; ++c) {
+    const int64_t k0 = c * Kc;
+    const int64_t kc = std::min(Kc, (P.K - k0 + T.BK - 1) / T.BK * T.BK);
+    launch_pack(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K, k0,
+                kc, Mp, Aop.conj, P.a_kc, Apk, stream_);
+    launch_pack(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K, k0,
+                kc, Np, Bop.conj, P.b_kc, Bpk, stream_);
+    const double bc = c == 0 ? beta : 1.0;
+    launch_bgemm(Apk, Bpk, static_cast<double *>(C.ptr), W, P.rowC, P.colC, P.M,
+                 P.N, Mp, Np, kc, splits, alpha, bc, stream_);
+    if (splits > 1)
+      launch_splitk_reduce(Field::Complex, W, static_cast<double *>(C.ptr), P.rowC,
+                           P.colC, P.M, P.N, splits, alpha, bc, stream_);
+  }
+  if (profiling_) {
+    CC4S_HIP(hipEventRecord(e1, stream_));
+    pending_.push_back(Pending{e0, e1, "mixed_c64", flops});
+  }
 }
 
 void ContractionEngine::sum(double alpha, const View &A, double beta,

@@ -59,8 +59,20 @@ class ContractionEngine {
   // upper bound for split-K workspace (bytes)
   size_t max_workspace = size_t(1) << 30;
 
+  // Mixed precision: complex contractions of at least `mixed_min_flops`
+  // algorithmic flops run on the split-bf16 MFMA kernel (FP32 accumulation
+  // per K-chunk, FP64 result); everything else stays FP64.
+  void set_mixed(bool on) { mixed_ = on; }
+  bool mixed() const { return mixed_; }
+  double mixed_min_flops = 1e9;
+  size_t pack_budget = size_t(1) << 30;  // bytes of packed operands per chunk
+
  private:
   GettPlan &plan_for(const View &A, const View &B, const View &C);
+  void contract_mixed(GettPlan &P, double alpha, const View &A, const View &B,
+                      double beta, const View &C, double flops);
+  bool mixed_ = false;
+  DeviceBuffer pack_;
   struct Pending {
     hipEvent_t start, stop;
     std::string name;
@@ -97,5 +109,28 @@ std::string gett_variant(const GettLaunch &L);
 // LDS buffer; 2: 8 waves, 128x128 tile, double-buffered LDS); plans created
 // afterwards use its tile shape
 void set_complex_gett_generation(int generation);
+
+// split-K reduction shared by both kernel families:
+// C[rowC[m] + colC[n]] = alpha * sum_s W[s][m][n] + beta * C
+void launch_splitk_reduce(Field f, const double *W, double *C,
+                          const int64_t *rowC, const int64_t *colC, int64_t M,
+                          int64_t N, int splits, double alpha, double beta,
+                          hipStream_t s);
+
+// split-bf16 path (mixed.hip)
+struct MixedTile {
+  int BM, BN, BK;
+};
+MixedTile mixed_tile();
+// packs k in [k0, k0+Kc) of an operand (FP64 complex, addressed by
+// offMN[mn] + offK[k]) into out[Kc][MNp] split-bf16 words, zero-padded
+void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
+                 int64_t MN, int64_t K, int64_t k0, int64_t Kc, int64_t MNp,
+                 bool conj, bool kcontig, void *out, hipStream_t s);
+// C (or split-K partials W) from packed operands [Kc][Mp] and [Kc][Np]
+void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
+                  const int64_t *rowC, const int64_t *colC, int64_t M, int64_t N,
+                  int64_t Mp, int64_t Np, int64_t Kc, int splits, double alpha,
+                  double beta, hipStream_t s);
 
 }  // namespace cc4s_amd

@@ -116,6 +116,9 @@ CoupledCluster::CoupledCluster(const CoupledClusterParams &p, int device)
   R_.allocate(s.nset() * es);
   energy_partials_.allocate(energy_partials_size());
   mixer_.reset(new DiisMixer(s.field, s.nset(), p_.max_residua, stream_));
+  CC4S_ASSERT(!p_.mixed || s.field == Field::Complex,
+              "mixed precision needs the complex field");
+  mixed_phase_ = p_.mixed;
 }
 
 CoupledCluster::~CoupledCluster() {
@@ -196,7 +199,10 @@ IterationReport CoupledCluster::iterate() {
   IterationReport rep;
   rep.iteration = iterations_done + 1;
   const void *T = have_amplitudes_ ? T_.get() : nullptr;
+  rep.mixed = mixed_phase_;
+  engine_->set_mixed(mixed_phase_);
   ccsd_->get_residuum(T, R_.get());                     // :136
+  engine_->set_mixed(false);
   // :137-145: T' = -R/(Delta+s) and dT = T' - T, written into the mixer slot
   amplitude_update(stream_, s.field, R_.get(), T,
                    static_cast<const double *>(eps_h_.get()),
@@ -215,6 +221,12 @@ IterationReport CoupledCluster::iterate() {
   if (is_second_order) second_order = rep.energy.real();      // :164-166
   rep.converged = std::abs(rep.energy - previous_e_) < p_.energy_convergence &&
                   rep.residuum_norm < p_.amplitudes_convergence;  // :167-173
+  // the convergence test only counts FP64 passes; the mixed phase ends once
+  // the extrapolated amplitude change falls below the switch threshold
+  if (rep.mixed) rep.converged = false;
+  if (mixed_phase_ && have_amplitudes_ && !is_second_order &&
+      rep.residuum_norm < p_.mixed_switch)
+    mixed_phase_ = false;
   previous_e_ = rep.energy;
   iterations_done += 1;
   return rep;
@@ -249,7 +261,10 @@ void CoupledCluster::residuum(const double *T1, const double *T2, double *R1,
     upload_set(scratch_.get(), T1, T2);
     T = scratch_.get();
   }
+  // a context created for mixed precision evaluates its residuum that way
+  engine_->set_mixed(p_.mixed);
   ccsd_->get_residuum(T, R_.get());
+  engine_->set_mixed(false);
   download_set(R_.get(), R1, R2);
 }
 

@@ -19,6 +19,10 @@ struct CoupledClusterParams {
   double level_shift = 0.0;              // CoupledCluster.hpp:49
   int64_t max_residua = 4;               // DiisMixer.cxx:33
   CcsdOptions ccsd;
+  // mixed precision: large contractions on split-bf16 MFMA while the
+  // mixer's dR >= mixed_switch, FP64 afterwards
+  bool mixed = false;
+  double mixed_switch = 1e-5;
 };
 
 // DIIS over (amplitudes, amplitude change) pairs held in HBM as ring-buffer
@@ -55,6 +59,7 @@ struct IterationReport {
   cplx energy, direct, exchange;
   double delta_energy = 0, residuum_norm = 0, seconds = 0, flops = 0;
   bool converged = false;
+  bool mixed = false;  // pass ran its large contractions in split bf16
 };
 
 class CoupledCluster {
@@ -93,6 +98,7 @@ class CoupledCluster {
   DeviceBuffer T_, R_, scratch_, energy_partials_;
   bool have_amplitudes_ = false;
   bool inputs_set_ = false;
+  bool mixed_phase_ = false;  // still iterating in mixed precision
   cplx previous_e_ = 0.0;
 };
 

@@ -1,6 +1,7 @@
 // Contraction-kernel benchmark on the CCSD statement shapes.
 //
 //   gett_bench [generation=2] [repetitions=5] [No Nv NG slice]
+//   (generation 1, 2: FP64 kernels; 3: split-bf16 mixed precision)
 //
 // Runs the complex GETT on device buffers for the shapes of the ladder
 // (formation and contraction, Ccsd.cxx:588-590), the ring and crossed-ring
@@ -38,7 +39,8 @@ int main(int argc, char **argv) {
   const int64_t v = argc > 4 ? std::atoll(argv[4]) : 256;
   const int64_t G = argc > 5 ? std::atoll(argv[5]) : 1500;
   const int64_t s = argc > 6 ? std::atoll(argv[6]) : o;
-  set_complex_gett_generation(gen);
+  // generation 3: the split-bf16 mixed-precision path (packing included)
+  set_complex_gett_generation(gen == 1 ? 1 : 2);
   const std::vector<Case> cases = {
       {"ladder-form  Gxc,Gyd->cxdy", "Gxc", "Gyd", "cxdy", {G, s, v}, {G, s, v}, {v, s, v, s}},
       {"ladder-ctr   cxdy,cdij->xyij", "cxdy", "cdij", "xyij", {v, s, v, s}, {v, v, o, o}, {s, s, o, o}},
@@ -52,6 +54,10 @@ int main(int argc, char **argv) {
     CC4S_HIP(hipStreamCreate(&st));
     ContractionEngine E(st);
     E.set_profiling(true);
+    if (gen == 3) {
+      E.set_mixed(true);
+      E.mixed_min_flops = 0.0;
+    }
     for (const auto &c : cases) {
       Tensor A(Field::Complex, c.ia, c.la), B(Field::Complex, c.ib, c.lb),
           C(Field::Complex, c.ic, c.lc);

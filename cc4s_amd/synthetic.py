@@ -165,6 +165,35 @@ def max_abs_vpphh(gamma: np.ndarray, No: int, field: str) -> float:
     return float(np.max(np.abs(V)))
 
 
+def mp2_of_vertex(gamma: np.ndarray, eps: np.ndarray, No: int,
+                  field: str) -> float:
+    """Second-order energy of the vertex as SecondOrderPerturbationTheory.cxx:
+    108-127 forms it (no singles), from the pphh slice defined as in
+    max_abs_vpphh."""
+    NG, Np, _ = gamma.shape
+    Nv = Np - No
+    if field == "complex":
+        left = np.conj(gamma[:, :No, No:]).transpose(0, 2, 1)
+        L = left.reshape(NG, Nv * No, order="F")
+        R = gamma[:, No:, :No].reshape(NG, Nv * No, order="F")
+        V = L.T @ R
+    else:
+        ph = gamma[:, No:, :No]
+        L = np.ascontiguousarray(ph.real.reshape(NG, Nv * No, order="F"))
+        Li = np.ascontiguousarray(ph.imag.reshape(NG, Nv * No, order="F"))
+        V = L.T @ L + Li.T @ Li
+    V = V.reshape(Nv, No, Nv, No, order="F").transpose(0, 2, 1, 3)   # [a,b,i,j]
+    dph = eps[No:, None] - eps[None, :No]
+    Dpphh = dph[:, None, :, None] + dph[None, :, None, :]
+    T = np.conj(V) / Dpphh
+    return float(np.real(-2.0 * np.sum(V * T) + np.sum(V.transpose(0, 1, 3, 2) * T)))
+
+
+# MP2 energy per occupied orbital of the config-0 workload (No=8, Nv=40,
+# NG=200, seed 100): -0.5527070232566246 / 8
+MP2_PER_OCCUPIED = -0.5527070232566246 / 8
+
+
 def eigen_energies(No: int, Nv: int, seed: int) -> np.ndarray:
     occ = np.sort(-1.2 + 0.8 * uniforms(seed, STREAM_EPS_OCC, np.arange(No)))
     vir = np.sort(0.2 + 3.8 * uniforms(seed, STREAM_EPS_VIR, np.arange(Nv)))
@@ -199,21 +228,36 @@ class CcsdInputs:
 
 
 def generate(No: int, Nv: int, NG: int, field: str, seed: int,
-             target_max_v: float = TARGET_MAX_V) -> CcsdInputs:
-    """Seeded inputs for one workload (see module docstring)."""
+             target_max_v: float = TARGET_MAX_V,
+             scale_rule: str = "max_v") -> CcsdInputs:
+    """Seeded inputs for one workload (see module docstring).
+
+    scale_rule "max_v" (the workload definition): c such that
+    max|V_abij| = target_max_v.  scale_rule "mp2_per_occupied": c such that
+    the second-order energy per occupied orbital equals that of the config-0
+    workload (E2 is proportional to c^4); at No, Nv beyond config 0 the max_v
+    rule gives couplings for which the CCSD iteration does not converge, this
+    rule keeps the correlation strength per electron fixed."""
     if field not in ("real", "complex"):
         raise ValueError("field must be 'real' or 'complex'")
     Np = No + Nv
     gamma = raw_vertex(NG, Np, field, seed)
-    vmax = max_abs_vpphh(gamma, No, field)
-    c = float(np.sqrt(target_max_v / vmax))
-    gamma *= c
     eps = eigen_energies(No, Nv, seed)
+    if scale_rule == "max_v":
+        vmax = max_abs_vpphh(gamma, No, field)
+        c = float(np.sqrt(target_max_v / vmax))
+    elif scale_rule == "mp2_per_occupied":
+        e2 = mp2_of_vertex(gamma, eps, No, field)
+        c = float((MP2_PER_OCCUPIED * No / e2) ** 0.25)
+    else:
+        raise ValueError(f"unknown scale rule {scale_rule!r}")
+    gamma *= c
     return CcsdInputs(gamma=gamma, eps=eps, No=No, field=field, scale=c)
 
 
-def generate_config(cfg: CcsdConfig) -> CcsdInputs:
-    return generate(cfg.No, cfg.Nv, cfg.NG, cfg.field, cfg.seed)
+def generate_config(cfg: CcsdConfig, scale_rule: str = "max_v") -> CcsdInputs:
+    return generate(cfg.No, cfg.Nv, cfg.NG, cfg.field, cfg.seed,
+                    scale_rule=scale_rule)
 
 
 def random_amplitudes(No: int, Nv: int, field: str, seed: int,

@@ -44,9 +44,12 @@ typedef struct cc4s_ccsd_context cc4s_ccsd_context;
  * VertexCoulombIntegrals.cxx:45-52). */
 enum { CC4S_FIELD_REAL = 0, CC4S_FIELD_COMPLEX = 1 };
 
-/* Arithmetic of the ladder and ring contractions.  FP64 reproduces the
- * reference; MIXED is reserved for the split 16-bit MFMA mode and is refused
- * by this build. */
+/* Arithmetic of the large contractions (ladder, ring, Gamma terms).  FP64
+ * reproduces the reference.  MIXED (complex field only) runs them on bf16
+ * MFMA with every FP64 operand split into hi + lo parts and FP32
+ * accumulation, until the mixer's dR falls below mixed_switch, then switches
+ * to FP64 for the remaining passes (iterative refinement of the same fixed
+ * point).  Denominators, energy, DIIS and every small contraction stay FP64. */
 enum { CC4S_PRECISION_FP64 = 0, CC4S_PRECISION_MIXED = 1 };
 
 /* Step arguments of CoupledCluster (CoupledCluster.cxx:59-117,260-262,
@@ -62,7 +65,7 @@ typedef struct {
   int ppl;                        /* ppl (1) */
   int64_t integrals_slice_size;   /* integralsSliceSize (0 = No) */
   int precision;                  /* CC4S_PRECISION_* */
-  double mixed_switch;            /* reserved */
+  double mixed_switch;            /* MIXED: dR below which FP64 takes over (1e-5) */
   int device;                     /* HIP device ordinal */
 } cc4s_ccsd_params;
 
@@ -162,7 +165,7 @@ int cc4s_contract(int field, double alpha, const double *A, const char *idx_a,
                   const int64_t *lens_a, int conj_a, const double *B,
                   const char *idx_b, const int64_t *lens_b, int conj_b,
                   double beta, double *C, const char *idx_c,
-                  const int64_t *lens_c, int device);
+                  const int64_t *lens_c, int precision, int device);
 
 #ifdef __cplusplus
 }

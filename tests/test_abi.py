@@ -45,5 +45,7 @@ def test_default_params_are_the_reference_defaults():
 
 
 def test_unsupported_precision_is_refused_before_touching_a_device():
-    with pytest.raises(api.CcsdError, match="precision"):
-        api.CoupledCluster(2, 4, 3, precision=api.PRECISION_MIXED)
+    with pytest.raises(api.CcsdError, match="precision mode"):
+        api.CoupledCluster(2, 4, 3, precision=7)
+    with pytest.raises(api.CcsdError, match="complex field"):
+        api.CoupledCluster(2, 4, 3, field="real", precision=api.PRECISION_MIXED)

@@ -1,0 +1,88 @@
+"""Mixed precision (split-bf16 MFMA, FP32 accumulation, FP64 refinement).
+
+Bars: a split-bf16 contraction within 5e-5 relative of the FP64 result
+(each operand part carries ~16 significant bits); a mixed residuum within
+1e-4 relative of the FP64 one; a mixed-then-FP64 run converging to the FP64
+energy within 1 microhartree.
+"""
+
+import numpy as np
+import pytest
+
+from cc4s_amd import api
+from cc4s_amd import synthetic as syn
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("ac", "cbij", "abij"), ("Gai", "Gbj", "abij"), ("akic", "cbkj", "abij"),
+         ("bkci", "ackj", "abij"), ("klcd", "adkl", "ac"), ("cxdy", "cdij", "xyij"),
+         ("Gxc", "Gyd", "cxdy")]
+LENS = {"a": 21, "b": 21, "c": 21, "d": 21, "i": 6, "j": 6, "k": 6, "l": 6,
+        "G": 77, "x": 9, "y": 7}
+
+
+@pytest.mark.parametrize("case", CASES, ids=["%s,%s->%s" % c for c in CASES])
+def test_mixed_contract_accuracy(case):
+    ia, ib, ic = case
+    rng = np.random.default_rng(7)
+    mk = lambda idx: (rng.standard_normal(tuple(LENS[c] for c in idx))
+                      + 1j * rng.standard_normal(tuple(LENS[c] for c in idx)))
+    A, B, C = mk(ia), mk(ib), mk(ic)
+    for alpha, beta in ((1.0, 0.0), (-0.5, 1.0)):
+        ref = alpha * np.einsum(f"{ia},{ib}->{ic}", A, B) + beta * C
+        out = api.contract(alpha, A, ia, B, ib, beta, C, ic,
+                           precision=api.PRECISION_MIXED)
+        assert rel_err(out, ref) < 5e-5, (case, alpha, beta, rel_err(out, ref))
+
+
+def test_mixed_contract_conjugation_and_large_k():
+    rng = np.random.default_rng(3)
+    M, N, K = 300, 200, 9000
+    A = rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
+    C = np.zeros((M, N), complex)
+    out = api.contract(1.0, A, "mk", B, "kn", 0.0, C, "mn", conj_a=True,
+                       precision=api.PRECISION_MIXED)
+    assert rel_err(out, np.conj(A) @ B) < 5e-5
+
+
+def test_mixed_rejects_real_field():
+    with pytest.raises(api.CcsdError, match="complex"):
+        api.contract(1.0, np.ones((3, 4)), "mk", np.ones((4, 2)), "kn", 0.0,
+                     np.zeros((3, 2)), "mn", precision=api.PRECISION_MIXED)
+
+
+def test_mixed_residuum_accuracy():
+    No, Nv, NG = 8, 40, 200
+    inp = syn.generate(No, Nv, NG, "complex", 5)
+    T = syn.random_amplitudes(No, Nv, "complex", 9, t1_scale=0.02, t2_scale=0.02)
+    out = {}
+    for prec in (api.PRECISION_FP64, api.PRECISION_MIXED):
+        with api.CoupledCluster(No, Nv, NG, precision=prec) as cc:
+            cc.set_inputs(inp.gamma, inp.eps)
+            out[prec] = cc.residuum(T)
+    assert rel_err(out[1][1], out[0][1]) < 1e-4
+    assert rel_err(out[1][0], out[0][0]) < 1e-4
+
+
+@pytest.mark.parametrize("shape", [(6, 24, 60), (12, 60, 300)])
+def test_mixed_run_reaches_fp64_energy(shape):
+    No, Nv, NG = shape
+    print(f"mixed run {shape}", flush=True)
+    inp = syn.generate(No, Nv, NG, "complex", 42, scale_rule="mp2_per_occupied")
+    kw = dict(maxIterations=60, energyConvergence=1e-10,
+              amplitudesConvergence=1e-8, maxResidua=8)
+    res = {}
+    for prec in (api.PRECISION_FP64, api.PRECISION_MIXED):
+        with api.CoupledCluster(No, Nv, NG, precision=prec, **kw) as cc:
+            cc.set_inputs(inp.gamma, inp.eps)
+            res[prec] = cc.run()
+    fp64, mixed = res[0], res[1]
+    assert fp64["convergenceReached"] and mixed["convergenceReached"]
+    assert abs(mixed["energy"]["correlation"] - fp64["energy"]["correlation"]) < 1e-6
+    used = [t["precision_used"] for t in mixed["trace"]]
+    assert used[0] == api.PRECISION_MIXED and used[-1] == api.PRECISION_FP64
+    # once switched, FP64 to the end
+    first64 = used.index(api.PRECISION_FP64)
+    assert all(u == api.PRECISION_FP64 for u in used[first64:])
