@@ -22,14 +22,20 @@ inline void *offset_ptr(const void *p, int64_t elements, Field f) {
 }  // namespace
 
 struct Ccsd::Work {
-  // Coulomb integral slices (complex: 7, real: 3)
-  Tensor Vpphh, Vphhp, Vhhpp, Vphph, Vhhhh, Vhhhp, Vphhh;
-  // intermediates
-  Tensor X, Y, Kac, Lac, Kki, Lki, Kck, Xakic, Xakci, Xklij;
+  // Coulomb integral slices kept in HBM: pphh, hhhh, hhhp (both fields),
+  // phhh and hhpp (complex).  For an exactly Hermitian vertex hhpp is the
+  // conjugate transpose of pphh (V_klcd = conj V_cdkl) and is not stored.
+  // phhp and phph are never formed: their three uses go through Gamma.
+  Tensor Vpphh, Vhhpp, Vhhhh, Vhhhp, Vphhh;
+  View Vhhpp_view;
+  // intermediates; Xakci lives in Xakic's buffer (Xakic is consumed first)
+  Tensor X, Y, Kac, Lac, Kki, Lki, Kck, Xakic, Xklij;
+  View Xakci;
   Tensor DGph, DGpp, DGhh, Dpp;  // dressed vertices
-  Tensor wG, Zgvo, Wgvo, Wvooo, Woo;
+  Tensor wG, Zgvo, Wgvo, Wvooo, Woo, Zgoo;
   Tensor slab, Rxy;
   int64_t NGe = 0;  // length of the G index of the real-path vertex views
+  bool hermitian = false;
 };
 
 Ccsd::Ccsd(ContractionEngine &E, const CcsdShape &shape, const CcsdOptions &opt,
@@ -51,7 +57,7 @@ Ccsd::Ccsd(ContractionEngine &E, const CcsdShape &shape, const CcsdOptions &opt,
   w.Lki.reset(f, "ki", {o, o});
   w.Kck.reset(f, "ck", {v, o});
   w.Xakic.reset(f, "akic", {v, o, o, v});
-  w.Xakci.reset(f, "akci", {v, o, v, o});
+  w.Xakci = dense_view(w.Xakic.buf.get(), f, "akci", {v, o, v, o});
   w.Xklij.reset(f, "klij", {o, o, o, o});
   w.DGph.reset(f, "Gai", {G, v, o});
   w.DGpp.reset(f, "Gab", {G, v, v});
@@ -64,6 +70,7 @@ Ccsd::Ccsd(ContractionEngine &E, const CcsdShape &shape, const CcsdOptions &opt,
   if (f == Field::Complex) {
     w.Y.reset(f, "abij", {v, v, o, o});
     w.Dpp.reset(f, "Gab", {G, v, v});
+    w.Zgoo.reset(f, "Gkj", {G, o, o});
   }
 }
 
@@ -127,13 +134,21 @@ void Ccsd::form_integrals() {
   make(w.Vhhhh, "hhhh");
   make(w.Vhhhp, "hhhp");
   if (f == Field::Complex) {
-    make(w.Vphhp, "phhp");
-    make(w.Vhhpp, "hhpp");
-    make(w.Vphph, "phph");
     make(w.Vphhh, "phhh");
+    w.hermitian = is_hermitian(E_.stream(), gamma_, s_.NG, s_.Np());
+    if (w.hermitian) {
+      // conj(V_cdkl) = sum_G Gamma[G,k,c] conj Gamma[G,d,l]
+      //             = sum_G conj(Gamma[G,c,k]) Gamma[G,l,d] = V_klcd
+      w.Vhhpp_view = w.Vpphh.v.transposed({2, 3, 0, 1}).conjugated();
+    } else {
+      make(w.Vhhpp, "hhpp");
+      w.Vhhpp_view = w.Vhhpp.v;
+    }
   }
   CC4S_HIP(hipStreamSynchronize(E_.stream()));
 }
+
+bool Ccsd::vertex_is_hermitian() const { return w_->hermitian; }
 
 void Ccsd::get_residuum(const void *T, void *R) {
   auto t0 = std::chrono::steady_clock::now();
@@ -231,7 +246,9 @@ void Ccsd::residuum_complex(const void *Tp, void *Rp) {
     E.sum(al, A, be, Cv);
   };
   auto &X = w.X, &Y = w.Y;
-  auto &Vhhpp = w.Vhhpp, &Vhhhp = w.Vhhhp, &Vphhh = w.Vphhh, &Vphhp = w.Vphhp;
+  auto &Vhhhp = w.Vhhhp, &Vphhh = w.Vphhh;
+  auto Vhhpp = [&w](const std::string &names) { return w.Vhhpp_view.as(names); };
+  auto Xakci = [&w](const std::string &names) { return w.Xakci.as(names); };
 
   tau(st, f, X.buf.get(), T2.ptr, T1.ptr, 1.0, o, v);            // :460-461
   tau(st, f, Y.buf.get(), T2.ptr, T1.ptr, 2.0, o, v);            // :462-463
@@ -254,7 +271,10 @@ void Ccsd::residuum_complex(const void *Tp, void *Rp) {
   C(1.0, gam("pp", "Gbc"), T1.as("cj"), 0.0, w.Wgvo("Gbj"));    // :489
   C(1.0, w.DGph("Gai"), w.Wgvo("Gbj"), 1.0, R2);
   C(-1.0, Vphhh("akij"), T1.as("bk"), 1.0, R2);                 // :490
-  C(1.0, Vphhp("akic"), T1.as("cj"), 0.0, w.Wvooo("akij"));     // :491
+  // :491 with V_akic = sum_G cTGamma_ph[G,a,i] Gamma_hp[G,k,c]
+  // (VertexCoulombIntegrals.cxx:166-168, slice phhp) contracted through G
+  C(1.0, gam("hp", "Gkc"), T1.as("cj"), 0.0, w.Zgoo("Gkj"));
+  C(1.0, gam("ph", "Gai", true), w.Zgoo("Gkj"), 0.0, w.Wvooo("akij"));
   C(-1.0, w.Wvooo("akij"), T1.as("bk"), 1.0, R2);
   C(1.0, gam("pp", "Gad", true), T1.as("di"), 1.0, w.DGph("Gai"));   // :494
   C(-0.5, gam("hp", "Gld", true), Y("dail"), 1.0, w.DGph("Gai"));   // :495
@@ -269,10 +289,10 @@ void Ccsd::residuum_complex(const void *Tp, void *Rp) {
   C(-1.0, gam("hp", "Glc", true), T1.as("al"), 1.0, w.DGpp("Gac"));  // :510
   S(1.0, gam("hh", "Gij"), 0.0, w.DGhh("Gij"));                 // :511
   C(1.0, gam("hp", "Gkd"), T1.as("di"), 1.0, w.DGhh("Gki"));    // :512
-  C(1.0, w.DGpp("Gac"), w.DGhh("Gki"), 0.0, w.Xakci("akci"));   // :514
-  C(-0.5, Vhhpp("lkcd"), T2.as("dail"), 1.0, w.Xakci("akci"));  // :515
-  C(-1.0, w.Xakci("akci"), T2.as("cbkj"), 1.0, R2);             // :516
-  C(-1.0, w.Xakci("bkci"), T2.as("ackj"), 1.0, R2);             // :517
+  C(1.0, w.DGpp("Gac"), w.DGhh("Gki"), 0.0, Xakci("akci"));     // :514
+  C(-0.5, Vhhpp("lkcd"), T2.as("dail"), 1.0, Xakci("akci"));    // :515
+  C(-1.0, Xakci("akci"), T2.as("cbkj"), 1.0, R2);               // :516
+  C(-1.0, Xakci("bkci"), T2.as("ackj"), 1.0, R2);               // :517
   symmetrize_pairs(st, f, R2.ptr, o, v);                        // :520
   S(1.0, Vpphh, 1.0, R2);                                       // :529
   S(1.0, w.Vhhhh("klij"), 0.0, w.Xklij("klij"));                // :534
@@ -294,8 +314,11 @@ void Ccsd::residuum_complex(const void *Tp, void *Rp) {
   C(-1.0, w.Kck("ck"), T2.as("caik"), 1.0, R1);                 // :618
   C(1.0, w.Kck("ck"), T1.as("ci"), 0.0, w.Woo("ki"));           // :619
   C(1.0, w.Woo("ki"), T1.as("ak"), 1.0, R1);
-  C(2.0, Vphhp("akic"), T1.as("ck"), 1.0, R1);                  // :620
-  C(-1.0, w.Vphph("akci"), T1.as("ck"), 1.0, R1);               // :621
+  // :620, V_akic T_ck = sum_G cTGamma_ph[G,a,i] (sum_kc Gamma_hp[G,k,c] T_ck)
+  C(2.0, gam("ph", "Gai", true), w.wG("G"), 1.0, R1);
+  // :621, V_akci = sum_G cTGamma_pp[G,a,c] Gamma_hh[G,k,i] (slice phph)
+  C(1.0, gam("hh", "Gki"), T1.as("ck"), 0.0, w.Wgvo("Gci"));
+  C(-1.0, gam("pp", "Gac", true), w.Wgvo("Gci"), 1.0, R1);
   C(1.0, gam("hp", "Gkd"), X("cdik"), 0.0, w.Wgvo("Gci"));      // :622-623
   C(2.0, gam("pp", "Gac", true), w.Wgvo("Gci"), 1.0, R1);
   C(1.0, gam("hp", "Gkc"), X("cdik"), 0.0, w.Wgvo("Gdi"));      // :624-625
@@ -361,14 +384,14 @@ void Ccsd::residuum_real(const void *Tp, void *Rp) {
   S(2.0, T2.as("cbkj"), 0.0, X("cbkj"));                         // :227
   S(-1.0, T2.as("bckj"), 1.0, X("cbkj"));                        // :228
   C(1.0, w.Xakic("akic"), X("cbkj"), 1.0, R2);                   // :229
-  C(-0.5, Vp("cdlk"), T2.as("dail"), 0.0, w.Xakci("akci"));      // :239
+  C(-0.5, Vp("cdlk"), T2.as("dail"), 0.0, w.Xakci.as("akci"));      // :239
   S(1.0, gam("pp", "Gab"), 0.0, w.DGpp("Gab"));                  // :240-241
   C(-1.0, gam("ph", "Gcl"), T1.as("al"), 1.0, w.DGpp("Gac"));    // :242-243
   S(1.0, gam("hh", "Gij"), 0.0, w.DGhh("Gij"));                  // :244-245
   C(1.0, gam("ph", "Gdk"), T1.as("di"), 1.0, w.DGhh("Gki"));     // :246-247
-  C(1.0, w.DGpp("Gac"), w.DGhh("Gki"), 1.0, w.Xakci("akci"));    // :248-249
-  C(-1.0, w.Xakci("akci"), T2.as("cbkj"), 1.0, R2);              // :252
-  C(-1.0, w.Xakci("bkci"), T2.as("ackj"), 1.0, R2);              // :253
+  C(1.0, w.DGpp("Gac"), w.DGhh("Gki"), 1.0, w.Xakci.as("akci"));    // :248-249
+  C(-1.0, w.Xakci.as("akci"), T2.as("cbkj"), 1.0, R2);              // :252
+  C(-1.0, w.Xakci.as("bkci"), T2.as("ackj"), 1.0, R2);              // :253
   symmetrize_pairs(st, f, R2.ptr, o, v);                         // :256
   S(1.0, Vpphh, 1.0, R2);                                        // :263
   tau(st, f, X.buf.get(), T2.ptr, T1.ptr, 1.0, o, v);            // :267-268

@@ -48,6 +48,8 @@ class Ccsd {
   void get_residuum(const void *T, void *R);
 
   const View &Vpphh() const;
+  // Gamma^G_qp == conj(Gamma^G_pq) bit for bit (checked on the device)
+  bool vertex_is_hermitian() const;
   CcsdTimings timings;
   bool profile = false;
 

@@ -2,7 +2,9 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 
 namespace cc4s_amd {
 
@@ -10,14 +12,30 @@ namespace cc4s_amd {
 // DiisMixer
 // ---------------------------------------------------------------------------
 
-DiisMixer::DiisMixer(Field f, int64_t nset, int64_t N, hipStream_t s)
-    : f_(f), n_(nset), N_(N), s_(s) {
+namespace {
+constexpr size_t kStagingBytes = size_t(1) << 30;  // host-slot transfer chunk
+}
+
+size_t DiisMixer::device_bytes(Field f, int64_t No, int64_t Nv, int64_t N,
+                               bool on_host) {
+  const size_t set = static_cast<size_t>(packed_set_size(No, Nv)) * scalar_bytes(f);
+  return on_host ? N * set + std::min(set, kStagingBytes) : 2 * N * set;
+}
+
+DiisMixer::DiisMixer(Field f, int64_t No, int64_t Nv, int64_t N,
+                     bool amplitudes_on_host, hipStream_t s)
+    : f_(f), No_(No), Nv_(Nv), n_(packed_set_size(No, Nv)), N_(N),
+      on_host_(amplitudes_on_host), s_(s) {
   CC4S_ASSERT(N >= 1 && N <= 16, "maxResidua must be in [1, 16]");
-  A_.resize(N);
+  const size_t set = static_cast<size_t>(n_) * scalar_bytes(f);
   R_.resize(N);
-  for (int64_t i = 0; i < N; ++i) {
-    A_[i].allocate(nset * scalar_bytes(f));
-    R_[i].allocate(nset * scalar_bytes(f));
+  for (int64_t i = 0; i < N; ++i) R_[i].allocate(set);
+  if (on_host_) {
+    for (int64_t i = 0; i < N; ++i) Ahost_.emplace_back(set);
+    staging_.allocate(std::min(set, kStagingBytes));
+  } else {
+    A_.resize(N);
+    for (int64_t i = 0; i < N; ++i) A_[i].allocate(set);
   }
   filled_.assign(N, false);
   const size_t M = N + 1;
@@ -59,6 +77,27 @@ std::vector<double> DiisMixer::solve(std::vector<double> a, size_t n) const {
   return x;
 }
 
+void DiisMixer::write_next(const void *R, const void *T, const double *eps_h,
+                           const double *eps_p, double shift) {
+  const size_t set = static_cast<size_t>(n_) * scalar_bytes(f_);
+  if (!on_host_) {
+    amplitude_update_packed(s_, f_, R, T, eps_h, eps_p, shift, A_[next_].get(),
+                            R_[next_].get(), No_, Nv_);
+    return;
+  }
+  // T' goes through the residuum slot to the host: the update kernel runs
+  // for T' alone, the slot is copied out, then it runs again for dT (on the
+  // first pass dT = T' is already in place)
+  amplitude_update_packed(s_, f_, R, T, eps_h, eps_p, shift, R_[next_].get(),
+                          nullptr, No_, Nv_);
+  CC4S_HIP(hipMemcpyAsync(Ahost_[next_].get(), R_[next_].get(), set,
+                          hipMemcpyDeviceToHost, s_));
+  if (T)
+    amplitude_update_packed(s_, f_, R, T, eps_h, eps_p, shift, nullptr,
+                            R_[next_].get(), No_, Nv_);
+  CC4S_HIP(hipStreamSynchronize(s_));
+}
+
 void DiisMixer::append(void *out) {
   const size_t N = static_cast<size_t>(N_);
   filled_[next_] = true;                          // DiisMixer.cxx:65-66
@@ -71,8 +110,8 @@ void DiisMixer::append(void *out) {
       which.push_back(i);
     }
   std::vector<cplx> dots;
-  multi_dot(s_, f_, xs.data(), static_cast<int>(xs.size()), R_[next_].get(), n_,
-            partials_.get(), dots);
+  multi_dot_packed(s_, f_, xs.data(), static_cast<int>(xs.size()), R_[next_].get(),
+                   No_, Nv_, partials_.get(), dots);
   for (size_t k = 0; k < which.size(); ++k) {
     const size_t i = which[k];
     const double overlap = 2.0 * dots[k].real();
@@ -87,15 +126,36 @@ void DiisMixer::append(void *out) {
   const std::vector<double> column = solve(matrix, dim);   // :89
   // next = sum_j c_j A_(next-j), residuum = sum_j c_j R_(next-j) (:92-104)
   std::vector<const void *> As, Rs;
+  std::vector<size_t> slots;
   coeffs_.clear();
   for (size_t j = 0; j < count_; ++j) {
     const size_t i = (next_ + N - j) % N;
-    As.push_back(A_[i].get());
+    slots.push_back(i);
+    if (!on_host_) As.push_back(A_[i].get());
     Rs.push_back(R_[i].get());
     coeffs_.push_back(column[i + 1]);
   }
-  const double rr = combine(s_, f_, As.data(), Rs.data(), coeffs_, n_, out,
-                            partials_.get());
+  double rr;
+  if (!on_host_) {
+    rr = combine_packed(s_, f_, As.data(), Rs.data(), coeffs_, No_, Nv_, out,
+                        partials_.get());
+  } else {
+    // stream every host slot through the staging buffer, accumulating in
+    // the same order (most recent first)
+    const size_t es = scalar_bytes(f_);
+    const int64_t chunk = static_cast<int64_t>(staging_.bytes() / es);
+    for (size_t j = 0; j < slots.size(); ++j) {
+      const char *src = static_cast<const char *>(Ahost_[slots[j]].get());
+      for (int64_t t0 = 0; t0 < n_; t0 += chunk) {
+        const int64_t len = std::min(chunk, n_ - t0);
+        CC4S_HIP(hipMemcpyAsync(staging_.get(), src + t0 * es, len * es,
+                                hipMemcpyHostToDevice, s_));
+        unpack_accumulate(s_, f_, staging_.get(), t0, len, coeffs_[j], j == 0, out,
+                          No_, Nv_);
+      }
+    }
+    rr = norm_packed(s_, f_, Rs.data(), coeffs_, No_, Nv_, partials_.get());
+  }
   norm_ = std::sqrt(rr);                          // :106
   next_ = (next_ + 1) % N;                        // :105
 }
@@ -115,10 +175,38 @@ CoupledCluster::CoupledCluster(const CoupledClusterParams &p, int device)
   T_.allocate(s.nset() * es);
   R_.allocate(s.nset() * es);
   energy_partials_.allocate(energy_partials_size());
-  mixer_.reset(new DiisMixer(s.field, s.nset(), p_.max_residua, stream_));
+  CC4S_ASSERT(p_.max_residua >= 1 && p_.max_residua <= 16,
+              "maxResidua must be in [1, 16]");
   CC4S_ASSERT(!p_.mixed || s.field == Field::Complex,
               "mixed precision needs the complex field");
   mixed_phase_ = p_.mixed;
+}
+
+// The DIIS history is the largest allocation; it is placed once the vertex,
+// the integrals and the intermediates are resident.  Amplitude slots go to
+// pinned host memory when HBM cannot hold both kinds of slot (override with
+// CC4S_DIIS_PLACEMENT=device|host).
+void CoupledCluster::create_mixer() {
+  const auto &s = p_.shape;
+  size_t free_b = 0, total_b = 0;
+  CC4S_HIP(hipMemGetInfo(&free_b, &total_b));
+  // room kept for buffers allocated on first use (ladder slab, operand
+  // packs, split-K partials, residuum scratch)
+  const size_t reserve = (size_t(6) << 30) + s.nset() * scalar_bytes(s.field);
+  const size_t avail = free_b > reserve ? free_b - reserve : 0;
+  const char *env = std::getenv("CC4S_DIIS_PLACEMENT");
+  const std::string placement = env ? env : "auto";
+  bool on_host;
+  if (placement == "host") {
+    on_host = true;
+  } else if (placement == "device") {
+    on_host = false;
+  } else {
+    on_host = DiisMixer::device_bytes(s.field, s.No, s.Nv, p_.max_residua, false) > avail;
+    CC4S_ASSERT(DiisMixer::device_bytes(s.field, s.No, s.Nv, p_.max_residua, true) <= avail,
+                "out of device memory for the DIIS history");
+  }
+  mixer_.reset(new DiisMixer(s.field, s.No, s.Nv, p_.max_residua, on_host, stream_));
 }
 
 CoupledCluster::~CoupledCluster() {
@@ -154,6 +242,7 @@ void CoupledCluster::set_inputs(const double *gamma, bool on_device,
   ccsd_.reset(new Ccsd(*engine_, s, p_.ccsd, gamma_.get(), gamma_real));
   ccsd_->form_integrals();
   CC4S_HIP(hipStreamSynchronize(stream_));
+  if (!mixer_) create_mixer();
   inputs_set_ = true;
 }
 
@@ -204,11 +293,8 @@ IterationReport CoupledCluster::iterate() {
   ccsd_->get_residuum(T, R_.get());                     // :136
   engine_->set_mixed(false);
   // :137-145: T' = -R/(Delta+s) and dT = T' - T, written into the mixer slot
-  amplitude_update(stream_, s.field, R_.get(), T,
-                   static_cast<const double *>(eps_h_.get()),
-                   static_cast<const double *>(eps_p_.get()), p_.level_shift,
-                   mixer_->next_amplitudes_slot(), mixer_->next_residuum_slot(),
-                   s.No, s.Nv);
+  mixer_->write_next(R_.get(), T, static_cast<const double *>(eps_h_.get()),
+                     static_cast<const double *>(eps_p_.get()), p_.level_shift);
   const bool is_second_order = !have_amplitudes_;
   mixer_->append(T_.get());                             // :145-147
   have_amplitudes_ = true;

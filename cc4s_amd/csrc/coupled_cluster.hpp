@@ -25,27 +25,37 @@ struct CoupledClusterParams {
   double mixed_switch = 1e-5;
 };
 
-// DIIS over (amplitudes, amplitude change) pairs held in HBM as ring-buffer
-// slots.  The new pair is written straight into slot(next) by the
-// denominator-update kernel, then append() updates the overlap matrix,
-// solves the bordered system on the host in FP64 and forms the extrapolated
-// amplitudes (DiisMixer.cxx:58-107).
+// DIIS over (amplitudes, amplitude change) pairs held as ring-buffer slots
+// (DiisMixer.cxx:58-107).  Slots are pair-packed sets: T1 plus the pairs
+// (a,i) <= (b,j) of T2, half the size of a full set, exact because
+// T2_abij = T2_baji.  The amplitude-change slots stay in HBM (they take part
+// in every overlap); the amplitude slots are in HBM too, or in pinned host
+// memory when HBM cannot hold them (they are read once per pass, to form the
+// extrapolated amplitudes).
 class DiisMixer {
  public:
-  DiisMixer(Field f, int64_t nset, int64_t N, hipStream_t s);
-  void *next_amplitudes_slot() const { return A_[next_].get(); }
-  void *next_residuum_slot() const { return R_[next_].get(); }
-  // uses the pair in slot(next); writes sum_j c_j A_j to `out`
+  DiisMixer(Field f, int64_t No, int64_t Nv, int64_t N, bool amplitudes_on_host,
+            hipStream_t s);
+  // T' = -(R - shift T)/(Delta + shift) and dT = T' - T into slot(next)
+  void write_next(const void *R, const void *T, const double *eps_h,
+                  const double *eps_p, double shift);
+  // overlaps, bordered solve, and `out` (full layout) = sum_j c_j A_j
   void append(void *out);
   double residuum_norm() const { return norm_; }
   const std::vector<double> &coefficients() const { return coeffs_; }
+  bool amplitudes_on_host() const { return on_host_; }
+  // device bytes a mixer with these parameters allocates
+  static size_t device_bytes(Field f, int64_t No, int64_t Nv, int64_t N, bool on_host);
 
  private:
   std::vector<double> solve(std::vector<double> matrix, size_t dim) const;
   Field f_;
-  int64_t n_, N_;
+  int64_t No_, Nv_, n_, N_;
+  bool on_host_;
   hipStream_t s_;
   std::vector<DeviceBuffer> A_, R_;
+  std::vector<HostBuffer> Ahost_;
+  DeviceBuffer staging_;
   std::vector<bool> filled_;
   std::vector<double> B_;
   size_t next_ = 0, count_ = 0;
@@ -87,6 +97,7 @@ class CoupledCluster {
   void energy(const void *T, cplx *E, cplx *D, cplx *X);
   void upload_set(void *dst, const double *T1, const double *T2);
   void download_set(const void *src, double *T1, double *T2);
+  void create_mixer();
 
   CoupledClusterParams p_;
   int device_;

@@ -366,6 +366,332 @@ double combine(hipStream_t s, Field f, const void *const *A, const void *const *
   return h.x;
 }
 
+// ---------------------------------------------------------------------------
+// pair-packed sets
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// (P, Q) with P <= Q of pair index pi = Q(Q+1)/2 + P
+__device__ __forceinline__ void pair_of(int64_t pi, int64_t &P, int64_t &Q) {
+  int64_t q = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(pi) + 1.0) - 1.0) * 0.5);
+  while (q * (q + 1) / 2 > pi) --q;
+  while ((q + 1) * (q + 2) / 2 <= pi) ++q;
+  Q = q;
+  P = pi - q * (q + 1) / 2;
+}
+
+__device__ __forceinline__ int64_t pair_index(int64_t P, int64_t Q) {
+  return P <= Q ? Q * (Q + 1) / 2 + P : P * (P + 1) / 2 + Q;
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void update_packed_kernel(
+    const typename Sc<CPLX>::T *R, const typename Sc<CPLX>::T *T,
+    const double *eh, const double *ep, double shift, typename Sc<CPLX>::T *Tn,
+    typename Sc<CPLX>::T *dT, int64_t No, int64_t Nv) {
+  using S = Sc<CPLX>;
+  const int64_t n1 = Nv * No;
+  const int64_t np = n1 * (n1 + 1) / 2;
+  for (int64_t t = blockIdx.x * int64_t(kBlock) + threadIdx.x; t < n1 + np;
+       t += int64_t(gridDim.x) * kBlock) {
+    double delta;
+    int64_t e;
+    if (t < n1) {
+      const int64_t a = t % Nv, i = t / Nv;
+      delta = (0.0 + ep[a]) - eh[i];
+      e = t;
+    } else {
+      int64_t P, Q;
+      pair_of(t - n1, P, Q);
+      const int64_t a = P % Nv, i = P / Nv, b = Q % Nv, j = Q / Nv;
+      delta = (((0.0 + ep[a]) - eh[i]) + ep[b]) - eh[j];
+      e = n1 + a + Nv * (b + Nv * (i + No * j));
+    }
+    typename S::T rv = R[e];
+    if (T && shift != 0.0) rv = S::add(rv, S::scale(T[e], -shift));
+    const typename S::T tn = S::scale(rv, -1.0 / (delta + shift));
+    if (Tn) Tn[t] = tn;
+    if (dT) dT[t] = T ? S::add(tn, S::scale(T[e], -1.0)) : tn;
+  }
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void dots_packed_kernel(DotArgs p, int count,
+                                                             const typename Sc<CPLX>::T *y,
+                                                             int64_t n1, double2 *partials) {
+  using S = Sc<CPLX>;
+  __shared__ double2 sh[kBlock];
+  const int64_t np = n1 * (n1 + 1) / 2;
+  double2 acc[kMaxDiis];
+#pragma unroll
+  for (int k = 0; k < kMaxDiis; ++k) acc[k] = make_double2(0.0, 0.0);
+  for (int64_t t = blockIdx.x * int64_t(kBlock) + threadIdx.x; t < n1 + np;
+       t += int64_t(gridDim.x) * kBlock) {
+    double w = 1.0;
+    if (t >= n1) {
+      int64_t P, Q;
+      pair_of(t - n1, P, Q);
+      w = P == Q ? 1.0 : 2.0;  // an off-diagonal pair stands for two elements
+    }
+    const typename S::T yv = y[t];
+#pragma unroll
+    for (int k = 0; k < kMaxDiis; ++k) {
+      if (k < count) {
+        const double2 v = S::cdot(static_cast<const typename S::T *>(p.x[k])[t], yv);
+        acc[k].x += w * v.x;
+        acc[k].y += w * v.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kMaxDiis; ++k) {
+    if (k < count) {
+      const double2 s = block_sum(acc[k], sh);
+      if (threadIdx.x == 0) partials[blockIdx.x * count + k] = s;
+    }
+  }
+}
+
+// full-layout output; the norm runs over full elements, i.e. it is the norm
+// of the full symmetric tensor
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void combine_packed_kernel(DotArgs p, int count,
+                                                                typename Sc<CPLX>::T *out,
+                                                                int64_t No, int64_t Nv,
+                                                                double2 *partials) {
+  using S = Sc<CPLX>;
+  __shared__ double2 sh[kBlock];
+  const int64_t n1 = Nv * No;
+  const int64_t n = n1 + n1 * n1;
+  double2 nrm = make_double2(0.0, 0.0);
+  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kBlock) {
+    int64_t t = e;
+    if (e >= n1) {
+      int64_t r = e - n1;
+      const int64_t a = r % Nv; r /= Nv;
+      const int64_t b = r % Nv; r /= Nv;
+      const int64_t i = r % No;
+      const int64_t j = r / No;
+      t = n1 + pair_index(a + Nv * i, b + Nv * j);
+    }
+    typename S::T av = S::zero(), rv = S::zero();
+    for (int k = 0; k < count; ++k) {
+      av = S::add(av, S::scale(static_cast<const typename S::T *>(p.A[k])[t], p.c[k]));
+      rv = S::add(rv, S::scale(static_cast<const typename S::T *>(p.R[k])[t], p.c[k]));
+    }
+    out[e] = av;
+    nrm.x += S::cdot(rv, rv).x;
+  }
+  const double2 s = block_sum(nrm, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void unpack_accumulate_kernel(
+    const typename Sc<CPLX>::T *x, int64_t t0, int64_t len, double c, int first,
+    typename Sc<CPLX>::T *out, int64_t No, int64_t Nv) {
+  using S = Sc<CPLX>;
+  const int64_t n1 = Nv * No;
+  for (int64_t u = blockIdx.x * int64_t(kBlock) + threadIdx.x; u < len;
+       u += int64_t(gridDim.x) * kBlock) {
+    const int64_t t = t0 + u;
+    const typename S::T v = S::scale(x[u], c);
+    if (t < n1) {
+      out[t] = first ? v : S::add(out[t], v);
+      continue;
+    }
+    int64_t P, Q;
+    pair_of(t - n1, P, Q);
+    const int64_t a = P % Nv, i = P / Nv, b = Q % Nv, j = Q / Nv;
+    const int64_t e = n1 + a + Nv * (b + Nv * (i + No * j));
+    out[e] = first ? v : S::add(out[e], v);
+    if (P != Q) {
+      const int64_t m = n1 + b + Nv * (a + Nv * (j + No * i));
+      out[m] = first ? v : S::add(out[m], v);
+    }
+  }
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void norm_packed_kernel(DotArgs p, int count,
+                                                             int64_t n1,
+                                                             double2 *partials) {
+  using S = Sc<CPLX>;
+  __shared__ double2 sh[kBlock];
+  const int64_t np = n1 * (n1 + 1) / 2;
+  double nrm = 0.0;
+  for (int64_t t = blockIdx.x * int64_t(kBlock) + threadIdx.x; t < n1 + np;
+       t += int64_t(gridDim.x) * kBlock) {
+    double w = 1.0;
+    if (t >= n1) {
+      int64_t P, Q;
+      pair_of(t - n1, P, Q);
+      w = P == Q ? 1.0 : 2.0;
+    }
+    typename S::T rv = S::zero();
+    for (int k = 0; k < count; ++k)
+      rv = S::add(rv, S::scale(static_cast<const typename S::T *>(p.R[k])[t], p.c[k]));
+    nrm += w * S::cdot(rv, rv).x;
+  }
+  const double2 s = block_sum(make_double2(nrm, 0.0), sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+}  // namespace
+
+void unpack_accumulate(hipStream_t s, Field f, const void *chunk, int64_t t0,
+                       int64_t len, double c, bool first, void *out, int64_t No,
+                       int64_t Nv) {
+  if (f == Field::Complex)
+    unpack_accumulate_kernel<true><<<grid_for(len), kBlock, 0, s>>>(
+        static_cast<const double2 *>(chunk), t0, len, c, first ? 1 : 0,
+        static_cast<double2 *>(out), No, Nv);
+  else
+    unpack_accumulate_kernel<false><<<grid_for(len), kBlock, 0, s>>>(
+        static_cast<const double *>(chunk), t0, len, c, first ? 1 : 0,
+        static_cast<double *>(out), No, Nv);
+  CC4S_HIP(hipGetLastError());
+}
+
+double norm_packed(hipStream_t s, Field f, const void *const *R,
+                   const std::vector<double> &c, int64_t No, int64_t Nv,
+                   void *partials) {
+  const int count = static_cast<int>(c.size());
+  CC4S_ASSERT(count >= 1 && count <= kMaxDiis, "too many DIIS vectors");
+  DotArgs a;
+  for (int k = 0; k < count; ++k) {
+    a.R[k] = R[k];
+    a.c[k] = c[k];
+  }
+  double2 *part = static_cast<double2 *>(partials);
+  double2 *res = part + kReduceBlocks;
+  if (f == Field::Complex)
+    norm_packed_kernel<true><<<kReduceBlocks, kBlock, 0, s>>>(a, count, Nv * No, part);
+  else
+    norm_packed_kernel<false><<<kReduceBlocks, kBlock, 0, s>>>(a, count, Nv * No, part);
+  CC4S_HIP(hipGetLastError());
+  finish_kernel<<<1, kBlock, 0, s>>>(part, 1, kReduceBlocks, res);
+  CC4S_HIP(hipGetLastError());
+  double2 h;
+  CC4S_HIP(hipMemcpyAsync(&h, res, sizeof(h), hipMemcpyDeviceToHost, s));
+  CC4S_HIP(hipStreamSynchronize(s));
+  return h.x;
+}
+
+int64_t packed_set_size(int64_t No, int64_t Nv) {
+  const int64_t n1 = Nv * No;
+  return n1 + n1 * (n1 + 1) / 2;
+}
+
+void amplitude_update_packed(hipStream_t s, Field f, const void *R, const void *T,
+                             const double *eps_h, const double *eps_p, double shift,
+                             void *Tnew, void *dT, int64_t No, int64_t Nv) {
+  const int64_t n = packed_set_size(No, Nv);
+  if (f == Field::Complex)
+    update_packed_kernel<true><<<grid_for(n), kBlock, 0, s>>>(
+        static_cast<const double2 *>(R), static_cast<const double2 *>(T), eps_h,
+        eps_p, shift, static_cast<double2 *>(Tnew), static_cast<double2 *>(dT), No, Nv);
+  else
+    update_packed_kernel<false><<<grid_for(n), kBlock, 0, s>>>(
+        static_cast<const double *>(R), static_cast<const double *>(T), eps_h,
+        eps_p, shift, static_cast<double *>(Tnew), static_cast<double *>(dT), No, Nv);
+  CC4S_HIP(hipGetLastError());
+}
+
+void multi_dot_packed(hipStream_t s, Field f, const void *const *x, int count,
+                      const void *y, int64_t No, int64_t Nv, void *partials,
+                      std::vector<cplx> &out) {
+  CC4S_ASSERT(count >= 1 && count <= kMaxDiis, "too many DIIS vectors");
+  DotArgs a;
+  for (int k = 0; k < count; ++k) a.x[k] = x[k];
+  double2 *part = static_cast<double2 *>(partials);
+  double2 *res = part + static_cast<size_t>(kReduceBlocks) * count;
+  if (f == Field::Complex)
+    dots_packed_kernel<true><<<kReduceBlocks, kBlock, 0, s>>>(
+        a, count, static_cast<const double2 *>(y), Nv * No, part);
+  else
+    dots_packed_kernel<false><<<kReduceBlocks, kBlock, 0, s>>>(
+        a, count, static_cast<const double *>(y), Nv * No, part);
+  CC4S_HIP(hipGetLastError());
+  finish_kernel<<<1, kBlock, 0, s>>>(part, count, kReduceBlocks, res);
+  CC4S_HIP(hipGetLastError());
+  std::vector<double2> h(count);
+  CC4S_HIP(hipMemcpyAsync(h.data(), res, count * sizeof(double2),
+                          hipMemcpyDeviceToHost, s));
+  CC4S_HIP(hipStreamSynchronize(s));
+  out.resize(count);
+  for (int k = 0; k < count; ++k) out[k] = cplx(h[k].x, h[k].y);
+}
+
+double combine_packed(hipStream_t s, Field f, const void *const *A,
+                      const void *const *R, const std::vector<double> &c,
+                      int64_t No, int64_t Nv, void *out, void *partials) {
+  const int count = static_cast<int>(c.size());
+  CC4S_ASSERT(count >= 1 && count <= kMaxDiis, "too many DIIS vectors");
+  DotArgs a;
+  for (int k = 0; k < count; ++k) {
+    a.A[k] = A[k];
+    a.R[k] = R[k];
+    a.c[k] = c[k];
+  }
+  double2 *part = static_cast<double2 *>(partials);
+  double2 *res = part + kReduceBlocks;
+  if (f == Field::Complex)
+    combine_packed_kernel<true><<<kReduceBlocks, kBlock, 0, s>>>(
+        a, count, static_cast<double2 *>(out), No, Nv, part);
+  else
+    combine_packed_kernel<false><<<kReduceBlocks, kBlock, 0, s>>>(
+        a, count, static_cast<double *>(out), No, Nv, part);
+  CC4S_HIP(hipGetLastError());
+  finish_kernel<<<1, kBlock, 0, s>>>(part, 1, kReduceBlocks, res);
+  CC4S_HIP(hipGetLastError());
+  double2 h;
+  CC4S_HIP(hipMemcpyAsync(&h, res, sizeof(h), hipMemcpyDeviceToHost, s));
+  CC4S_HIP(hipStreamSynchronize(s));
+  return h.x;
+}
+
+namespace {
+
+// per-block count of (G,p,q), p <= q, with Gamma[G,q,p] != conj Gamma[G,p,q]
+__global__ __launch_bounds__(kBlock) void hermitian_kernel(const double2 *g,
+                                                           int64_t NG, int64_t Np,
+                                                           double2 *partials) {
+  __shared__ double2 sh[kBlock];
+  const int64_t n = NG * Np * Np;
+  double bad = 0.0;
+  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kBlock) {
+    const int64_t G = e % NG;
+    const int64_t r = e / NG;
+    const int64_t p = r % Np, q = r / Np;
+    if (p > q) continue;
+    const double2 a = g[e];
+    const double2 b = g[G + NG * (q + Np * p)];
+    if (!(b.x == a.x && b.y == -a.y)) bad += 1.0;
+  }
+  const double2 s = block_sum(make_double2(bad, 0.0), sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+}  // namespace
+
+bool is_hermitian(hipStream_t s, const void *gamma, int64_t NG, int64_t Np) {
+  DeviceBuffer part((kReduceBlocks + 1) * sizeof(double2));
+  double2 *p = part.as<double2>();
+  hermitian_kernel<<<kReduceBlocks, kBlock, 0, s>>>(
+      static_cast<const double2 *>(gamma), NG, Np, p);
+  CC4S_HIP(hipGetLastError());
+  finish_kernel<<<1, kBlock, 0, s>>>(p, 1, kReduceBlocks, p + kReduceBlocks);
+  CC4S_HIP(hipGetLastError());
+  double2 h;
+  CC4S_HIP(hipMemcpyAsync(&h, p + kReduceBlocks, sizeof(h), hipMemcpyDeviceToHost, s));
+  CC4S_HIP(hipStreamSynchronize(s));
+  return h.x == 0.0;
+}
+
 void copy(hipStream_t s, void *y, const void *x, size_t bytes) {
   CC4S_HIP(hipMemcpyAsync(y, x, bytes, hipMemcpyDeviceToDevice, s));
 }

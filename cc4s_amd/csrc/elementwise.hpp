@@ -44,6 +44,41 @@ double combine(hipStream_t s, Field f, const void *const *A,
                const void *const *R, const std::vector<double> &c,
                int64_t n, void *out, void *partials);
 
+// --- pair-packed amplitude sets ---------------------------------------------
+// A packed set stores T1 (Nv*No elements) followed by T2 for the pairs
+// P = a + Nv*i <= Q = b + Nv*j only, at Q*(Q+1)/2 + P; T2_abij = T2_baji
+// makes the other half redundant.  Used for the DIIS history.
+int64_t packed_set_size(int64_t No, int64_t Nv);
+
+// amplitude_update writing packed T' and dT (reads R, T in full layout)
+void amplitude_update_packed(hipStream_t s, Field f, const void *R, const void *T,
+                             const double *eps_h, const double *eps_p,
+                             double shift, void *Tnew, void *dT, int64_t No,
+                             int64_t Nv);
+// <x_k|y> of packed sets, equal to the full-layout product
+void multi_dot_packed(hipStream_t s, Field f, const void *const *x, int count,
+                      const void *y, int64_t No, int64_t Nv, void *partials,
+                      std::vector<cplx> &out);
+// out (full layout) = sum_k c_k A_k (packed); returns <r|r>, r = sum_k c_k R_k
+double combine_packed(hipStream_t s, Field f, const void *const *A,
+                      const void *const *R, const std::vector<double> &c,
+                      int64_t No, int64_t Nv, void *out, void *partials);
+
+// out (full layout) (first ? = : +=) c * x for the packed elements
+// [t0, t0 + len) of one set, x = chunk[0 .. len); writes both mirror
+// positions of a pair
+void unpack_accumulate(hipStream_t s, Field f, const void *chunk, int64_t t0,
+                       int64_t len, double c, bool first, void *out, int64_t No,
+                       int64_t Nv);
+// |sum_k c_k R_k|^2 of packed sets (the full-layout norm)
+double norm_packed(hipStream_t s, Field f, const void *const *R,
+                   const std::vector<double> &c, int64_t No, int64_t Nv,
+                   void *partials);
+
+// true when the complex vertex [NG, Np, Np] satisfies
+// Gamma[G,q,p] == conj(Gamma[G,p,q]) exactly for every G, p, q
+bool is_hermitian(hipStream_t s, const void *gamma, int64_t NG, int64_t Np);
+
 // y = x (device copy on the stream)
 void copy(hipStream_t s, void *y, const void *x, size_t bytes);
 

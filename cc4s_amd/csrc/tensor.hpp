@@ -64,6 +64,31 @@ class DeviceBuffer {
   size_t bytes_ = 0;
 };
 
+// pinned host memory (hipHostMalloc), for data that does not fit in HBM
+class HostBuffer {
+ public:
+  HostBuffer() = default;
+  explicit HostBuffer(size_t bytes) {
+    if (bytes) CC4S_HIP(hipHostMalloc(&p_, bytes, hipHostMallocDefault));
+    bytes_ = bytes;
+  }
+  ~HostBuffer() {
+    if (p_) (void)hipHostFree(p_);
+  }
+  HostBuffer(const HostBuffer &) = delete;
+  HostBuffer &operator=(const HostBuffer &) = delete;
+  HostBuffer(HostBuffer &&o) noexcept : p_(o.p_), bytes_(o.bytes_) {
+    o.p_ = nullptr;
+    o.bytes_ = 0;
+  }
+  void *get() const { return p_; }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void *p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
 struct View {
   void *ptr = nullptr;           // element (0,...,0)
   Field field = Field::Complex;  // element = double or (re, im) double pair

@@ -59,6 +59,35 @@ def test_residuum_parity(field, shape):
     cc.close()
 
 
+def test_residuum_parity_non_hermitian_vertex():
+    """A vertex without Gamma_qp = conj Gamma_pq: V_hhpp is then formed and
+    stored instead of being read as the conjugate transpose of V_pphh."""
+    No, Nv, NG = 4, 12, 30
+    inp = syn.generate(No, Nv, NG, "complex", 21)
+    rng = np.random.default_rng(4)
+    g = inp.gamma + 0.003 * (rng.standard_normal(inp.gamma.shape)
+                             + 1j * rng.standard_normal(inp.gamma.shape))
+    cc = api.CoupledCluster(No, Nv, NG, field="complex", integralsSliceSize=5)
+    cc.set_inputs(g, inp.eps)
+    P = O.CcsdProblem(g, inp.eps, No, "complex", integrals_slice_size=5)
+    T = _amps(No, Nv, "complex", 13)
+    R1, R2 = cc.residuum(T)
+    Q1, Q2 = P.residuum(T)
+    assert rel_err(R1, Q1) < R_TOL
+    assert rel_err(R2, Q2) < R_TOL
+    cc.close()
+
+
+@pytest.mark.parametrize("field", ["complex", "real"])
+def test_diis_amplitudes_on_host(field, monkeypatch):
+    """DIIS amplitude slots in pinned host memory give the same run."""
+    monkeypatch.setenv("CC4S_DIIS_PLACEMENT", "host")
+    kw = dict(maxIterations=30, energyConvergence=1e-10,
+              amplitudesConvergence=1e-8, maxResidua=5)
+    gpu, T_gpu, ref = _run_both(4, 14, 25, field, 17, **kw)
+    _check_run(gpu, T_gpu, ref)
+
+
 @pytest.mark.parametrize("field", ["complex", "real"])
 def test_energy_parity(field):
     No, Nv, NG = 5, 16, 40
