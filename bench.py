@@ -1,12 +1,24 @@
-"""Benchmark: seconds per CCSD iteration on one MI355X.
+"""Benchmark: seconds per CCSD iteration on one MI355X, and mixed vs FP64
+time to convergence.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
-                [--no-cpu-baseline]
+                [--scale mp2_per_occupied|max_v] [--converge both|fp64|mixed|none]
+                [--mixed-switch 1e-5] [--no-cpu-baseline]
 
 A step is one pass of the CCSD iteration loop (residuum, denominator update,
 DIIS, energy) on the seeded synthetic workload of the chosen configuration,
-with the Coulomb vertex already resident in HBM.  The first W passes are
-untimed (pass 1 has no residuum contraction: R = V).  Prints one JSON line.
+with the Coulomb vertex already resident in HBM.  The FP64 context does W
+untimed passes (pass 1 has no residuum contraction: R = V), then K timed
+passes: `value` is their mean wall time.  With --converge the FP64 context
+keeps iterating to the convergence criteria, and a second context runs the
+mixed-precision path (split-bf16 contractions until dR < mixed switch, then
+FP64) to the same criteria; both times to convergence and |E_mixed - E_FP64|
+are reported.  Prints one JSON line.
+
+Input scale: "mp2_per_occupied" (default) scales the vertex so that the MP2
+energy per occupied orbital equals that of the config-0 workload; "max_v" is
+max|V_abij| = 0.05 Eh, for which the iteration diverges beyond config 0.
+Per-pass cost does not depend on the scale.
 """
 
 from __future__ import annotations
@@ -26,7 +38,6 @@ from cc4s_amd import api  # noqa: E402
 from cc4s_amd import synthetic as syn  # noqa: E402
 
 FP64_MFMA_PEAK_TFLOPS = 78.6      # MI355X dense FP64 matrix
-HBM_PEAK_GBS = 8000.0
 METRIC = ("s per CCSD iteration + % MFMA peak, mixed vs FP64 GPU at "
           "|ΔE|≤1 µEh")
 VARIANTS = [f"gett_{t}_{a}_{b}" for t in ("c64", "f64") for a in ("ak", "am")
@@ -35,6 +46,15 @@ VARIANTS = [f"gett_{t}_{a}_{b}" for t in ("c64", "f64") for a in ("ak", "am")
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def gpu_mp2(gamma, eps, No, field):
+    """Second-order energy of a vertex from the first pass of the GPU path
+    (R = V, T = -V/Delta; the energy of that pass is the closed form)."""
+    NG, Np, _ = gamma.shape
+    with api.CoupledCluster(No, Np - No, NG, field=field) as cc:
+        cc.set_inputs(gamma, eps)
+        return cc.iterate()["energy"]
 
 
 def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
@@ -62,6 +82,7 @@ def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
         pairs = 0
         t0 = time.perf_counter()
         ns = (Nv + s - 1) // s
+        done = False
         for m in range(ns):
             for n in range(m, ns):
                 a, b = n * s, m * s
@@ -69,9 +90,10 @@ def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
                 O.ein("xycd,cdij->xyij", V, X)
                 flops += cmul * (cfg.NG * s * s * Nv * Nv + s * s * Nv * Nv * No * No)
                 pairs += 1
-                if time.perf_counter() - t0 > seconds_budget:
+                done = time.perf_counter() - t0 > seconds_budget
+                if done:
                     break
-            if time.perf_counter() - t0 > seconds_budget:
+            if done:
                 break
         dt = time.perf_counter() - t0
     finally:
@@ -90,12 +112,39 @@ def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
     }
 
 
+def make_context(cfg, precision, switch):
+    return api.CoupledCluster(cfg.No, cfg.Nv, cfg.NG, field=cfg.field,
+                              maxIterations=cfg.max_iterations,
+                              energyConvergence=cfg.energy_convergence,
+                              amplitudesConvergence=cfg.amplitudes_convergence,
+                              maxResidua=cfg.max_residua, precision=precision,
+                              mixedSwitch=switch)
+
+
+def converge(cc, label, first_passes):
+    """Iterates to convergence; returns (passes, seconds, energy, converged)."""
+    passes = list(first_passes)
+    while not (passes and passes[-1]["converged"]) and \
+            len(passes) < cc.params.max_iterations:
+        r = cc.iterate()
+        passes.append(r)
+        log(f"{label} {r['iteration']:3d} {'mixed' if r['precision_used'] else 'fp64 '} "
+            f"E={r['energy']:.12f} dE={r['delta_energy']:.3e} "
+            f"dR={r['residuum_norm']:.3e} {r['seconds']:.3f} s")
+    return passes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--scale", default="mp2_per_occupied",
+                    choices=("mp2_per_occupied", "max_v"))
+    ap.add_argument("--converge", default="both",
+                    choices=("both", "fp64", "mixed", "none"))
+    ap.add_argument("--mixed-switch", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -103,73 +152,110 @@ def main():
                           "--gpus must be 1"}))
         sys.exit(2)
     cfg = syn.CONFIGS[args.config]
-    log(f"generating inputs: {cfg.name}")
+    log(f"generating inputs: {cfg.name}, scale rule {args.scale}")
     t = time.perf_counter()
-    inp = syn.generate_config(cfg)
+    big = cfg.No * cfg.Nv > 16384
+    inp = syn.generate_config(cfg, scale_rule=args.scale,
+                              mp2_fn=gpu_mp2 if big else None)
     log(f"  {time.perf_counter() - t:.1f} s, scale c = {inp.scale:.6e}")
 
-    cc = api.CoupledCluster(cfg.No, cfg.Nv, cfg.NG, field=cfg.field,
-                            maxIterations=cfg.max_iterations,
-                            energyConvergence=cfg.energy_convergence,
-                            amplitudesConvergence=cfg.amplitudes_convergence,
-                            maxResidua=cfg.max_residua)
-    t = time.perf_counter()
-    cc.set_inputs(inp.gamma, inp.eps)
-    log(f"inputs resident, V slices formed: {time.perf_counter() - t:.2f} s")
-    for w in range(args.warmup):
-        r = cc.iterate()
-        log(f"warmup {r['iteration']}: E={r['energy']:.12f} dR={r['residuum_norm']:.3e} "
-            f"{r['seconds']:.3f} s")
-    cc.set_profiling(True)
+    out_conv = {}
     reps = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        r = cc.iterate()
-        reps.append(r)
-        log(f"step {r['iteration']}: E={r['energy']:.12f} dE={r['delta_energy']:.3e} "
-            f"dR={r['residuum_norm']:.3e} {r['seconds']:.3f} s "
-            f"{r['flops'] / r['seconds'] / 1e12:.2f} TFLOP/s")
-    elapsed = time.perf_counter() - t0
-    s_per_iter = elapsed / args.steps
-    iter_flops = float(np.mean([r["flops"] for r in reps]))
-
-    stats = {v: cc.kernel_stats(v) for v in VARIANTS}
-    dom = max(stats, key=lambda v: stats[v]["ms"])
-    d = stats[dom]
-    avg_ms = d["ms"] / max(1, d["launches"])
-    achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
-    lad_ms, lad_flops, lad_calls = cc.ladder_stats()
-    log(f"kernel stats: {json.dumps(stats)}")
-    log(f"ladder: {lad_calls} calls, {lad_ms:.1f} ms, "
-        f"{lad_flops / (lad_ms * 1e-3) / 1e12 if lad_ms else 0:.2f} TFLOP/s")
-    cc.close()
+    stats = {}
+    lad = (0.0, 0.0, 0)
+    value = None
+    iter_flops = None
+    if args.converge in ("both", "fp64", "none"):
+        cc = make_context(cfg, api.PRECISION_FP64, args.mixed_switch)
+        t = time.perf_counter()
+        cc.set_inputs(inp.gamma, inp.eps)
+        log(f"inputs resident, V slices formed: {time.perf_counter() - t:.2f} s")
+        passes = []
+        for w in range(args.warmup):
+            r = cc.iterate()
+            passes.append(r)
+            log(f"warmup {r['iteration']}: E={r['energy']:.12f} "
+                f"dR={r['residuum_norm']:.3e} {r['seconds']:.3f} s")
+        cc.set_profiling(True)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            r = cc.iterate()
+            reps.append(r)
+            passes.append(r)
+            log(f"step {r['iteration']}: E={r['energy']:.12f} dE={r['delta_energy']:.3e} "
+                f"dR={r['residuum_norm']:.3e} {r['seconds']:.3f} s "
+                f"{r['flops'] / r['seconds'] / 1e12:.2f} TFLOP/s")
+        value = (time.perf_counter() - t0) / args.steps
+        iter_flops = float(np.mean([r["flops"] for r in reps]))
+        stats = {v: cc.kernel_stats(v) for v in VARIANTS}
+        lad = cc.ladder_stats()
+        cc.set_profiling(False)
+        if args.converge in ("both", "fp64"):
+            passes = converge(cc, "fp64", passes)
+            out_conv["fp64"] = {
+                "iterations": len(passes),
+                "seconds": float(sum(p["seconds"] for p in passes)),
+                "energy": passes[-1]["energy"],
+                "converged": bool(passes[-1]["converged"]),
+            }
+        cc.close()
+    if args.converge in ("both", "mixed"):
+        cc = make_context(cfg, api.PRECISION_MIXED, args.mixed_switch)
+        cc.set_inputs(inp.gamma, inp.eps)
+        passes = converge(cc, "mixed", [])
+        mixed_times = [p["seconds"] for p in passes[1:] if p["precision_used"]]
+        out_conv["mixed"] = {
+            "iterations": len(passes),
+            "mixed_iterations": sum(1 for p in passes if p["precision_used"]),
+            "seconds": float(sum(p["seconds"] for p in passes)),
+            "seconds_per_mixed_iteration": (float(np.mean(mixed_times))
+                                            if mixed_times else None),
+            "energy": passes[-1]["energy"],
+            "converged": bool(passes[-1]["converged"]),
+            "switch": args.mixed_switch,
+        }
+        cc.close()
+    if "fp64" in out_conv and "mixed" in out_conv:
+        out_conv["speedup_time_to_convergence"] = (
+            out_conv["fp64"]["seconds"] / out_conv["mixed"]["seconds"])
+        out_conv["abs_energy_difference"] = abs(
+            out_conv["fp64"]["energy"] - out_conv["mixed"]["energy"])
 
     out = {
         "metric": METRIC,
-        "value": s_per_iter,
+        "value": value,
         "unit": "s/iteration",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * s_per_iter,
+        "ms_per_step": 1e3 * value if value is not None else None,
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded SplitMix64/Box-Muller vertex, cc4s layout)",
+        "data": (f"synthetic (seeded SplitMix64/Box-Muller vertex, cc4s layout, "
+                 f"scale rule {args.scale})"),
         "config": {
             "workload": f"FP64 CCSD iteration, {cfg.name}, DIIS {cfg.max_residua}",
             "No": cfg.No, "Nv": cfg.Nv, "NG": cfg.NG, "field": cfg.field,
             "precision": "fp64", "parallelism": "1 GPU",
+            "time_to_convergence": out_conv,
+        },
+    }
+    if value is not None:
+        dom = max(stats, key=lambda v: stats[v]["ms"])
+        d = stats[dom]
+        achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
+        lad_ms, lad_flops, _ = lad
+        out["config"].update({
             "flops_per_iteration": iter_flops,
-            "tflops_per_s": iter_flops / s_per_iter / 1e12,
-            "frac_fp64_mfma_peak_iteration": iter_flops / s_per_iter / 1e12
+            "tflops_per_s": iter_flops / value / 1e12,
+            "frac_fp64_mfma_peak_iteration": iter_flops / value / 1e12
             / FP64_MFMA_PEAK_TFLOPS,
             "ladder_tflops_per_s": (lad_flops / (lad_ms * 1e-3) / 1e12
                                     if lad_ms else None),
-            "energy_last_step": reps[-1]["energy"],
-        },
-        "roofline": {
+        })
+        out["roofline"] = {
             "kernel": dom,
             "bound": "mfma",
             "achieved": achieved,
@@ -178,13 +264,13 @@ def main():
             "frac": achieved / FP64_MFMA_PEAK_TFLOPS,
             "traffic": None,
             "launches": d["launches"],
-            "avg_launch_ms": avg_ms,
+            "avg_launch_ms": d["ms"] / max(1, d["launches"]),
             "flops_per_launch": d["flops"] / max(1, d["launches"]),
-        },
-    }
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle sample) ...")
-        out["cpu_baseline"] = cpu_baseline(cfg, inp, iter_flops)
+        }
+        log(f"kernel stats: {json.dumps(stats)}")
+        if not args.no_cpu_baseline:
+            log("cpu baseline (oracle sample) ...")
+            out["cpu_baseline"] = cpu_baseline(cfg, inp, iter_flops)
     print(json.dumps(out), flush=True)
 
 

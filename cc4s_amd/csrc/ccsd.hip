@@ -69,7 +69,6 @@ Ccsd::Ccsd(ContractionEngine &E, const CcsdShape &shape, const CcsdOptions &opt,
   w.Woo.reset(f, "ki", {o, o});
   if (f == Field::Complex) {
     w.Y.reset(f, "abij", {v, v, o, o});
-    w.Dpp.reset(f, "Gab", {G, v, v});
     w.Zgoo.reset(f, "Gkj", {G, o, o});
   }
 }
@@ -143,6 +142,8 @@ void Ccsd::form_integrals() {
     } else {
       make(w.Vhhpp, "hhpp");
       w.Vhhpp_view = w.Vhhpp.v;
+      // the two dressed pp vertices differ only for a non-Hermitian vertex
+      w.Dpp.reset(f, "Gab", {s_.NG, v, v});
     }
   }
   CC4S_HIP(hipStreamSynchronize(E_.stream()));
@@ -302,9 +303,15 @@ void Ccsd::residuum_complex(const void *Tp, void *Rp) {
   C(1.0, Vhhpp("klcd"), X("cdij"), 0.0, w.Xklij("klij"));       // :541
   C(1.0, w.Xklij("klij"), opt_.ppl ? T2.as("abkl") : X("abkl"), 1.0, R2);  // :542-546
   if (opt_.ppl) {
-    S(1.0, gam("pp", "Gab"), 0.0, w.Dpp("Gab"));                // :560
-    C(-1.0, gam("hp", "Gkb"), T1.as("ak"), 1.0, w.Dpp("Gab"));  // :561
-    ladder(w.DGpp.v, w.Dpp.v, X.v, R2);                         // :564-600
+    if (w.hermitian) {
+      // with Gamma_qp = conj Gamma_pq exactly, Gamma_pp[G,a,b] - Gamma_hp[G,k,b]
+      // T_ak (:560-561) equals cTDGamma_pp[G,a,b] of :509-510 term by term
+      ladder(w.DGpp.v, w.DGpp.v, X.v, R2);                      // :564-600
+    } else {
+      S(1.0, gam("pp", "Gab"), 0.0, w.Dpp("Gab"));              // :560
+      C(-1.0, gam("hp", "Gkb"), T1.as("ak"), 1.0, w.Dpp("Gab"));  // :561
+      ladder(w.DGpp.v, w.Dpp.v, X.v, R2);                       // :564-600
+    }
   }
   C(1.0, w.Kac("ac"), T1.as("ci"), 0.0, R1);                    // :609
   C(-1.0, w.Kki("ki"), T1.as("ak"), 1.0, R1);                   // :610

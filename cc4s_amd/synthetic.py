@@ -229,7 +229,7 @@ class CcsdInputs:
 
 def generate(No: int, Nv: int, NG: int, field: str, seed: int,
              target_max_v: float = TARGET_MAX_V,
-             scale_rule: str = "max_v") -> CcsdInputs:
+             scale_rule: str = "max_v", mp2_fn=None) -> CcsdInputs:
     """Seeded inputs for one workload (see module docstring).
 
     scale_rule "max_v" (the workload definition): c such that
@@ -237,7 +237,10 @@ def generate(No: int, Nv: int, NG: int, field: str, seed: int,
     the second-order energy per occupied orbital equals that of the config-0
     workload (E2 is proportional to c^4); at No, Nv beyond config 0 the max_v
     rule gives couplings for which the CCSD iteration does not converge, this
-    rule keeps the correlation strength per electron fixed."""
+    rule keeps the correlation strength per electron fixed.  `mp2_fn(gamma,
+    eps, No, field)` may replace the host computation of the second-order
+    energy of the unscaled vertex (for sizes where the pphh slice does not fit
+    in host memory comfortably)."""
     if field not in ("real", "complex"):
         raise ValueError("field must be 'real' or 'complex'")
     Np = No + Nv
@@ -247,7 +250,7 @@ def generate(No: int, Nv: int, NG: int, field: str, seed: int,
         vmax = max_abs_vpphh(gamma, No, field)
         c = float(np.sqrt(target_max_v / vmax))
     elif scale_rule == "mp2_per_occupied":
-        e2 = mp2_of_vertex(gamma, eps, No, field)
+        e2 = (mp2_fn or mp2_of_vertex)(gamma, eps, No, field)
         c = float((MP2_PER_OCCUPIED * No / e2) ** 0.25)
     else:
         raise ValueError(f"unknown scale rule {scale_rule!r}")
@@ -255,9 +258,10 @@ def generate(No: int, Nv: int, NG: int, field: str, seed: int,
     return CcsdInputs(gamma=gamma, eps=eps, No=No, field=field, scale=c)
 
 
-def generate_config(cfg: CcsdConfig, scale_rule: str = "max_v") -> CcsdInputs:
+def generate_config(cfg: CcsdConfig, scale_rule: str = "max_v",
+                    mp2_fn=None) -> CcsdInputs:
     return generate(cfg.No, cfg.Nv, cfg.NG, cfg.field, cfg.seed,
-                    scale_rule=scale_rule)
+                    scale_rule=scale_rule, mp2_fn=mp2_fn)
 
 
 def random_amplitudes(No: int, Nv: int, field: str, seed: int,
