@@ -190,6 +190,9 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
     CC4S_HIP(hipEventRecord(e0, E_.stream()));
   }
   const double flops0 = E_.stats.flops;
+  // tau is read unchanged by every slab contraction: in split-bf16 mode it
+  // is packed once for all of them
+  if (E_.mixed()) E_.set_operand_cache(tauv.ptr);
   for (int64_t m = 0; m < ns; ++m) {
     for (int64_t n = m; n < ns; ++n) {
       const int64_t a = n * s, b = m * s;
@@ -211,6 +214,7 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
       }
     }
   }
+  E_.clear_operand_cache();
   if (profile) {
     CC4S_HIP(hipEventRecord(e1, E_.stream()));
     CC4S_HIP(hipEventSynchronize(e1));

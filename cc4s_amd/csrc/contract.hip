@@ -943,10 +943,31 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
   const int64_t Mp = (P.M + T.BM - 1) / T.BM * T.BM;
   const int64_t Np = (P.N + T.BN - 1) / T.BN * T.BN;
   const int64_t Kfull = (P.K + T.BK - 1) / T.BK * T.BK;
-  int64_t Kc = static_cast<int64_t>(pack_budget / (static_cast<size_t>(Mp + Np) * 8));
+  const View &Aop = P.swap ? B : A;
+  const View &Bop = P.swap ? A : B;
+  // an operand marked constant is packed once over the full K range
+  const bool cacheA = cache_ptr_ && Aop.ptr == cache_ptr_;
+  const bool cacheB = !cacheA && cache_ptr_ && Bop.ptr == cache_ptr_;
+  if (cacheA || cacheB) {
+    const int64_t rows = cacheA ? Mp : Np;
+    if (cache_plan_ != &P || cache_is_a_ != cacheA) {
+      cache_.reserve(static_cast<size_t>(rows) * Kfull * 8);
+      if (cacheA)
+        launch_pack(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K, 0,
+                    Kfull, Mp, Aop.conj, P.a_kc, cache_.get(), stream_);
+      else
+        launch_pack(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K, 0,
+                    Kfull, Np, Bop.conj, P.b_kc, cache_.get(), stream_);
+      cache_plan_ = &P;
+      cache_is_a_ = cacheA;
+    }
+  }
+  const int64_t packed_rows = (cacheA ? 0 : Mp) + (cacheB ? 0 : Np);
+  int64_t Kc = packed_rows > 0
+                   ? static_cast<int64_t>(pack_budget / (static_cast<size_t>(packed_rows) * 8))
+                   : Kfull;
   Kc = std::max<int64_t>(T.BK, Kc / T.BK * T.BK);
   Kc = std::min(Kc, Kfull);
-  const int64_t chunks = (P.K + Kc - 1) / Kc;
   // split K inside a chunk when the tile grid cannot fill the CUs twice
   const int64_t tiles = (Mp / T.BM) * (Np / T.BN);
   int splits = 1;
@@ -955,16 +976,17 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
     while (sp > 1 && static_cast<size_t>(sp) * P.M * P.N * 16 > max_workspace) --sp;
     splits = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(sp, 256)));
   }
-  pack_.reserve(static_cast<size_t>(Mp + Np) * Kc * 8);
+  // no FP32 sum (one split of one chunk) runs over more than mixed_max_k
+  Kc = std::min<int64_t>(Kc, std::max<int64_t>(T.BK, mixed_max_k * splits / T.BK * T.BK));
+  const int64_t chunks = (P.K + Kc - 1) / Kc;
+  pack_.reserve(static_cast<size_t>(packed_rows) * Kc * 8);
   char *Apk = pack_.as<char>();
-  char *Bpk = Apk + static_cast<size_t>(Mp) * Kc * 8;
+  char *Bpk = Apk + (cacheA ? 0 : static_cast<size_t>(Mp) * Kc * 8);
   double *W = nullptr;
   if (splits > 1) {
     workspace_.reserve(static_cast<size_t>(splits) * P.M * P.N * 16);
     W = workspace_.as<double>();
   }
-  const View &Aop = P.swap ? B : A;
-  const View &Bop = P.swap ? A : B;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (profiling_) {
     CC4S_HIP(hipEventCreate(&e0));
@@ -974,12 +996,20 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
   for (int64_t c = 0; c < chunks; ++c) {
     const int64_t k0 = c * Kc;
     const int64_t kc = std::min(Kc, (P.K - k0 + T.BK - 1) / T.BK * T.BK);
-    launch_pack(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K, k0,
-                kc, Mp, Aop.conj, P.a_kc, Apk, stream_);
-    launch_pack(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K, k0,
-                kc, Np, Bop.conj, P.b_kc, Bpk, stream_);
+    const void *Ause = cacheA ? static_cast<const void *>(cache_.as<char>() +
+                                                          static_cast<size_t>(k0) * Mp * 8)
+                              : Apk;
+    const void *Buse = cacheB ? static_cast<const void *>(cache_.as<char>() +
+                                                          static_cast<size_t>(k0) * Np * 8)
+                              : Bpk;
+    if (!cacheA)
+      launch_pack(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K, k0,
+                  kc, Mp, Aop.conj, P.a_kc, Apk, stream_);
+    if (!cacheB)
+      launch_pack(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K, k0,
+                  kc, Np, Bop.conj, P.b_kc, Bpk, stream_);
     const double bc = c == 0 ? beta : 1.0;
-    launch_bgemm(Apk, Bpk, static_cast<double *>(C.ptr), W, P.rowC, P.colC, P.M,
+    launch_bgemm(Ause, Buse, static_cast<double *>(C.ptr), W, P.rowC, P.colC, P.M,
                  P.N, Mp, Np, kc, splits, alpha, bc, stream_);
     if (splits > 1)
       launch_splitk_reduce(Field::Complex, W, static_cast<double *>(C.ptr), P.rowC,

@@ -66,6 +66,21 @@ class ContractionEngine {
   bool mixed() const { return mixed_; }
   double mixed_min_flops = 1e9;
   size_t pack_budget = size_t(1) << 30;  // bytes of packed operands per chunk
+  // longest FP32 sum of the split-bf16 kernel (K is chunked beyond it and
+  // the chunk results are added in FP64)
+  int64_t mixed_max_k = 16384;
+  // While set, the split-bf16 packing of an operand whose memory starts at
+  // `ptr` is kept for the whole K range and reused by later contractions of
+  // the same plan; the caller guarantees that the memory does not change
+  // until clear_operand_cache() (the ladder's tau across its slab pairs).
+  void set_operand_cache(const void *ptr) {
+    cache_ptr_ = ptr;
+    cache_plan_ = nullptr;
+  }
+  void clear_operand_cache() {
+    cache_ptr_ = nullptr;
+    cache_plan_ = nullptr;
+  }
 
  private:
   GettPlan &plan_for(const View &A, const View &B, const View &C);
@@ -73,6 +88,10 @@ class ContractionEngine {
                       double beta, const View &C, double flops);
   bool mixed_ = false;
   DeviceBuffer pack_;
+  DeviceBuffer cache_;
+  const void *cache_ptr_ = nullptr;
+  const GettPlan *cache_plan_ = nullptr;
+  bool cache_is_a_ = true;
   struct Pending {
     hipEvent_t start, stop;
     std::string name;

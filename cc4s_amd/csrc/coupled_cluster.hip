@@ -191,8 +191,13 @@ void CoupledCluster::create_mixer() {
   size_t free_b = 0, total_b = 0;
   CC4S_HIP(hipMemGetInfo(&free_b, &total_b));
   // room kept for buffers allocated on first use (ladder slab, operand
-  // packs, split-K partials, residuum scratch)
-  const size_t reserve = (size_t(6) << 30) + s.nset() * scalar_bytes(s.field);
+  // packs, split-K partials, residuum scratch; in mixed mode the packed tau
+  // of the ladder, Nv^2 x No^2 rounded to the kernel tile, 8 bytes each)
+  size_t reserve = (size_t(6) << 30) + s.nset() * scalar_bytes(s.field);
+  if (p_.mixed) {
+    const int64_t rows = (s.No * s.No + 127) / 128 * 128;
+    reserve += static_cast<size_t>(s.Nv) * s.Nv * rows * 8;
+  }
   const size_t avail = free_b > reserve ? free_b - reserve : 0;
   const char *env = std::getenv("CC4S_DIIS_PLACEMENT");
   const std::string placement = env ? env : "auto";
