@@ -3,7 +3,7 @@ time to convergence.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
                 [--scale mp2_per_occupied|max_v] [--converge both|fp64|mixed|none]
-                [--mixed-switch 1e-5] [--no-cpu-baseline]
+                [--mixed-switch 1e-5] [--no-cpu-baseline] [--adjoint auto|on|off]
 
 A step is one pass of the CCSD iteration loop (residuum, denominator update,
 DIIS, energy) on the seeded synthetic workload of the chosen configuration,
@@ -13,7 +13,10 @@ passes: `value` is their mean wall time.  With --converge the FP64 context
 keeps iterating to the convergence criteria, and a second context runs the
 mixed-precision path (split-bf16 contractions until dR < mixed switch, then
 FP64) to the same criteria; both times to convergence and |E_mixed - E_FP64|
-are reported.  Prints one JSON line.
+are reported.  At the FP64 fixed point of the No=32, Nv=256, NG=1500
+workload the energy adjoint (gradient with respect to the vertex and the
+structure factor S(G), FP64 and FP32) is timed and checked against central
+differences of the forward energy.  Prints one JSON line.
 
 Input scale: "mp2_per_occupied" (default) scales the vertex so that the MP2
 energy per occupied orbital equals that of the config-0 workload; "max_v" is
@@ -112,13 +115,13 @@ def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
     }
 
 
-def make_context(cfg, precision, switch):
+def make_context(cfg, precision, switch, refinement=1):
     return api.CoupledCluster(cfg.No, cfg.Nv, cfg.NG, field=cfg.field,
                               maxIterations=cfg.max_iterations,
                               energyConvergence=cfg.energy_convergence,
                               amplitudesConvergence=cfg.amplitudes_convergence,
                               maxResidua=cfg.max_residua, precision=precision,
-                              mixedSwitch=switch)
+                              mixedSwitch=switch, refinement=refinement)
 
 
 def converge(cc, label, first_passes):
@@ -134,6 +137,49 @@ def converge(cc, label, first_passes):
     return passes
 
 
+def adjoint_report(cc, gamma, energy, n_fd=20, h=1e-5, seed=2024):
+    """Energy adjoint at the converged amplitudes: forward energy, FP64 and
+    FP32 backward (gradient + structure factor S(G)), sum_G S(G) against E,
+    FP32 against FP64, and central differences of the forward energy along
+    a scaling of one G slice by 1 +- h at `n_fd` seeded G (E(1+t) =
+    E_rest + (1+t)^2 S(G)), relative to max |S|."""
+    t = time.perf_counter()
+    e_fwd = cc.energy_forward()
+    t_fwd = time.perf_counter() - t
+    t = time.perf_counter()
+    grad, S = cc.energy_backward()
+    t64 = time.perf_counter() - t
+    t = time.perf_counter()
+    grad32, S32 = cc.energy_backward(precision=api.PRECISION_FP32)
+    t32 = time.perf_counter() - t
+    smax = float(np.max(np.abs(S)))
+    rng = np.random.default_rng(seed)
+    g = np.asfortranarray(gamma)
+    fd_err = 0.0
+    for G in rng.choice(g.shape[0], size=min(n_fd, g.shape[0]), replace=False):
+        sl = g[G].copy()
+        g[G] = sl * (1 + h)
+        ep = cc.energy_forward(g)
+        g[G] = sl * (1 - h)
+        em = cc.energy_forward(g)
+        g[G] = sl
+        fd_err = max(fd_err, abs((ep - em) / (4 * h) - S[G]) / smax)
+    scale = float(np.max(np.abs(grad)))
+    out = {
+        "energy_forward": e_fwd,
+        "forward_vs_iteration_energy": abs(e_fwd - energy),
+        "sum_S_minus_E_rel": abs(float(np.sum(S)) - e_fwd) / abs(e_fwd),
+        "fp32_grad_rel_err": float(np.max(np.abs(grad32 - grad))) / scale,
+        "fp32_S_rel_err": float(np.max(np.abs(S32 - S))) / smax,
+        "fd_S_rel_err_max": fd_err,
+        "fd_points": int(min(n_fd, g.shape[0])), "fd_step": h,
+        "seconds_forward": t_fwd, "seconds_backward_fp64": t64,
+        "seconds_backward_fp32": t32,
+    }
+    log(f"adjoint: {json.dumps(out)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,7 +191,13 @@ def main():
     ap.add_argument("--converge", default="both",
                     choices=("both", "fp64", "mixed", "none"))
     ap.add_argument("--mixed-switch", type=float, default=1e-5)
+    ap.add_argument("--refinement", type=int, default=1, choices=(0, 1),
+                    help="after the switch: 0 FP64 passes, 1 defect correction "
+                         "(FP64 anchors, corrected split-bf16 passes between)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adjoint", default="auto", choices=("auto", "on", "off"),
+                    help="energy adjoint at the FP64 fixed point (auto: on "
+                         "for the No=32, Nv=256, NG=1500 workloads)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; "
@@ -160,6 +212,7 @@ def main():
     log(f"  {time.perf_counter() - t:.1f} s, scale c = {inp.scale:.6e}")
 
     out_conv = {}
+    adjoint = None
     reps = []
     stats = {}
     lad = (0.0, 0.0, 0)
@@ -198,21 +251,32 @@ def main():
                 "energy": passes[-1]["energy"],
                 "converged": bool(passes[-1]["converged"]),
             }
+            want = args.adjoint == "on" or (args.adjoint == "auto"
+                                            and args.config in (1, 3))
+            if want and passes[-1]["converged"]:
+                adjoint = adjoint_report(cc, inp.gamma, passes[-1]["energy"])
         cc.close()
     if args.converge in ("both", "mixed"):
-        cc = make_context(cfg, api.PRECISION_MIXED, args.mixed_switch)
+        cc = make_context(cfg, api.PRECISION_MIXED, args.mixed_switch,
+                          args.refinement)
         cc.set_inputs(inp.gamma, inp.eps)
         passes = converge(cc, "mixed", [])
         mixed_times = [p["seconds"] for p in passes[1:] if p["precision_used"]]
+        fp64_times = [p["seconds"] for p in passes[1:] if not p["precision_used"]]
         out_conv["mixed"] = {
             "iterations": len(passes),
             "mixed_iterations": sum(1 for p in passes if p["precision_used"]),
+            "fp64_residuum_passes": len(fp64_times),
             "seconds": float(sum(p["seconds"] for p in passes)),
             "seconds_per_mixed_iteration": (float(np.mean(mixed_times))
                                             if mixed_times else None),
+            "seconds_per_fp64_pass": (float(np.mean(fp64_times))
+                                      if fp64_times else None),
             "energy": passes[-1]["energy"],
             "converged": bool(passes[-1]["converged"]),
             "switch": args.mixed_switch,
+            "refinement": ("defect correction" if args.refinement == 1
+                           else "FP64 passes"),
         }
         cc.close()
     if "fp64" in out_conv and "mixed" in out_conv:
@@ -242,6 +306,8 @@ def main():
             "time_to_convergence": out_conv,
         },
     }
+    if adjoint is not None:
+        out["config"]["adjoint"] = adjoint
     if value is not None:
         dom = max(stats, key=lambda v: stats[v]["ms"])
         d = stats[dom]

@@ -20,13 +20,16 @@ FIELD_REAL = 0
 FIELD_COMPLEX = 1
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+PRECISION_FP32 = 2
 
 # every symbol declared in include/cc4s_ccsd.h
 EXPORTS = (
     "cc4s_ccsd_default_params", "cc4s_ccsd_create", "cc4s_ccsd_set_inputs",
     "cc4s_ccsd_set_inputs_device", "cc4s_ccsd_set_amplitudes",
     "cc4s_ccsd_iterate", "cc4s_ccsd_run", "cc4s_ccsd_get_amplitudes",
-    "cc4s_ccsd_residuum", "cc4s_ccsd_energy", "cc4s_ccsd_set_profiling",
+    "cc4s_ccsd_residuum", "cc4s_ccsd_energy",
+    "cc4s_ccsd_energy_grad_forward", "cc4s_ccsd_energy_grad_backward",
+    "cc4s_ccsd_set_profiling",
     "cc4s_ccsd_ladder_stats", "cc4s_ccsd_kernel_stats",
     "cc4s_ccsd_last_error", "cc4s_ccsd_destroy",
     "cc4s_contract",
@@ -46,6 +49,8 @@ class Params(C.Structure):
         ("integrals_slice_size", C.c_int64),
         ("precision", C.c_int),
         ("mixed_switch", C.c_double),
+        ("refinement", C.c_int),
+        ("refinement_ratio", C.c_double),
         ("device", C.c_int),
     ]
 
@@ -104,6 +109,9 @@ def load_library(path: pathlib.Path | str | None = None):
     lib.cc4s_ccsd_get_amplitudes.argtypes = [ctxp, vp, vp]
     lib.cc4s_ccsd_residuum.argtypes = [ctxp, vp, vp, vp, vp]
     lib.cc4s_ccsd_energy.argtypes = [ctxp, vp, vp, dp]
+    lib.cc4s_ccsd_energy_grad_forward.argtypes = [ctxp, vp, C.c_int, dp]
+    lib.cc4s_ccsd_energy_grad_backward.argtypes = [ctxp, vp, C.c_int, vp, C.c_int,
+                                                   vp, C.c_int]
     lib.cc4s_ccsd_set_profiling.argtypes = [ctxp, C.c_int]
     lib.cc4s_ccsd_ladder_stats.argtypes = [ctxp, dp, dp, C.POINTER(C.c_int64)]
     lib.cc4s_ccsd_kernel_stats.argtypes = [ctxp, C.c_char_p, C.POINTER(C.c_int64),
@@ -150,7 +158,7 @@ class CoupledCluster:
                  energyConvergence=1e-7, amplitudesConvergence=1e-6,
                  levelShift=0.0, maxResidua=4, ppl=True,
                  integralsSliceSize=0, device=0, precision=PRECISION_FP64,
-                 mixedSwitch=1e-5):
+                 mixedSwitch=1e-5, refinement=None, refinementRatio=None):
         self.lib = load_library()
         p = default_params()
         p.field = FIELD_COMPLEX if field == "complex" else FIELD_REAL
@@ -165,6 +173,10 @@ class CoupledCluster:
         p.device = device
         p.precision = precision
         p.mixed_switch = mixedSwitch
+        if refinement is not None:
+            p.refinement = int(refinement)
+        if refinementRatio is not None:
+            p.refinement_ratio = refinementRatio
         self.params = p
         self.field = field
         self.No, self.Nv, self.NG = No, Nv, NG
@@ -256,6 +268,47 @@ class CoupledCluster:
         out = (C.c_double * 6)()
         self._check(self.lib.cc4s_ccsd_energy(self._ctx, _ptr(t1), _ptr(t2), out))
         return complex(out[0], out[1]), complex(out[2], out[3]), complex(out[4], out[5])
+
+    def _vertex_arg(self, gamma):
+        """(pointer, on_device, keep-alive) for a vertex argument: None (the
+        input vertex), a device address (int), or a host array."""
+        if gamma is None:
+            return None, 0, None
+        if isinstance(gamma, int):
+            return C.c_void_p(gamma), 1, None
+        g = _fortran(gamma, np.complex128)
+        Np = self.No + self.Nv
+        if g.shape != (self.NG, Np, Np):
+            raise ValueError(f"gamma shape {g.shape} != {(self.NG, Np, Np)}")
+        return _ptr(g), 0, g
+
+    def energy_forward(self, gamma=None):
+        """Correlation energy at the current amplitudes with the vertex
+        `gamma` (None: the input vertex; int: a device address)."""
+        ptr, dev, _keep = self._vertex_arg(gamma)
+        e = C.c_double()
+        self._check(self.lib.cc4s_ccsd_energy_grad_forward(self._ctx, ptr, dev,
+                                                           C.byref(e)))
+        return e.value
+
+    def energy_backward(self, gamma=None, grad=True, precision=PRECISION_FP64):
+        """(grad, S): grad = dE/dRe Gamma + i dE/dIm Gamma as a host array
+        [NG, Np, Np] (grad=True), written to a device address (grad=int), or
+        skipped (grad=False -> None); S[G] the structure factor."""
+        ptr, dev, _keep = self._vertex_arg(gamma)
+        Np = self.No + self.Nv
+        S = np.zeros(self.NG, dtype=np.float64)
+        out = None
+        if grad is True:
+            out = np.zeros((self.NG, Np, Np), dtype=np.complex128, order="F")
+            gptr, gdev = _ptr(out), 0
+        elif isinstance(grad, int) and grad is not False:
+            gptr, gdev = C.c_void_p(grad), 1
+        else:
+            gptr, gdev = None, 0
+        self._check(self.lib.cc4s_ccsd_energy_grad_backward(
+            self._ctx, ptr, dev, gptr, gdev, _ptr(S), precision))
+        return out, S
 
     def set_profiling(self, enabled=True):
         self._check(self.lib.cc4s_ccsd_set_profiling(self._ctx, 1 if enabled else 0))

@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libcc4s_ccsd.so"
 BENCH = HERE / "gett_bench"
-SOURCES = ["contract.hip", "mixed.hip", "elementwise.hip", "ccsd.hip",
+SOURCES = ["contract.hip", "mixed.hip", "elementwise.hip", "adjoint.hip", "ccsd.hip",
            "coupled_cluster.hip", "abi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -64,6 +64,8 @@ void cc4s_ccsd_default_params(cc4s_ccsd_params *p) {
   p->integrals_slice_size = 0;
   p->precision = CC4S_PRECISION_FP64;
   p->mixed_switch = 1e-5;
+  p->refinement = 1;
+  p->refinement_ratio = 1e-3;
   p->device = 0;
 }
 
@@ -82,6 +84,12 @@ int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
     CoupledClusterParams p;
     p.mixed = params->precision == CC4S_PRECISION_MIXED;
     p.mixed_switch = params->mixed_switch;
+    CC4S_ASSERT(params->refinement == 0 || params->refinement == 1,
+                "unknown refinement mode");
+    CC4S_ASSERT(params->refinement_ratio > 0.0 && params->refinement_ratio < 1.0,
+                "refinement_ratio must lie in (0, 1)");
+    p.defect_correction = params->refinement == 1;
+    p.refinement_ratio = params->refinement_ratio;
     p.shape.field = params->field == CC4S_FIELD_REAL ? Field::Real : Field::Complex;
     p.shape.No = params->No;
     p.shape.Nv = params->Nv;
@@ -181,6 +189,28 @@ int cc4s_ccsd_energy(cc4s_ccsd_context *ctx, const double *T1, const double *T2,
     out[0] = E.real(); out[1] = E.imag();
     out[2] = D.real(); out[3] = D.imag();
     out[4] = X.real(); out[5] = X.imag();
+  });
+}
+
+int cc4s_ccsd_energy_grad_forward(cc4s_ccsd_context *ctx, const double *gamma,
+                                  int gamma_on_device, double *energy) {
+  if (!ctx) return 1;
+  return guarded(ctx, [&] {
+    CC4S_ASSERT(energy, "null output");
+    *energy = ctx->cc->energy_forward(gamma, gamma_on_device != 0);
+  });
+}
+
+int cc4s_ccsd_energy_grad_backward(cc4s_ccsd_context *ctx, const double *gamma,
+                                   int gamma_on_device, double *grad,
+                                   int grad_on_device, double *structure_factor,
+                                   int precision) {
+  if (!ctx) return 1;
+  return guarded(ctx, [&] {
+    CC4S_ASSERT(precision == CC4S_PRECISION_FP64 || precision == CC4S_PRECISION_FP32,
+                "unknown precision for the energy gradient");
+    ctx->cc->energy_backward(gamma, gamma_on_device != 0, grad, grad_on_device != 0,
+                             structure_factor, precision == CC4S_PRECISION_FP32);
   });
 }
 

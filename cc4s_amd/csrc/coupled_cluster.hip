@@ -1,5 +1,7 @@
 #include "coupled_cluster.hpp"
 
+#include "adjoint.hpp"
+
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -179,7 +181,7 @@ CoupledCluster::CoupledCluster(const CoupledClusterParams &p, int device)
               "maxResidua must be in [1, 16]");
   CC4S_ASSERT(!p_.mixed || s.field == Field::Complex,
               "mixed precision needs the complex field");
-  mixed_phase_ = p_.mixed;
+  phase_ = p_.mixed ? Phase::Mixed : Phase::Fp64;
 }
 
 // The DIIS history is the largest allocation; it is placed once the vertex,
@@ -197,6 +199,8 @@ void CoupledCluster::create_mixer() {
   if (p_.mixed) {
     const int64_t rows = (s.No * s.No + 127) / 128 * 128;
     reserve += static_cast<size_t>(s.Nv) * s.Nv * rows * 8;
+    // the defect of the split-bf16 residuum (one amplitude set)
+    if (p_.defect_correction) reserve += s.nset() * scalar_bytes(s.field);
   }
   const size_t avail = free_b > reserve ? free_b - reserve : 0;
   const char *env = std::getenv("CC4S_DIIS_PLACEMENT");
@@ -293,10 +297,25 @@ IterationReport CoupledCluster::iterate() {
   IterationReport rep;
   rep.iteration = iterations_done + 1;
   const void *T = have_amplitudes_ ? T_.get() : nullptr;
-  rep.mixed = mixed_phase_;
-  engine_->set_mixed(mixed_phase_);
-  ccsd_->get_residuum(T, R_.get());                     // :136
-  engine_->set_mixed(false);
+  const int64_t ndoubles = s.nset() * (s.field == Field::Complex ? 2 : 1);
+  if (phase_ == Phase::Anchor && T) {
+    // defect of the split-bf16 residuum at T: c = R64(T) - Rlo(T); the pass
+    // itself continues with R64
+    correction_.reserve(s.nset() * scalar_bytes(s.field));
+    engine_->set_mixed(true);
+    ccsd_->get_residuum(T, correction_.get());
+    engine_->set_mixed(false);
+    ccsd_->get_residuum(T, R_.get());                   // :136
+    axpby(stream_, ndoubles, 1.0, R_.as<double>(), -1.0, correction_.as<double>());
+    rep.mixed = false;
+  } else {
+    rep.mixed = phase_ == Phase::Mixed || phase_ == Phase::Corrected;
+    engine_->set_mixed(rep.mixed);
+    ccsd_->get_residuum(T, R_.get());                   // :136
+    engine_->set_mixed(false);
+    if (phase_ == Phase::Corrected)
+      axpby(stream_, ndoubles, 1.0, correction_.as<double>(), 1.0, R_.as<double>());
+  }
   // :137-145: T' = -R/(Delta+s) and dT = T' - T, written into the mixer slot
   mixer_->write_next(R_.get(), T, static_cast<const double *>(eps_h_.get()),
                      static_cast<const double *>(eps_p_.get()), p_.level_shift);
@@ -315,9 +334,28 @@ IterationReport CoupledCluster::iterate() {
   // the convergence test only counts FP64 passes; the mixed phase ends once
   // the extrapolated amplitude change falls below the switch threshold
   if (rep.mixed) rep.converged = false;
-  if (mixed_phase_ && have_amplitudes_ && !is_second_order &&
-      rep.residuum_norm < p_.mixed_switch)
-    mixed_phase_ = false;
+  switch (phase_) {
+    case Phase::Fp64:
+      break;
+    case Phase::Mixed:
+      if (!is_second_order && rep.residuum_norm < p_.mixed_switch)
+        phase_ = p_.defect_correction ? Phase::Anchor : Phase::Fp64;
+      break;
+    case Phase::Anchor:
+      if (!rep.converged) {
+        phase_ = Phase::Corrected;
+        anchor_norm_ = rep.residuum_norm;
+        inner_passes_ = 0;
+      }
+      break;
+    case Phase::Corrected:
+      ++inner_passes_;
+      if (rep.residuum_norm < p_.refinement_ratio * anchor_norm_ ||
+          rep.residuum_norm < 0.1 * p_.amplitudes_convergence ||
+          inner_passes_ >= p_.max_inner_passes)
+        phase_ = Phase::Anchor;
+      break;
+  }
   previous_e_ = rep.energy;
   iterations_done += 1;
   return rep;
@@ -367,6 +405,162 @@ void CoupledCluster::energy_of(const double *T1, const double *T2, cplx *E,
   scratch_.reserve(s.nset() * scalar_bytes(s.field));
   upload_set(scratch_.get(), T1, T2);
   energy(scratch_.get(), E, D, X);
+}
+
+
+// --- energy adjoint ----------------------------------------------------------
+
+const void *CoupledCluster::adjoint_vertex(const double *gamma, bool on_device) {
+  if (!gamma) return gamma_.get();
+  if (on_device) return gamma;
+  const auto &s = p_.shape;
+  const size_t bytes = static_cast<size_t>(s.NG) * s.Np() * s.Np() * 16;
+  adj_gamma_.reserve(bytes);
+  CC4S_HIP(hipMemcpyAsync(adj_gamma_.get(), gamma, bytes, hipMemcpyHostToDevice, stream_));
+  return adj_gamma_.get();
+}
+
+// hp block [G, i, a] or ph block [G, b, j] of a vertex; the real field reads
+// it as real numbers over the (Re, Im) pairs, [2 NG, ., .]
+View CoupledCluster::vertex_block(const void *gamma, bool hp,
+                                  const std::string &names) const {
+  const auto &s = p_.shape;
+  const int64_t No = s.No, Np = s.Np(), NG = s.NG;
+  View full;
+  if (s.field == Field::Complex) {
+    full = dense_view(const_cast<void *>(gamma), Field::Complex, "GPQ", {NG, Np, Np});
+  } else {
+    full.ptr = const_cast<void *>(gamma);
+    full.field = Field::Real;
+    full.idx = "GPQ";
+    full.lens = {2 * NG, Np, Np};
+    full.strides = {1, 2 * NG, 2 * NG * Np};
+  }
+  const View v = hp ? full.sliced(1, 0, No).sliced(2, No, Np)
+                    : full.sliced(1, No, Np).sliced(2, 0, No);
+  return v.as(names);
+}
+
+// E = sum_G Re sum_ia conj(Gamma^G_ia) g^G_ia with g^G_ia = sum_bj conj(Z_abij)
+// Gamma^G_bj (complex), or 1/2 sum_G sum_ai Re(conj Gamma^G_ai g^G_ai) with
+// g^G_ai = sum_bj Zs_abij Gamma^G_bj (real): the pphh integrals are never
+// formed, Z is rebuilt from the amplitudes in the residuum buffer.
+double CoupledCluster::energy_forward(const double *gamma, bool gamma_on_device) {
+  CC4S_ASSERT(inputs_set_, "inputs not set");
+  CC4S_ASSERT(have_amplitudes_, "no amplitudes yet");
+  CC4S_HIP(hipSetDevice(device_));
+  const auto &s = p_.shape;
+  const int64_t No = s.No, Nv = s.Nv, NG = s.NG;
+  const void *g = adjoint_vertex(gamma, gamma_on_device);
+  energy_weights(stream_, s.field, T_.get(), No, Nv, R_.get());
+  const View Z = dense_view(R_.get(), s.field, "abij", {Nv, Nv, No, No});
+  const size_t block_bytes = static_cast<size_t>(NG) * No * Nv * 16;
+  adj_partials_.reserve(block_dots_partials_size(NG));
+  adj_S_.reserve(NG * sizeof(double));
+  if (s.field == Field::Complex) {
+    adj_hp_.reserve(block_bytes);
+    engine_->contract(1.0, Z.conjugated(), vertex_block(g, false, "Gbj"), 0.0,
+                      dense_view(adj_hp_.get(), Field::Complex, "Gia", {NG, No, Nv}));
+    block_dots(stream_, g, NG, s.Np(), 0, No, No, Nv, adj_hp_.get(), 1.0, false,
+               adj_partials_.get(), adj_S_.as<double>());
+  } else {
+    adj_ph_.reserve(block_bytes);
+    engine_->contract(1.0, Z, vertex_block(g, false, "Gbj"), 0.0,
+                      dense_view(adj_ph_.get(), Field::Real, "Gai", {2 * NG, Nv, No}));
+    block_dots(stream_, g, NG, s.Np(), No, Nv, 0, No, adj_ph_.get(), 0.5, false,
+               adj_partials_.get(), adj_S_.as<double>());
+  }
+  std::vector<double> S(NG);
+  CC4S_HIP(hipMemcpyAsync(S.data(), adj_S_.get(), NG * sizeof(double),
+                          hipMemcpyDeviceToHost, stream_));
+  CC4S_HIP(hipStreamSynchronize(stream_));
+  double E = 0.0;
+  for (double v : S) E += v;
+  return E;
+}
+
+// Complex field: g_hp^G_ia = sum_bj conj(Z_abij) Gamma^G_bj and
+// g_ph^G_bj = sum_ai Z_abij Gamma^G_ia; real field: g_ph^G_ai =
+// sum_bj Zs_abij Gamma^G_bj; every other block of the gradient is zero.
+void CoupledCluster::energy_backward(const double *gamma, bool gamma_on_device,
+                                     double *grad, bool grad_on_device, double *S,
+                                     bool fp32) {
+  CC4S_ASSERT(inputs_set_, "inputs not set");
+  CC4S_ASSERT(have_amplitudes_, "no amplitudes yet");
+  CC4S_HIP(hipSetDevice(device_));
+  const auto &s = p_.shape;
+  const int64_t No = s.No, Nv = s.Nv, NG = s.NG, Np = s.Np(), K = No * Nv;
+  const bool cplx_field = s.field == Field::Complex;
+  const void *g = adjoint_vertex(gamma, gamma_on_device);
+  const size_t block_bytes = static_cast<size_t>(NG) * No * Nv * 16;
+  if (cplx_field) adj_hp_.reserve(block_bytes);
+  adj_ph_.reserve(block_bytes);
+  if (!fp32) {
+    energy_weights(stream_, s.field, T_.get(), No, Nv, R_.get());
+    const View Z = dense_view(R_.get(), s.field, "abij", {Nv, Nv, No, No});
+    if (cplx_field) {
+      engine_->contract(1.0, Z.conjugated(), vertex_block(g, false, "Gbj"), 0.0,
+                        dense_view(adj_hp_.get(), Field::Complex, "Gia", {NG, No, Nv}));
+      engine_->contract(1.0, Z, vertex_block(g, true, "Gia"), 0.0,
+                        dense_view(adj_ph_.get(), Field::Complex, "Gbj", {NG, Nv, No}));
+    } else {
+      engine_->contract(1.0, Z, vertex_block(g, false, "Gbj"), 0.0,
+                        dense_view(adj_ph_.get(), Field::Real, "Gai", {2 * NG, Nv, No}));
+    }
+  } else {
+    // M (K x K complex FP32) fits in the residuum buffer (nset x 8+ bytes)
+    energy_weights_fp32(stream_, s.field, T_.get(), No, Nv, R_.get());
+    const size_t half = static_cast<size_t>(NG) * K * 8;
+    adj_b_.reserve(half);
+    vertex_block_fp32(stream_, g, NG, Np, No, Nv, 0, No, adj_b_.get());  // [G,(b,j)]
+    if (cplx_field) {
+      adj_a_.reserve(half);
+      vertex_block_fp32(stream_, g, NG, Np, 0, No, No, Nv, adj_a_.get());  // [G,(i,a)]
+      // g_hp[G,(i,a)] = sum_(b,j) Gamma_ph[G,(b,j)] conj M[(i,a) + K (b,j)]
+      gemm_fp32(stream_, adj_b_.get(), R_.get(), K, 1, true, adj_hp_.get(), NG, K, K);
+      // g_ph[G,(b,j)] = sum_(i,a) Gamma_hp[G,(i,a)] M[(i,a) + K (b,j)]
+      gemm_fp32(stream_, adj_a_.get(), R_.get(), 1, K, false, adj_ph_.get(), NG, K, K);
+    } else {
+      // g_ph[G,(a,i)] = sum_(b,j) Gamma_ph[G,(b,j)] M[(a,i) + K (b,j)]
+      gemm_fp32(stream_, adj_b_.get(), R_.get(), K, 1, false, adj_ph_.get(), NG, K, K);
+    }
+  }
+  if (S) {
+    adj_partials_.reserve(block_dots_partials_size(NG));
+    adj_S_.reserve(NG * sizeof(double));
+    double *Sd = adj_S_.as<double>();
+    if (cplx_field) {
+      block_dots(stream_, g, NG, Np, 0, No, No, Nv, adj_hp_.get(), 0.5, false,
+                 adj_partials_.get(), Sd);
+      block_dots(stream_, g, NG, Np, No, Nv, 0, No, adj_ph_.get(), 0.5, true,
+                 adj_partials_.get(), Sd);
+    } else {
+      block_dots(stream_, g, NG, Np, No, Nv, 0, No, adj_ph_.get(), 0.5, false,
+                 adj_partials_.get(), Sd);
+    }
+    CC4S_HIP(hipMemcpyAsync(S, Sd, NG * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  }
+  if (grad) {
+    // element (G, p, q) of the gradient at G + NG (p + Np q); the hp block is
+    // Nv runs of NG*No elements, the ph block No runs of NG*Nv elements
+    const size_t bytes = static_cast<size_t>(NG) * Np * Np * 16;
+    const hipMemcpyKind kind = grad_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (grad_on_device) {
+      CC4S_HIP(hipMemsetAsync(grad, 0, bytes, stream_));
+    } else {
+      CC4S_HIP(hipStreamSynchronize(stream_));
+      std::memset(grad, 0, bytes);
+    }
+    const size_t pitch = static_cast<size_t>(NG) * Np * 16;
+    if (cplx_field)
+      CC4S_HIP(hipMemcpy2DAsync(grad + 2 * NG * Np * No, pitch, adj_hp_.get(),
+                                static_cast<size_t>(NG) * No * 16,
+                                static_cast<size_t>(NG) * No * 16, Nv, kind, stream_));
+    CC4S_HIP(hipMemcpy2DAsync(grad + 2 * NG * No, pitch, adj_ph_.get(),
+                              static_cast<size_t>(NG) * Nv * 16,
+                              static_cast<size_t>(NG) * Nv * 16, No, kind, stream_));
+  }
+  CC4S_HIP(hipStreamSynchronize(stream_));
 }
 
 }  // namespace cc4s_amd

@@ -23,6 +23,11 @@ struct CoupledClusterParams {
   // mixer's dR >= mixed_switch, FP64 afterwards
   bool mixed = false;
   double mixed_switch = 1e-5;
+  // after the switch: FP64 anchor passes with split-bf16 passes on the
+  // defect-corrected residuum between them (false: FP64 passes only)
+  bool defect_correction = true;
+  double refinement_ratio = 1e-3;  // dR reduction between anchors
+  int max_inner_passes = 12;
 };
 
 // DIIS over (amplitudes, amplitude change) pairs held as ring-buffer slots
@@ -86,6 +91,18 @@ class CoupledCluster {
   void residuum(const double *T1, const double *T2, double *R1, double *R2);
   void energy_of(const double *T1, const double *T2, cplx *E, cplx *D, cplx *X);
 
+  // Energy adjoint at the current amplitudes.  `gamma` is a vertex
+  // [NG, Np, Np] (host or device memory; null: the input vertex).
+  // energy_forward: Re E of CoupledCluster.cxx:194-253 with the pphh
+  // integrals formed from `gamma` (VertexCoulombIntegrals.cxx:68-171).
+  double energy_forward(const double *gamma, bool gamma_on_device);
+  // grad = dE/dRe Gamma + i dE/dIm Gamma ([NG, Np, Np] complex, host or
+  // device memory, may be null) and S[G] = 1/2 sum_pq Re(conj Gamma^G_pq
+  // grad^G_pq) (host, NG values, may be null); fp32: gradient products in
+  // FP32 arithmetic, otherwise FP64 MFMA.
+  void energy_backward(const double *gamma, bool gamma_on_device, double *grad,
+                       bool grad_on_device, double *S, bool fp32);
+
   const CoupledClusterParams &params() const { return p_; }
   double second_order = std::numeric_limits<double>::quiet_NaN();
   cplx final_energy, final_direct, final_exchange;
@@ -98,6 +115,8 @@ class CoupledCluster {
   void upload_set(void *dst, const double *T1, const double *T2);
   void download_set(const void *src, double *T1, double *T2);
   void create_mixer();
+  const void *adjoint_vertex(const double *gamma, bool on_device);
+  View vertex_block(const void *gamma, bool hp, const std::string &names) const;
 
   CoupledClusterParams p_;
   int device_;
@@ -107,9 +126,17 @@ class CoupledCluster {
   std::unique_ptr<Ccsd> ccsd_;
   std::unique_ptr<DiisMixer> mixer_;
   DeviceBuffer T_, R_, scratch_, energy_partials_;
+  // energy adjoint: uploaded vertex, gradient blocks, FP32 operands, dots
+  DeviceBuffer adj_gamma_, adj_hp_, adj_ph_, adj_a_, adj_b_, adj_partials_, adj_S_;
   bool have_amplitudes_ = false;
   bool inputs_set_ = false;
-  bool mixed_phase_ = false;  // still iterating in mixed precision
+  // mixed precision: split-bf16 passes, then FP64 anchors (with corrected
+  // split-bf16 passes between them) or plain FP64 passes
+  enum class Phase { Fp64, Mixed, Anchor, Corrected };
+  Phase phase_ = Phase::Fp64;
+  double anchor_norm_ = 0;
+  int inner_passes_ = 0;
+  DeviceBuffer correction_;  // R64(T_k) - Rlo(T_k) of the last anchor
   cplx previous_e_ = 0.0;
 };
 

@@ -30,6 +30,9 @@ Nv sorted uniform draws in [0.2, 4.0]; fermiEnergy = -0.1.
 from __future__ import annotations
 
 import dataclasses
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
@@ -113,35 +116,46 @@ CONFIGS = {
 
 
 def raw_vertex(NG: int, Np: int, field: str, seed: int,
-               q_chunk: int = 16) -> np.ndarray:
+               q_chunk: int = 4, workers: int | None = None) -> np.ndarray:
     """Unscaled vertex (w_G = 1/(1+G/16), c = 1), shape (NG, Np, Np),
-    Fortran order so that memory is first-index-fastest like cc4s."""
-    gamma = np.zeros((NG, Np, Np), dtype=np.complex128, order="F")
+    Fortran order so that memory is first-index-fastest like cc4s.
+
+    Only the upper triangle p <= q is drawn (column blocks of q in parallel
+    threads; numpy releases the GIL inside its loops), then mirrored.  The
+    bytes do not depend on `q_chunk` or `workers`."""
+    gamma = np.empty((NG, Np, Np), dtype=np.complex128, order="F")
     g = np.arange(NG, dtype=np.uint64)
     w = 1.0 / (1.0 + np.arange(NG, dtype=np.float64) / 16.0)
-    p_all = np.arange(Np, dtype=np.uint64)
-    for q0 in range(0, Np, q_chunk):
+    complex_field = field == "complex"
+
+    def draw(q0):
         q1 = min(Np, q0 + q_chunk)
         q = np.arange(q0, q1, dtype=np.uint64)
-        # global element index I = G + NG*(p + Np*q), shape (NG, Np, nq)
+        p = np.arange(q1, dtype=np.uint64)
+        # global element index I = G + NG*(p + Np*q), shape (NG, q1, nq)
         idx = (g[:, None, None] + np.uint64(NG) * (
-            p_all[None, :, None] + np.uint64(Np) * q[None, None, :]))
+            p[None, :, None] + np.uint64(Np) * q[None, None, :]))
         re = normals(seed, STREAM_GAMMA, np.uint64(2) * idx)
-        if field == "complex":
+        if complex_field:
             im = normals(seed, STREAM_GAMMA, np.uint64(2) * idx + np.uint64(1))
             block = np.sqrt(0.5) * (re + 1j * im)
+            for j in range(q1 - q0):          # real diagonal, N(0, 1)
+                block[:, q0 + j, j] = np.sqrt(2.0) * block[:, q0 + j, j].real
         else:
             block = re.astype(np.complex128)
-        gamma[:, :, q0:q1] = block
-    # symmetrise from the upper triangle (p <= q)
-    pp, qq = np.tril_indices(Np, -1)          # p > q entries get mirrored
-    if field == "complex":
-        gamma[:, pp, qq] = np.conj(gamma[:, qq, pp])
-        d = np.arange(Np)
-        gamma[:, d, d] = np.sqrt(2.0) * gamma[:, d, d].real
-    else:
-        gamma[:, pp, qq] = gamma[:, qq, pp]
-    gamma *= w[:, None, None]
+        block *= w[:, None, None]
+        gamma[:, :q1, q0:q1] = block
+
+    def mirror(q0):
+        for q in range(q0, min(Np, q0 + q_chunk)):
+            upper = gamma[:, q, q + 1:]
+            gamma[:, q + 1:, q] = np.conj(upper) if complex_field else upper
+
+    n = workers or min(16, os.cpu_count() or 1)
+    starts = list(range(0, Np, q_chunk))
+    with ThreadPoolExecutor(max_workers=n) as ex:
+        list(ex.map(draw, starts[::-1]))       # largest blocks first
+        list(ex.map(mirror, starts))
     return gamma
 
 

@@ -47,10 +47,16 @@ enum { CC4S_FIELD_REAL = 0, CC4S_FIELD_COMPLEX = 1 };
 /* Arithmetic of the large contractions (ladder, ring, Gamma terms).  FP64
  * reproduces the reference.  MIXED (complex field only) runs them on bf16
  * MFMA with every FP64 operand split into hi + lo parts and FP32
- * accumulation, until the mixer's dR falls below mixed_switch, then switches
- * to FP64 for the remaining passes (iterative refinement of the same fixed
- * point).  Denominators, energy, DIIS and every small contraction stay FP64. */
-enum { CC4S_PRECISION_FP64 = 0, CC4S_PRECISION_MIXED = 1 };
+ * accumulation, until the mixer's dR falls below mixed_switch; then FP64
+ * residua refine the fixed point.  With refinement 0 every later pass is an
+ * FP64 pass.  With refinement 1 (defect correction) a pass evaluates the FP64
+ * residuum R64(T_k) and the split-bf16 one Rlo(T_k) at the same amplitudes
+ * (an anchor pass, updated with R64), and the passes after it use the
+ * corrected split-bf16 residuum Rlo(T) + R64(T_k) - Rlo(T_k) until dR falls
+ * below refinement_ratio times the anchor's dR; only anchor passes can
+ * declare convergence.  Denominators, energy, DIIS and every small
+ * contraction stay FP64.  FP32 is accepted by the energy gradient only. */
+enum { CC4S_PRECISION_FP64 = 0, CC4S_PRECISION_MIXED = 1, CC4S_PRECISION_FP32 = 2 };
 
 /* Step arguments of CoupledCluster (CoupledCluster.cxx:59-117,260-262,
  * CoupledCluster.hpp:45-49, Ccsd.cxx:95,287-289, DiisMixer.cxx:33). */
@@ -66,6 +72,8 @@ typedef struct {
   int64_t integrals_slice_size;   /* integralsSliceSize (0 = No) */
   int precision;                  /* CC4S_PRECISION_* */
   double mixed_switch;            /* MIXED: dR below which FP64 takes over (1e-5) */
+  int refinement;                 /* MIXED: 0 FP64 passes, 1 defect correction (1) */
+  double refinement_ratio;        /* MIXED, refinement 1: dR reduction per anchor (1e-3) */
   int device;                     /* HIP device ordinal */
 } cc4s_ccsd_params;
 
@@ -134,6 +142,31 @@ int cc4s_ccsd_residuum(cc4s_ccsd_context *ctx, const double *T1,
  * 194-253): out[0..5] = Re E, Im E, Re D, Im D, Re X, Im X. */
 int cc4s_ccsd_energy(cc4s_ccsd_context *ctx, const double *T1,
                      const double *T2, double *out);
+
+/* Energy adjoint at the context's current amplitudes (after iterate/run or
+ * set_amplitudes).  The reference has no such entry point: its finite-size
+ * analysis (FiniteSizeCorrection.cxx:50-133) contracts the converged
+ * amplitudes with the vertex into a transition structure factor; this pair
+ * exposes the correlation energy as a function of the vertex and its exact
+ * gradient, whose per-G split is that structure factor without the Coulomb
+ * potential and singular-vector factors (FiniteSizeCorrection.cxx:74-124).
+ *
+ * forward: *energy = Re E of CoupledCluster.cxx:194-253 with the pphh
+ * integrals formed from `gamma` as VertexCoulombIntegrals.cxx:68-171 forms
+ * them.  gamma: [NG, Np, Np] complex interleaved, in host memory
+ * (gamma_on_device 0) or device memory (1); NULL: the vertex of set_inputs. */
+int cc4s_ccsd_energy_grad_forward(cc4s_ccsd_context *ctx, const double *gamma,
+                                  int gamma_on_device, double *energy);
+/* backward: grad[G,p,q] = dE/dRe Gamma^G_pq + i dE/dIm Gamma^G_pq ([NG, Np,
+ * Np] complex interleaved, host or device memory per grad_on_device; NULL to
+ * skip) and structure_factor[G] = 1/2 sum_pq Re(conj(Gamma^G_pq) grad^G_pq)
+ * (host, NG values; NULL to skip), whose sum over G is the energy.
+ * precision: CC4S_PRECISION_FP64 (FP64 MFMA) or CC4S_PRECISION_FP32 (FP32
+ * operands and arithmetic). */
+int cc4s_ccsd_energy_grad_backward(cc4s_ccsd_context *ctx, const double *gamma,
+                                   int gamma_on_device, double *grad,
+                                   int grad_on_device, double *structure_factor,
+                                   int precision);
 
 /* Profiling (no counterpart in the reference, whose Timer/OperationsCounter
  * report only per-step totals, Cc4s.cxx:160-173): when enabled, every

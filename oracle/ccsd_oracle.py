@@ -14,6 +14,11 @@ It restates, in numpy (einsum -> BLAS), the reference's algorithm:
 * ``energy``                CoupledCluster.cxx:194-253
 * ``DiisMixer``             mixers/DiisMixer.cxx:27-148 (+ TensorSet.hpp:209-224)
 * ``coupled_cluster``       CoupledCluster.cxx:55-192
+* ``vertex_energy`` / ``vertex_gradient``  the energy of CoupledCluster.cxx:
+  194-253 as a function of Gamma (V from VertexCoulombIntegrals.cxx:68-171)
+  at fixed amplitudes, its gradient and per-G split (the structure factor of
+  FiniteSizeCorrection.cxx:110-124 without its 1/V(G) and singular-vector
+  factors)
 
 Arrays are indexed in the reference's index order (T2[a,b,i,j], Gamma[G,p,q])
 with numpy's own memory order; only the C-ABI boundary cares about the cc4s
@@ -28,7 +33,11 @@ binary, which needs CTF and yaml-cpp that are not available offline):
   (SecondOrderPerturbationTheory.cxx:108-127);
 * DIIS with maxResidua=1 is plain Jacobi; the converged fixed point is
   independent of levelShift and integralsSliceSize;
-* V slices are checked against brute-force sums over G.
+* V slices are checked against brute-force sums over G;
+* the vertex gradient against central finite differences of
+  ``vertex_energy`` in every real component, ``vertex_energy`` against
+  ``energy`` through the V slices, and sum_G S(G) = E (degree-2
+  homogeneity).
 """
 
 from __future__ import annotations
@@ -496,3 +505,53 @@ def mp2_energy(P):
     D = -2.0 * np.sum(V * T)
     X = np.sum(V.transpose(0, 1, 3, 2) * T)
     return float(np.real(D + X))
+
+
+# ---------------------------------------------------------------------------
+# Energy as a function of the vertex (adjoint checks)
+# ---------------------------------------------------------------------------
+
+def energy_weights(amplitudes, field):
+    """Z with E = sum_abij Z_abij conj V_abij: Z = 2 tau_abij - tau_baij,
+    tau = T2 + T1 T1 (CoupledCluster.cxx:204-232).  Real field: the
+    symmetrised Z_abij + Z_baji (V_abij is formed from one vertex block
+    twice there, VertexCoulombIntegrals.cxx:87-102)."""
+    T1, T2 = amplitudes
+    tau = T2 + ein("ai,bj->abij", T1, T1)
+    Z = 2.0 * tau - tau.transpose(1, 0, 2, 3)
+    if field == "real":
+        Z = Z + Z.transpose(1, 0, 3, 2)
+    return Z
+
+
+def vertex_energy(gamma, amplitudes, No, field):
+    """Re E (CoupledCluster.cxx:194-253) with V_abij formed from `gamma`
+    (VertexCoulombIntegrals.cxx:68-102 real, 143-171 complex)."""
+    V = vertex_integrals(slice_vertex(gamma, No), field, ("pphh",))["pphh"]
+    T1, T2 = amplitudes
+    tau = T2 + ein("ai,bj->abij", T1, T1)
+    Vc = np.conj(V)
+    E = 2.0 * np.sum(tau * Vc) - np.sum(tau * Vc.transpose(1, 0, 2, 3))
+    return float(np.real(E))
+
+
+def vertex_gradient(gamma, amplitudes, No, field):
+    """(g, S): g[G,p,q] = dE/dRe Gamma^G_pq + i dE/dIm Gamma^G_pq of
+    ``vertex_energy`` and S[G] = 1/2 sum_pq Re(conj(Gamma^G_pq) g^G_pq).
+
+    complex: E = sum_G sum_abij Z_abij Gamma^G_ia conj(Gamma^G_bj), so
+      g^G_ia = sum_bj conj(Z_abij) Gamma^G_bj, g^G_bj = sum_ai Z_abij Gamma^G_ia;
+    real: E = 1/2 sum_G sum_abij Zs_abij (Re Re + Im Im)(Gamma^G_ai, Gamma^G_bj),
+      g^G_ai = sum_bj Zs_abij Gamma^G_bj.
+    The hh and pp blocks do not enter E."""
+    g = np.zeros(gamma.shape, dtype=np.complex128)
+    Z = energy_weights(amplitudes, field)
+    hp = gamma[:, :No, No:]
+    ph = gamma[:, No:, :No]
+    if field == "complex":
+        g[:, :No, No:] = ein("abij,Gbj->Gia", np.conj(Z), ph)
+        g[:, No:, :No] = ein("abij,Gia->Gbj", Z, hp)
+    else:
+        g[:, No:, :No] = ein("abij,Gbj->Gai", Z, ph)
+    S = 0.5 * np.real(np.sum(np.conj(gamma) * g, axis=(1, 2)))
+    return g, S

@@ -42,6 +42,10 @@ def test_default_params_are_the_reference_defaults():
     assert p.ppl == 1
     assert p.integrals_slice_size == 0
     assert p.field == api.FIELD_COMPLEX
+    # mixed-precision settings (no counterpart in the reference)
+    assert p.precision == api.PRECISION_FP64
+    assert p.mixed_switch == 1e-5
+    assert p.refinement == 1 and p.refinement_ratio == 1e-3
 
 
 def test_unsupported_precision_is_refused_before_touching_a_device():
@@ -49,3 +53,11 @@ def test_unsupported_precision_is_refused_before_touching_a_device():
         api.CoupledCluster(2, 4, 3, precision=7)
     with pytest.raises(api.CcsdError, match="complex field"):
         api.CoupledCluster(2, 4, 3, field="real", precision=api.PRECISION_MIXED)
+
+
+def test_invalid_refinement_is_refused_before_touching_a_device():
+    with pytest.raises(api.CcsdError, match="refinement mode"):
+        api.CoupledCluster(2, 4, 3, precision=api.PRECISION_MIXED, refinement=3)
+    with pytest.raises(api.CcsdError, match="refinement_ratio"):
+        api.CoupledCluster(2, 4, 3, precision=api.PRECISION_MIXED,
+                           refinementRatio=1.5)
