@@ -3,17 +3,19 @@ time to convergence.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
                 [--scale mp2_per_occupied|max_v] [--converge both|fp64|mixed|none]
-                [--mixed-switch 1e-5] [--no-cpu-baseline] [--adjoint auto|on|off]
+                [--mixed-switch 1e-5,1e-7] [--no-cpu-baseline] [--adjoint auto|on|off]
 
 A step is one pass of the CCSD iteration loop (residuum, denominator update,
 DIIS, energy) on the seeded synthetic workload of the chosen configuration,
 with the Coulomb vertex already resident in HBM.  The FP64 context does W
 untimed passes (pass 1 has no residuum contraction: R = V), then K timed
 passes: `value` is their mean wall time.  With --converge the FP64 context
-keeps iterating to the convergence criteria, and a second context runs the
-mixed-precision path (split-bf16 contractions until dR < mixed switch, then
-FP64) to the same criteria; both times to convergence and |E_mixed - E_FP64|
-are reported.  At the FP64 fixed point of the No=32, Nv=256, NG=1500
+keeps iterating to the convergence criteria, and one further context per
+mixed switch value runs the mixed-precision path (split-bf16 contractions
+until dR < switch, then FP64) to the same criteria; every time to
+convergence, its ratio to the FP64 one and |E_mixed - E_FP64| are reported.
+The first switch value is the workload's written 1e-5; the second (1e-7) sits
+a decade above the rounding floor of the split-bf16 residuum (dR ~1e-8).  At the FP64 fixed point of the No=32, Nv=256, NG=1500
 workload the energy adjoint (gradient with respect to the vertex and the
 structure factor S(G), FP64 and FP32) is timed and checked against central
 differences of the forward energy.  Prints one JSON line.
@@ -115,13 +117,13 @@ def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
     }
 
 
-def make_context(cfg, precision, switch, refinement=1):
+def make_context(cfg, precision, switch):
     return api.CoupledCluster(cfg.No, cfg.Nv, cfg.NG, field=cfg.field,
                               maxIterations=cfg.max_iterations,
                               energyConvergence=cfg.energy_convergence,
                               amplitudesConvergence=cfg.amplitudes_convergence,
                               maxResidua=cfg.max_residua, precision=precision,
-                              mixedSwitch=switch, refinement=refinement)
+                              mixedSwitch=switch)
 
 
 def converge(cc, label, first_passes):
@@ -190,10 +192,10 @@ def main():
                     choices=("mp2_per_occupied", "max_v"))
     ap.add_argument("--converge", default="both",
                     choices=("both", "fp64", "mixed", "none"))
-    ap.add_argument("--mixed-switch", type=float, default=1e-5)
-    ap.add_argument("--refinement", type=int, default=1, choices=(0, 1),
-                    help="after the switch: 0 FP64 passes, 1 defect correction "
-                         "(FP64 anchors, corrected split-bf16 passes between)")
+    ap.add_argument("--mixed-switch", default="1e-5,1e-7",
+                    help="comma-separated dR thresholds of the mixed -> FP64 "
+                         "switch; one mixed run per value (the first is the "
+                         "workload's written 1e-5)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint", default="auto", choices=("auto", "on", "off"),
                     help="energy adjoint at the FP64 fixed point (auto: on "
@@ -203,6 +205,7 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; "
                           "--gpus must be 1"}))
         sys.exit(2)
+    switches = [float(x) for x in str(args.mixed_switch).split(",") if x]
     cfg = syn.CONFIGS[args.config]
     log(f"generating inputs: {cfg.name}, scale rule {args.scale}")
     t = time.perf_counter()
@@ -219,7 +222,7 @@ def main():
     value = None
     iter_flops = None
     if args.converge in ("both", "fp64", "none"):
-        cc = make_context(cfg, api.PRECISION_FP64, args.mixed_switch)
+        cc = make_context(cfg, api.PRECISION_FP64, switches[0])
         t = time.perf_counter()
         cc.set_inputs(inp.gamma, inp.eps)
         log(f"inputs resident, V slices formed: {time.perf_counter() - t:.2f} s")
@@ -256,17 +259,17 @@ def main():
             if want and passes[-1]["converged"]:
                 adjoint = adjoint_report(cc, inp.gamma, passes[-1]["energy"])
         cc.close()
-    if args.converge in ("both", "mixed"):
-        cc = make_context(cfg, api.PRECISION_MIXED, args.mixed_switch,
-                          args.refinement)
+    for n, sw in enumerate(switches if args.converge in ("both", "mixed") else []):
+        key = "mixed" if n == 0 else f"mixed_switch_{sw:g}"
+        cc = make_context(cfg, api.PRECISION_MIXED, sw)
         cc.set_inputs(inp.gamma, inp.eps)
-        passes = converge(cc, "mixed", [])
+        passes = converge(cc, key, [])
         mixed_times = [p["seconds"] for p in passes[1:] if p["precision_used"]]
         fp64_times = [p["seconds"] for p in passes[1:] if not p["precision_used"]]
-        out_conv["mixed"] = {
+        out_conv[key] = {
             "iterations": len(passes),
             "mixed_iterations": sum(1 for p in passes if p["precision_used"]),
-            "fp64_residuum_passes": len(fp64_times),
+            "fp64_passes": len(fp64_times),
             "seconds": float(sum(p["seconds"] for p in passes)),
             "seconds_per_mixed_iteration": (float(np.mean(mixed_times))
                                             if mixed_times else None),
@@ -274,16 +277,14 @@ def main():
                                       if fp64_times else None),
             "energy": passes[-1]["energy"],
             "converged": bool(passes[-1]["converged"]),
-            "switch": args.mixed_switch,
-            "refinement": ("defect correction" if args.refinement == 1
-                           else "FP64 passes"),
+            "switch": sw,
         }
         cc.close()
-    if "fp64" in out_conv and "mixed" in out_conv:
-        out_conv["speedup_time_to_convergence"] = (
-            out_conv["fp64"]["seconds"] / out_conv["mixed"]["seconds"])
-        out_conv["abs_energy_difference"] = abs(
-            out_conv["fp64"]["energy"] - out_conv["mixed"]["energy"])
+        if "fp64" in out_conv:
+            out_conv[key]["speedup_time_to_convergence"] = (
+                out_conv["fp64"]["seconds"] / out_conv[key]["seconds"])
+            out_conv[key]["abs_energy_difference"] = abs(
+                out_conv["fp64"]["energy"] - out_conv[key]["energy"])
 
     out = {
         "metric": METRIC,

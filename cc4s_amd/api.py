@@ -49,8 +49,6 @@ class Params(C.Structure):
         ("integrals_slice_size", C.c_int64),
         ("precision", C.c_int),
         ("mixed_switch", C.c_double),
-        ("refinement", C.c_int),
-        ("refinement_ratio", C.c_double),
         ("device", C.c_int),
     ]
 
@@ -158,7 +156,7 @@ class CoupledCluster:
                  energyConvergence=1e-7, amplitudesConvergence=1e-6,
                  levelShift=0.0, maxResidua=4, ppl=True,
                  integralsSliceSize=0, device=0, precision=PRECISION_FP64,
-                 mixedSwitch=1e-5, refinement=None, refinementRatio=None):
+                 mixedSwitch=1e-5):
         self.lib = load_library()
         p = default_params()
         p.field = FIELD_COMPLEX if field == "complex" else FIELD_REAL
@@ -173,10 +171,6 @@ class CoupledCluster:
         p.device = device
         p.precision = precision
         p.mixed_switch = mixedSwitch
-        if refinement is not None:
-            p.refinement = int(refinement)
-        if refinementRatio is not None:
-            p.refinement_ratio = refinementRatio
         self.params = p
         self.field = field
         self.No, self.Nv, self.NG = No, Nv, NG

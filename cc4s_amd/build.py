@@ -18,6 +18,8 @@ CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libcc4s_ccsd.so"
 BENCH = HERE / "gett_bench"
+DRIVER = HERE / "cc4s_ccsd"
+DRIVER_SOURCES = ["main.cpp", "yaml.cpp", "tensor_io.cpp"]
 SOURCES = ["contract.hip", "mixed.hip", "elementwise.hip", "adjoint.hip", "ccsd.hip",
            "coupled_cluster.hip", "abi.hip"]
 
@@ -67,6 +69,18 @@ def build(jobs: int = 8, verbose: bool = True) -> pathlib.Path:
         raise RuntimeError(f"benchmark build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {BENCH}")
+    # host drop-in driver for cc4s operation files (host C++ over the C ABI)
+    drv = CSRC / "driver"
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall",
+           f"-I{ROOT / 'include'}", f"-I{drv}",
+           *(str(drv / f) for f in DRIVER_SOURCES),
+           "-o", str(DRIVER), f"-L{HERE}", "-lcc4s_ccsd", "-Wl,-rpath,$ORIGIN",
+           "-Wl,--allow-shlib-undefined"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {DRIVER}")
     return LIB
 
 

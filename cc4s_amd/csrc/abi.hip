@@ -64,8 +64,6 @@ void cc4s_ccsd_default_params(cc4s_ccsd_params *p) {
   p->integrals_slice_size = 0;
   p->precision = CC4S_PRECISION_FP64;
   p->mixed_switch = 1e-5;
-  p->refinement = 1;
-  p->refinement_ratio = 1e-3;
   p->device = 0;
 }
 
@@ -84,12 +82,6 @@ int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
     CoupledClusterParams p;
     p.mixed = params->precision == CC4S_PRECISION_MIXED;
     p.mixed_switch = params->mixed_switch;
-    CC4S_ASSERT(params->refinement == 0 || params->refinement == 1,
-                "unknown refinement mode");
-    CC4S_ASSERT(params->refinement_ratio > 0.0 && params->refinement_ratio < 1.0,
-                "refinement_ratio must lie in (0, 1)");
-    p.defect_correction = params->refinement == 1;
-    p.refinement_ratio = params->refinement_ratio;
     p.shape.field = params->field == CC4S_FIELD_REAL ? Field::Real : Field::Complex;
     p.shape.No = params->No;
     p.shape.Nv = params->Nv;

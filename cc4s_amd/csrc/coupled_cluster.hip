@@ -181,7 +181,7 @@ CoupledCluster::CoupledCluster(const CoupledClusterParams &p, int device)
               "maxResidua must be in [1, 16]");
   CC4S_ASSERT(!p_.mixed || s.field == Field::Complex,
               "mixed precision needs the complex field");
-  phase_ = p_.mixed ? Phase::Mixed : Phase::Fp64;
+  mixed_phase_ = p_.mixed;
 }
 
 // The DIIS history is the largest allocation; it is placed once the vertex,
@@ -199,8 +199,6 @@ void CoupledCluster::create_mixer() {
   if (p_.mixed) {
     const int64_t rows = (s.No * s.No + 127) / 128 * 128;
     reserve += static_cast<size_t>(s.Nv) * s.Nv * rows * 8;
-    // the defect of the split-bf16 residuum (one amplitude set)
-    if (p_.defect_correction) reserve += s.nset() * scalar_bytes(s.field);
   }
   const size_t avail = free_b > reserve ? free_b - reserve : 0;
   const char *env = std::getenv("CC4S_DIIS_PLACEMENT");
@@ -297,25 +295,10 @@ IterationReport CoupledCluster::iterate() {
   IterationReport rep;
   rep.iteration = iterations_done + 1;
   const void *T = have_amplitudes_ ? T_.get() : nullptr;
-  const int64_t ndoubles = s.nset() * (s.field == Field::Complex ? 2 : 1);
-  if (phase_ == Phase::Anchor && T) {
-    // defect of the split-bf16 residuum at T: c = R64(T) - Rlo(T); the pass
-    // itself continues with R64
-    correction_.reserve(s.nset() * scalar_bytes(s.field));
-    engine_->set_mixed(true);
-    ccsd_->get_residuum(T, correction_.get());
-    engine_->set_mixed(false);
-    ccsd_->get_residuum(T, R_.get());                   // :136
-    axpby(stream_, ndoubles, 1.0, R_.as<double>(), -1.0, correction_.as<double>());
-    rep.mixed = false;
-  } else {
-    rep.mixed = phase_ == Phase::Mixed || phase_ == Phase::Corrected;
-    engine_->set_mixed(rep.mixed);
-    ccsd_->get_residuum(T, R_.get());                   // :136
-    engine_->set_mixed(false);
-    if (phase_ == Phase::Corrected)
-      axpby(stream_, ndoubles, 1.0, correction_.as<double>(), 1.0, R_.as<double>());
-  }
+  rep.mixed = mixed_phase_;
+  engine_->set_mixed(mixed_phase_);
+  ccsd_->get_residuum(T, R_.get());                     // :136
+  engine_->set_mixed(false);
   // :137-145: T' = -R/(Delta+s) and dT = T' - T, written into the mixer slot
   mixer_->write_next(R_.get(), T, static_cast<const double *>(eps_h_.get()),
                      static_cast<const double *>(eps_p_.get()), p_.level_shift);
@@ -332,29 +315,18 @@ IterationReport CoupledCluster::iterate() {
   rep.converged = std::abs(rep.energy - previous_e_) < p_.energy_convergence &&
                   rep.residuum_norm < p_.amplitudes_convergence;  // :167-173
   // the convergence test only counts FP64 passes; the mixed phase ends once
-  // the extrapolated amplitude change falls below the switch threshold
+  // the extrapolated amplitude change falls below the switch threshold, or
+  // stops falling (the rounding floor of the split-bf16 residuum)
   if (rep.mixed) rep.converged = false;
-  switch (phase_) {
-    case Phase::Fp64:
-      break;
-    case Phase::Mixed:
-      if (!is_second_order && rep.residuum_norm < p_.mixed_switch)
-        phase_ = p_.defect_correction ? Phase::Anchor : Phase::Fp64;
-      break;
-    case Phase::Anchor:
-      if (!rep.converged) {
-        phase_ = Phase::Corrected;
-        anchor_norm_ = rep.residuum_norm;
-        inner_passes_ = 0;
-      }
-      break;
-    case Phase::Corrected:
-      ++inner_passes_;
-      if (rep.residuum_norm < p_.refinement_ratio * anchor_norm_ ||
-          rep.residuum_norm < 0.1 * p_.amplitudes_convergence ||
-          inner_passes_ >= p_.max_inner_passes)
-        phase_ = Phase::Anchor;
-      break;
+  if (mixed_phase_ && !is_second_order) {
+    if (best_mixed_norm_ == 0 || rep.residuum_norm * p_.mixed_stall_gain < best_mixed_norm_) {
+      best_mixed_norm_ = rep.residuum_norm;
+      stalled_passes_ = 0;
+    } else {
+      ++stalled_passes_;
+    }
+    if (rep.residuum_norm < p_.mixed_switch || stalled_passes_ >= p_.mixed_stall_passes)
+      mixed_phase_ = false;
   }
   previous_e_ = rep.energy;
   iterations_done += 1;

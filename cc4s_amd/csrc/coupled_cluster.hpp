@@ -23,11 +23,11 @@ struct CoupledClusterParams {
   // mixer's dR >= mixed_switch, FP64 afterwards
   bool mixed = false;
   double mixed_switch = 1e-5;
-  // after the switch: FP64 anchor passes with split-bf16 passes on the
-  // defect-corrected residuum between them (false: FP64 passes only)
-  bool defect_correction = true;
-  double refinement_ratio = 1e-3;  // dR reduction between anchors
-  int max_inner_passes = 12;
+  // the mixed phase also ends after `mixed_stall_passes` passes without a
+  // `mixed_stall_gain`-fold drop of dR below its best value: the split
+  // residuum's rounding floor (dR ~1e-8 at No=32, Nv=256) was reached
+  int mixed_stall_passes = 3;
+  double mixed_stall_gain = 1.5;
 };
 
 // DIIS over (amplitudes, amplitude change) pairs held as ring-buffer slots
@@ -130,13 +130,9 @@ class CoupledCluster {
   DeviceBuffer adj_gamma_, adj_hp_, adj_ph_, adj_a_, adj_b_, adj_partials_, adj_S_;
   bool have_amplitudes_ = false;
   bool inputs_set_ = false;
-  // mixed precision: split-bf16 passes, then FP64 anchors (with corrected
-  // split-bf16 passes between them) or plain FP64 passes
-  enum class Phase { Fp64, Mixed, Anchor, Corrected };
-  Phase phase_ = Phase::Fp64;
-  double anchor_norm_ = 0;
-  int inner_passes_ = 0;
-  DeviceBuffer correction_;  // R64(T_k) - Rlo(T_k) of the last anchor
+  bool mixed_phase_ = false;  // still iterating in mixed precision
+  double best_mixed_norm_ = 0;
+  int stalled_passes_ = 0;
   cplx previous_e_ = 0.0;
 };
 

@@ -1,0 +1,519 @@
+// cc4s_ccsd: runs the CCSD steps of a cc4s operation file on one MI355X.
+//
+//   cc4s_ccsd [-i cc4s.in] [-o cc4s.out.yaml] [--dry-run]
+//
+// The operation file is the reference's (a YAML list of steps with name/in/out,
+// Cc4s.cxx:52-175); the algorithms this drop-in executes are the CCSD path:
+//   Read                    Read.cxx:27-43 (Reader.hpp: Tensor, TensorSet)
+//   DefineHolesAndParticles DefineHolesAndParticles.cxx:38-96
+//   SliceOperator           SliceOperator.cxx:50-131
+//   VertexCoulombIntegrals  VertexCoulombIntegrals.cxx:36-52 (the field choice)
+//   CoupledCluster          CoupledCluster.cxx:55-192, method Ccsd, DiisMixer,
+//                           through the C ABI of include/cc4s_ccsd.h
+//   Write                   Write.cxx:27-42 (Writer.hpp)
+// Any other algorithm stops the run with "unknown algorithm: <name>", as the
+// reference does for names it does not know.  Step arguments that are unquoted
+// names of stored objects are replaced by the objects (Cc4s.cxx:177-195), the
+// step's "out" names store its results (:197-211), and the report is
+// rewritten after every step (:73-81) with in/out/statistics per step.
+//
+// Arguments of CoupledCluster beyond the reference's: precision (fp64 |
+// mixed), mixedSwitch (1e-5), device (0).
+#include "cc4s_ccsd.h"
+#include "tensor_io.hpp"
+#include "yaml.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace cc4s_amd;
+using yaml::Node;
+using yaml::NodePtr;
+
+namespace {
+
+struct Object {
+  enum class Kind { Data, Tensor, TensorSet, SlicedVertex, Integrals };
+  Kind kind = Kind::Data;
+  NodePtr data;
+  std::shared_ptr<HostTensor> tensor;
+  std::shared_ptr<HostTensorSet> set;
+  // SlicedVertex: the whole vertex; holes [0, No), particles [Np - Nv, Np)
+  std::shared_ptr<HostTensor> vertex;
+  int64_t No = 0, Nv = 0;
+  // Integrals: field of the Coulomb integrals and the sliced vertex they use
+  bool real_field = false;
+  std::shared_ptr<Object> sliced_vertex;
+  std::string label;
+};
+using ObjectPtr = std::shared_ptr<Object>;
+
+std::string next_label() {
+  static int counter = 0;
+  char buf[32];
+  std::snprintf(buf, sizeof(buf), "0x%06x", 0x1000 + 16 * ++counter);
+  return buf;
+}
+
+ObjectPtr make_object(Object::Kind k) {
+  auto o = std::make_shared<Object>();
+  o->kind = k;
+  o->label = next_label();
+  return o;
+}
+
+struct Options {
+  std::string in = "cc4s.in", out = "cc4s.out.yaml";
+  bool dry_run = false;
+};
+
+class Runner {
+ public:
+  explicit Runner(const Options &o) : opt_(o) {}
+
+  int run() {
+    auto steps = yaml::parse_file(opt_.in);
+    if (!steps->is_map()) throw std::runtime_error("expecting map as input");
+    report_ = Node::map();
+    auto env = Node::map();
+    env->set("program", "cc4s_ccsd (MI355X CCSD drop-in)");
+    env->set("dry run only", static_cast<int64_t>(opt_.dry_run ? 1 : 0));
+    report_->set("executionEnvironment", env);
+    executed_ = Node::map();
+    report_->set("steps", executed_);
+    const auto t0 = std::chrono::steady_clock::now();
+    double total_flops = 0;
+    for (size_t i = 0; i < steps->size(); ++i) {
+      auto step = steps->get(i);
+      if (!step || !step->is_map()) throw std::runtime_error("step " + std::to_string(i) + " is not a map");
+      total_flops += run_step(i, step);
+      emit();
+    }
+    const double total = seconds_since(t0);
+    auto stats = Node::map();
+    stats->set("realtime", total);
+    stats->set("floatingPointOperations", total_flops);
+    stats->set("flops", total > 0 ? total_flops / total : 0.0);
+    report_->set("statistics", stats);
+    emit();
+    return 0;
+  }
+
+ private:
+  static double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  void emit() { yaml::emit_file(report_, (opt_.dry_run ? "dry-" : "") + opt_.out); }
+
+  // --- arguments --------------------------------------------------------------
+
+  ObjectPtr symbol(const NodePtr &args, const std::string &key, bool required = true) {
+    auto n = args ? args->get(key) : nullptr;
+    if (!n) {
+      if (required) throw std::runtime_error("expecting key '" + key + "'");
+      return nullptr;
+    }
+    if (n->is_scalar() && !n->quoted) {
+      auto it = storage_.find(n->value);
+      if (it != storage_.end()) return it->second;
+    }
+    if (required) throw std::runtime_error("'" + key + "' does not name a stored object");
+    return nullptr;
+  }
+
+  // the arguments as reported: stored objects by label
+  NodePtr reported(const NodePtr &args) {
+    auto r = Node::map();
+    if (!args) return r;
+    for (const auto &e : args->entries) {
+      const auto &v = e.second;
+      if (v && v->is_scalar() && !v->quoted && storage_.count(v->value))
+        r->set(e.first, storage_.at(v->value)->label);
+      else
+        r->set(e.first, v);
+    }
+    return r;
+  }
+
+  double run_step(size_t i, const NodePtr &step) {
+    const std::string name = yaml::get_text(step, "name", "");
+    std::printf("step: %zu, %s\n", i + 1, name.c_str());
+    std::fflush(stdout);
+    NodePtr args = step->get("in");
+    if (!args || !args->is_map()) args = Node::map();
+    auto in_report = reported(args);
+    std::map<std::string, ObjectPtr> results;
+    NodePtr out_report = Node::map();
+    double flops = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (name == "Read") {
+      read(args, results);
+    } else if (name == "DefineHolesAndParticles") {
+      define_holes_and_particles(args, results);
+    } else if (name == "SliceOperator") {
+      slice_operator(args, results);
+    } else if (name == "VertexCoulombIntegrals") {
+      vertex_coulomb_integrals(args, results);
+    } else if (name == "CoupledCluster") {
+      flops = coupled_cluster(args, in_report, results, out_report);
+    } else if (name == "Write") {
+      write(args, out_report);
+    } else {
+      throw std::runtime_error("unknown algorithm: " + name);
+    }
+    const double dt = seconds_since(t0);
+    // store the results under the names of "out" (Cc4s.cxx:197-211)
+    for (const auto &r : results)
+      out_report->set(r.first, r.second->kind == Object::Kind::Data ? r.second->data
+                                                                     : Node::scalar(r.second->label));
+    auto outs = step->get("out");
+    if (outs && outs->is_map()) {
+      for (const auto &e : outs->entries) {
+        auto it = results.find(e.first);
+        if (it == results.end()) {
+          std::fprintf(stderr, "warning: symbol '%s' should be stored but is not present in output\n",
+                       e.first.c_str());
+          continue;
+        }
+        storage_[e.second->text()] = it->second;
+      }
+    }
+    auto rec = Node::map();
+    rec->set("name", name);
+    rec->set("in", in_report);
+    rec->set("out", out_report);
+    auto stats = Node::map();
+    stats->set("realtime", dt);
+    stats->set("floatingPointOperations", flops);
+    stats->set("flops", dt > 0 ? flops / dt : 0.0);
+    rec->set("statistics", stats);
+    executed_->set(std::to_string(i), rec);
+    std::printf("realtime %.6f s\n--\n", dt);
+    std::fflush(stdout);
+    return flops;
+  }
+
+  // --- algorithms ---------------------------------------------------------------
+
+  void read(const NodePtr &args, std::map<std::string, ObjectPtr> &results) {
+    const std::string file = yaml::get_text(args, "fileName", "");
+    ReadItem item;
+    try {
+      item = read_file(file);
+    } catch (const std::exception &e) {
+      throw std::runtime_error("Failed to read file '" + file + "': " + e.what());
+    }
+    ObjectPtr o;
+    if (item.kind == ReadItem::Kind::Tensor) {
+      o = make_object(Object::Kind::Tensor);
+      o->tensor = std::make_shared<HostTensor>(std::move(item.tensor));
+    } else if (item.kind == ReadItem::Kind::TensorSet) {
+      o = make_object(Object::Kind::TensorSet);
+      o->set = std::make_shared<HostTensorSet>(std::move(item.set));
+    } else {
+      o = make_object(Object::Kind::Data);
+      o->data = item.data;
+    }
+    results["destination"] = o;
+  }
+
+  void define_holes_and_particles(const NodePtr &args, std::map<std::string, ObjectPtr> &results) {
+    auto eps = symbol(args, "eigenEnergies");
+    if (eps->kind != Object::Kind::Tensor || eps->tensor->complex || eps->tensor->lens.size() != 1)
+      throw std::runtime_error("expecting a Real64 tensor of rank 1 as 'eigenEnergies'");
+    const auto &t = *eps->tensor;
+    auto meta = t.meta_data;
+    auto energies = meta ? meta->get("energies") : nullptr;
+    if (!energies || !energies->is_map()) throw std::runtime_error("eigenEnergies without metaData.energies");
+    const int64_t Np = static_cast<int64_t>(energies->size());
+    const double fermi = yaml::get_real(meta, "fermiEnergy", 0.0);
+    if (!meta->has("fermiEnergy")) throw std::runtime_error("eigenEnergies without metaData.fermiEnergy");
+    int64_t No = 0;
+    while (No < Np && energies->get(No)->real() < fermi) ++No;
+    if (No == 0) throw std::runtime_error("Fermi energy below all eigen energies.");
+    if (No >= Np) throw std::runtime_error("Fermi energy above all eigen energies.");
+    if (t.lens[0] < Np) throw std::runtime_error("eigenEnergies shorter than metaData.energies");
+    std::printf("number of holes     No: %lld\nnumber of particles Nv: %lld\nnumber of states    Np: %lld\n",
+                static_cast<long long>(No), static_cast<long long>(Np - No),
+                static_cast<long long>(Np));
+    // eps_h = eps[0, No), eps_p = eps[No, Np) of the tensor elements (:65-83)
+    HostTensor h, p;
+    h.lens = {No};
+    p.lens = {Np - No};
+    h.dimension_types = p.dimension_types = t.dimension_types;
+    h.unit = p.unit = t.unit;
+    h.meta_data = p.meta_data = t.meta_data;
+    h.data.assign(t.data.begin(), t.data.begin() + No);
+    p.data.assign(t.data.begin() + No, t.data.begin() + Np);
+    auto o = make_object(Object::Kind::TensorSet);
+    o->set = std::make_shared<HostTensorSet>();
+    (*o->set)["h"] = std::move(h);
+    (*o->set)["p"] = std::move(p);
+    results["slicedEigenEnergies"] = o;
+  }
+
+  static void sliced_counts(const ObjectPtr &e, int64_t &No, int64_t &Nv) {
+    if (!e || e->kind != Object::Kind::TensorSet || !e->set->count("h") || !e->set->count("p"))
+      throw std::runtime_error("expecting TensorSet of Real64 as 'slicedEigenEnergies'");
+    No = e->set->at("h").lens.at(0);
+    Nv = e->set->at("p").lens.at(0);
+  }
+
+  void slice_operator(const NodePtr &args, std::map<std::string, ObjectPtr> &results) {
+    auto e = symbol(args, "slicedEigenEnergies");
+    auto op = symbol(args, "operator");
+    int64_t No, Nv;
+    sliced_counts(e, No, Nv);
+    if (op->kind != Object::Kind::Tensor) throw std::runtime_error("expecting a tensor as 'operator'");
+    const auto &t = *op->tensor;
+    // the CCSD path slices the vertex [G, State, State]
+    if (t.lens.size() != 3 || t.dimension_types.size() != 3 || t.dimension_types[1] != "State" ||
+        t.dimension_types[2] != "State" || t.dimension_types[0] == "State")
+      throw std::runtime_error("expecting an operator [AuxiliaryField, State, State] to slice");
+    if (t.lens[1] != t.lens[2] || t.lens[1] < No + Nv)
+      throw std::runtime_error("operator has fewer states than holes + particles");
+    if (!t.complex) throw std::runtime_error("expecting a Complex64 Coulomb vertex");
+    std::printf("Slicing CoulombVertex into holes and particles.\n");
+    auto o = make_object(Object::Kind::SlicedVertex);
+    o->vertex = op->tensor;
+    o->No = No;
+    o->Nv = Nv;
+    results["slicedOperator"] = o;
+  }
+
+  void vertex_coulomb_integrals(const NodePtr &args, std::map<std::string, ObjectPtr> &results) {
+    auto v = symbol(args, "slicedCoulombVertex");
+    if (v->kind != Object::Kind::SlicedVertex) throw std::runtime_error("expecting a sliced Coulomb vertex");
+    const bool half = yaml::get_boolean(v->vertex->meta_data, "halfGrid", false);
+    std::printf("Using %s Coulomb integrals\n", half ? "real" : "complex");
+    auto o = make_object(Object::Kind::Integrals);
+    o->real_field = half;
+    o->sliced_vertex = v;
+    results["coulombIntegrals"] = o;
+  }
+
+  double coupled_cluster(const NodePtr &args, NodePtr &in_report,
+                         std::map<std::string, ObjectPtr> &results, NodePtr &out_report) {
+    auto ints = symbol(args, "coulombIntegrals");
+    auto eps = symbol(args, "slicedEigenEnergies");
+    auto vtx = symbol(args, "slicedCoulombVertex");
+    if (ints->kind != Object::Kind::Integrals) throw std::runtime_error("expecting Coulomb integrals as 'coulombIntegrals'");
+    if (vtx->kind != Object::Kind::SlicedVertex) throw std::runtime_error("expecting a sliced Coulomb vertex");
+    const std::string method = yaml::get_text(args, "method", "Ccsd");
+    if (method != "Ccsd") throw std::runtime_error("Unknown method: '" + method + "'");
+    auto mixer = args->get("mixer");
+    const std::string mixer_type = yaml::get_text(mixer, "type", "DiisMixer");
+    if (mixer_type != "DiisMixer") throw std::runtime_error("Unknown mixer type: '" + mixer_type + "'");
+    int64_t No, Nv;
+    sliced_counts(eps, No, Nv);
+    if (No != vtx->No || Nv != vtx->Nv) throw std::runtime_error("slices of different hole/particle counts");
+
+    cc4s_ccsd_params p;
+    cc4s_ccsd_default_params(&p);
+    p.field = ints->real_field ? CC4S_FIELD_REAL : CC4S_FIELD_COMPLEX;
+    p.No = No;
+    p.Nv = Nv;
+    p.NG = vtx->vertex->lens[0];
+    p.max_iterations = yaml::get_integer(args, "maxIterations", p.max_iterations);
+    p.energy_convergence = yaml::get_real(args, "energyConvergence", p.energy_convergence);
+    p.amplitudes_convergence = yaml::get_real(args, "amplitudesConvergence", p.amplitudes_convergence);
+    p.level_shift = yaml::get_real(args, "levelShift", p.level_shift);
+    p.max_residua = yaml::get_integer(mixer, "maxResidua", p.max_residua);
+    p.ppl = yaml::get_boolean(args, "ppl", true) ? 1 : 0;
+    p.integrals_slice_size = yaml::get_integer(args, "integralsSliceSize", No);
+    const std::string precision = yaml::get_text(args, "precision", "fp64");
+    if (precision == "mixed")
+      p.precision = CC4S_PRECISION_MIXED;
+    else if (precision != "fp64")
+      throw std::runtime_error("unknown precision '" + precision + "' (fp64 | mixed)");
+    p.mixed_switch = yaml::get_real(args, "mixedSwitch", p.mixed_switch);
+    p.device = static_cast<int>(yaml::get_integer(args, "device", 0));
+    // the report lists the defaults used, as the reference's argument map does
+    in_report->set("method", method);
+    in_report->set("maxIterations", p.max_iterations);
+    in_report->set("energyConvergence", p.energy_convergence);
+    in_report->set("amplitudesConvergence", p.amplitudes_convergence);
+    in_report->set("levelShift", p.level_shift);
+    in_report->set("ppl", static_cast<int64_t>(p.ppl));
+    in_report->set("integralsSliceSize", p.integrals_slice_size);
+
+    ObjectPtr initial = symbol(args, "initialAmplitudes", false);
+    if (args->has("initialAmplitudes") && !initial)
+      throw std::runtime_error("'initialAmplitudes' does not name a stored object");
+
+    // Gamma [NG, No+Nv, No+Nv]: the vertex itself, or its hole and particle
+    // states when it has more states (SliceOperator.cxx:112-118)
+    const HostTensor &V = *vtx->vertex;
+    const int64_t NG = V.lens[0], NpV = V.lens[1], Np = No + Nv;
+    std::vector<double> compact;
+    const double *gamma = V.data.data();
+    if (NpV != Np) {
+      compact.resize(static_cast<size_t>(NG) * Np * Np * 2);
+      auto state = [&](int64_t q) { return q < No ? q : NpV - Nv + (q - No); };
+      for (int64_t q = 0; q < Np; ++q)
+        for (int64_t r = 0; r < Np; ++r)
+          std::memcpy(&compact[2 * NG * (r + Np * q)],
+                      &V.data[2 * NG * (state(r) + NpV * state(q))], 2 * NG * sizeof(double));
+      gamma = compact.data();
+    }
+    std::vector<double> e(Np);
+    const auto &h = eps->set->at("h").data;
+    const auto &pp = eps->set->at("p").data;
+    std::copy(h.begin(), h.end(), e.begin());
+    std::copy(pp.begin(), pp.end(), e.begin() + No);
+
+    if (opt_.dry_run) {
+      std::printf("dry run: CCSD %s field, No=%lld Nv=%lld NG=%lld\n",
+                  ints->real_field ? "real" : "complex", static_cast<long long>(No),
+                  static_cast<long long>(Nv), static_cast<long long>(NG));
+      out_report->set("dryRun", static_cast<int64_t>(1));
+      return 0;
+    }
+
+    cc4s_ccsd_context *ctx = nullptr;
+    auto check = [&](int rc) {
+      if (rc != 0) {
+        std::string msg = cc4s_ccsd_last_error(ctx);
+        if (ctx) cc4s_ccsd_destroy(ctx);
+        ctx = nullptr;
+        throw std::runtime_error(msg);
+      }
+    };
+    check(cc4s_ccsd_create(&p, &ctx));
+    check(cc4s_ccsd_set_inputs(ctx, gamma, e.data()));
+    const bool cplx = !ints->real_field;
+    const size_t es = cplx ? 2 : 1;
+    if (initial) {
+      if (initial->kind != Object::Kind::TensorSet || !initial->set->count("ph") ||
+          !initial->set->count("pphh"))
+        throw std::runtime_error("expecting a TensorSet {ph, pphh} as 'initialAmplitudes'");
+      const HostTensor &t1 = initial->set->at("ph"), &t2 = initial->set->at("pphh");
+      if (t1.complex != cplx || t2.complex != cplx || t1.elements() != Nv * No ||
+          t2.elements() != Nv * Nv * No * No)
+        throw std::runtime_error("initialAmplitudes do not match the field or the sizes");
+      check(cc4s_ccsd_set_amplitudes(ctx, t1.data.data(), t2.data.data()));
+      std::printf("Using given initial amplitudes\n");
+    }
+    std::printf("Using method Ccsd. integralsSliceSize: %lld\nUsing mixer DiisMixer. maxResidua: %lld\n",
+                static_cast<long long>(p.integrals_slice_size), static_cast<long long>(p.max_residua));
+    std::printf("Maximum number of iterations: %lld\nUnless reaching energy convergence dE: %g\n"
+                "and amplitudes convergence dR: %g\n",
+                static_cast<long long>(p.max_iterations), p.energy_convergence,
+                p.amplitudes_convergence);
+    std::printf("Iter         Energy         dE           dR         time   TFLOP/s\n");
+    std::vector<cc4s_ccsd_iteration> trace(static_cast<size_t>(std::max<int64_t>(1, p.max_iterations)));
+    cc4s_ccsd_result r;
+    check(cc4s_ccsd_run(ctx, &r, trace.data(), static_cast<int64_t>(trace.size())));
+    double flops = 0;
+    for (int64_t k = 0; k < r.iterations && k < static_cast<int64_t>(trace.size()); ++k) {
+      const auto &it = trace[k];
+      flops += it.flops;
+      std::printf("%4lld %22.15f %12.5e %12.5e %8.3f %8.2f%s\n", static_cast<long long>(it.iteration),
+                  it.energy, it.delta_energy, it.residuum_norm, it.seconds,
+                  it.seconds > 0 ? it.flops / it.seconds / 1e12 : 0.0,
+                  it.precision_used == CC4S_PRECISION_MIXED ? "  (mixed)" : "");
+    }
+    std::printf("\nCcsd correlation energy:          %.12f\n", r.correlation);
+    std::printf("2nd-order correlation energy:     %.12f\n", r.second_order);
+    if (!r.convergence_reached) std::printf("A converged solution was not found\n");
+    std::fflush(stdout);
+
+    auto amps = make_object(Object::Kind::TensorSet);
+    amps->set = std::make_shared<HostTensorSet>();
+    HostTensor t1, t2;
+    t1.complex = t2.complex = cplx;
+    t1.lens = {Nv, No};
+    t2.lens = {Nv, Nv, No, No};
+    t1.data.resize(static_cast<size_t>(Nv * No) * es);
+    t2.data.resize(static_cast<size_t>(Nv * Nv * No * No) * es);
+    check(cc4s_ccsd_get_amplitudes(ctx, t1.data.data(), t2.data.data()));
+    cc4s_ccsd_destroy(ctx);
+    (*amps->set)["ph"] = std::move(t1);
+    (*amps->set)["pphh"] = std::move(t2);
+
+    // CoupledCluster.cxx:184-191,245-248; unit = unit(Gamma)^2
+    auto energy = Node::map();
+    energy->set("correlation", r.correlation);
+    energy->set("direct", r.direct);
+    energy->set("exchange", r.exchange);
+    if (!std::isnan(r.second_order)) energy->set("secondOrder", r.second_order);
+    energy->set("unit", V.unit * V.unit);
+    results["amplitudes"] = amps;
+    auto eobj = make_object(Object::Kind::Data);
+    eobj->data = energy;
+    results["energy"] = eobj;
+    auto cobj = make_object(Object::Kind::Data);
+    cobj->data = Node::scalar(r.convergence_reached ? "1" : "0");
+    results["convergenceReached"] = cobj;
+    return flops;
+  }
+
+  void write(const NodePtr &args, NodePtr &out_report) {
+    auto src = symbol(args, "source");
+    const std::string file = yaml::get_text(args, "fileName", "");
+    const bool binary = yaml::get_boolean(args, "binary", false);
+    NodePtr written;
+    if (src->kind == Object::Kind::Tensor) {
+      written = write_tensor_file(*src->tensor, file, binary);
+    } else if (src->kind == Object::Kind::TensorSet) {
+      written = write_tensor_set_file(*src->set, file, binary);
+    } else if (src->kind == Object::Kind::Data) {
+      yaml::emit_file(src->data, file);
+      written = src->data;
+    } else {
+      throw std::runtime_error("Write supports tensors, tensor sets and data");
+    }
+    out_report->set("persitentSource", written);  // key spelled as Write.cxx:39
+  }
+
+  Options opt_;
+  std::map<std::string, ObjectPtr> storage_;
+  NodePtr report_, executed_;
+};
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  Options opt;
+  for (int a = 1; a < argc; ++a) {
+    const std::string s = argv[a];
+    auto value = [&]() -> std::string {
+      if (a + 1 >= argc) throw std::runtime_error("missing value after " + s);
+      return argv[++a];
+    };
+    try {
+      if (s == "-i" || s == "--in")
+        opt.in = value();
+      else if (s == "-o" || s == "--out")
+        opt.out = value();
+      else if (s == "-l" || s == "--log")
+        (void)value();  // logging goes to stdout
+      else if (s == "-d" || s == "--dry-run")
+        opt.dry_run = true;
+      else if (s == "-h" || s == "--help") {
+        std::printf("usage: cc4s_ccsd [-i cc4s.in] [-o cc4s.out.yaml] [--dry-run]\n");
+        return 0;
+      } else {
+        throw std::runtime_error("unknown option " + s);
+      }
+    } catch (const std::exception &e) {
+      std::fprintf(stderr, "cc4s_ccsd: %s\n", e.what());
+      return 1;
+    }
+  }
+  try {
+    return Runner(opt).run();
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "ERROR: %s\n", e.what());
+    std::fflush(stdout);
+    return 1;
+  }
+}

@@ -692,24 +692,6 @@ bool is_hermitian(hipStream_t s, const void *gamma, int64_t NG, int64_t Np) {
   return h.x == 0.0;
 }
 
-namespace {
-
-__global__ __launch_bounds__(kBlock) void axpby_kernel(int64_t n, double a,
-                                                       const double *x, double b,
-                                                       double *y) {
-  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
-       e += int64_t(gridDim.x) * kBlock)
-    y[e] = a * x[e] + b * y[e];
-}
-
-}  // namespace
-
-void axpby(hipStream_t s, int64_t n, double a, const double *x, double b, double *y) {
-  if (n <= 0) return;
-  axpby_kernel<<<grid_for(n), kBlock, 0, s>>>(n, a, x, b, y);
-  CC4S_HIP(hipGetLastError());
-}
-
 void copy(hipStream_t s, void *y, const void *x, size_t bytes) {
   CC4S_HIP(hipMemcpyAsync(y, x, bytes, hipMemcpyDeviceToDevice, s));
 }

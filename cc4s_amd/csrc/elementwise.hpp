@@ -79,9 +79,6 @@ double norm_packed(hipStream_t s, Field f, const void *const *R,
 // Gamma[G,q,p] == conj(Gamma[G,p,q]) exactly for every G, p, q
 bool is_hermitian(hipStream_t s, const void *gamma, int64_t NG, int64_t Np);
 
-// y = a x + b y over n doubles
-void axpby(hipStream_t s, int64_t n, double a, const double *x, double b, double *y);
-
 // y = x (device copy on the stream)
 void copy(hipStream_t s, void *y, const void *x, size_t bytes);
 

@@ -1,7 +1,8 @@
 // Contraction-kernel benchmark on the CCSD statement shapes.
 //
 //   gett_bench [generation=2] [repetitions=5] [No Nv NG slice]
-//   (generation 1, 2: FP64 kernels; 3: split-bf16 mixed precision)
+//   (generation 1, 2: FP64 kernels; 3: split-bf16 mixed precision;
+//    0: a plain 1 GiB device copy kernel, the reference for HBM counters)
 //
 // Runs the complex GETT on device buffers for the shapes of the ladder
 // (formation and contraction, Ccsd.cxx:588-590), the ring and crossed-ring
@@ -24,6 +25,34 @@ __global__ void fill(double *p, size_t n, double s) {
     p[i] = s * static_cast<double>((i * 2654435761ull) % 1000) / 1000.0 - 0.5 * s;
 }
 
+// reads and writes n doubles once: 8n bytes fetched, 8n bytes written
+__global__ void copy_kernel(const double *__restrict__ x, double *__restrict__ y, size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull)
+    y[i] = x[i];
+}
+
+int copy_reference(int reps) {
+  const size_t n = size_t(1) << 27;  // 1 GiB of doubles
+  double *x = nullptr, *y = nullptr;
+  CC4S_HIP(hipMalloc(&x, n * 8));
+  CC4S_HIP(hipMalloc(&y, n * 8));
+  fill<<<4096, 256>>>(x, n, 1.0);
+  hipEvent_t e0, e1;
+  CC4S_HIP(hipEventCreate(&e0));
+  CC4S_HIP(hipEventCreate(&e1));
+  CC4S_HIP(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < reps; ++r) copy_kernel<<<8192, 256>>>(x, y, n);
+  CC4S_HIP(hipEventRecord(e1, nullptr));
+  CC4S_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  CC4S_HIP(hipEventElapsedTime(&ms, e0, e1));
+  std::printf("copy 1 GiB: %.3f ms/call, %.1f GB/s (read + write)\n", ms / reps,
+              2.0 * n * 8 / (ms / reps * 1e-3) / 1e9);
+  (void)hipFree(x);
+  (void)hipFree(y);
+  return 0;
+}
+
 struct Case {
   const char *name;
   std::string ia, ib, ic;
@@ -35,6 +64,14 @@ struct Case {
 int main(int argc, char **argv) {
   const int gen = argc > 1 ? std::atoi(argv[1]) : 2;
   const int reps = argc > 2 ? std::atoi(argv[2]) : 5;
+  if (gen == 0) {
+    try {
+      return copy_reference(reps);
+    } catch (const std::exception &e) {
+      std::fprintf(stderr, "gett_bench: %s\n", e.what());
+      return 1;
+    }
+  }
   const int64_t o = argc > 3 ? std::atoll(argv[3]) : 32;
   const int64_t v = argc > 4 ? std::atoll(argv[4]) : 256;
   const int64_t G = argc > 5 ? std::atoll(argv[5]) : 1500;

@@ -295,3 +295,44 @@ def random_amplitudes(No: int, Nv: int, field: str, seed: int,
     T2 = t2.reshape(Nv, Nv, No, No, order="F")
     T2 = 0.5 * (T2 + T2.transpose(1, 0, 3, 2))
     return np.asfortranarray(T1), np.asfortranarray(T2)
+
+
+def _tensor_header(scalar_type, dims, binary, unit=1.0, meta=None):
+    """cc4s tensor header (TensorIo.cxx:114-158), dims = [(length, type)]."""
+    return {
+        "type": "Tensor",
+        "dimensions": {i: {"length": int(n), "type": t} for i, (n, t) in enumerate(dims)},
+        "scalarType": scalar_type,
+        "version": 100,
+        "elements": {"type": "IeeeBinaryFile" if binary else "TextFile"},
+        "unit": float(unit),
+        "metaData": meta or {},
+    }
+
+
+def write_cc4s_inputs(inp: CcsdInputs, directory, binary: bool = True) -> None:
+    """Writes the inputs as the files a cc4s run reads: CoulombVertex.yaml /
+    .elements (Complex64 [AuxiliaryField NG, State Np, State Np], metaData
+    halfGrid) and EigenEnergies.yaml / .elements (Real64 [State Np], metaData
+    fermiEnergy and energies, DefineHolesAndParticles.cxx:42-59)."""
+    import pathlib
+    import yaml as _yaml
+    d = pathlib.Path(directory)
+    d.mkdir(parents=True, exist_ok=True)
+    NG, Np = inp.NG, inp.Np
+    hv = _tensor_header("Complex64", [(NG, "AuxiliaryField"), (Np, "State"), (Np, "State")],
+                        binary, meta={"halfGrid": inp.half_grid})
+    he = _tensor_header("Real64", [(len(inp.eps), "State")], binary,
+                        meta={"fermiEnergy": float(inp.fermi_energy),
+                              "energies": [float(e) for e in inp.eps]})
+    for name, header, values in (("CoulombVertex", hv, np.asfortranarray(inp.gamma)),
+                                 ("EigenEnergies", he, np.asarray(inp.eps, dtype=np.float64))):
+        (d / f"{name}.yaml").write_text(_yaml.safe_dump(header, sort_keys=False))
+        flat = values.reshape(-1, order="F")
+        if binary:
+            flat.tofile(d / f"{name}.elements")
+        elif np.iscomplexobj(flat):
+            (d / f"{name}.elements").write_text(
+                "".join(f"({v.real:.17g},{v.imag:.17g})\n" for v in flat))
+        else:
+            (d / f"{name}.elements").write_text("".join(f"{v:.17g}\n" for v in flat))

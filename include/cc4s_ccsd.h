@@ -47,15 +47,12 @@ enum { CC4S_FIELD_REAL = 0, CC4S_FIELD_COMPLEX = 1 };
 /* Arithmetic of the large contractions (ladder, ring, Gamma terms).  FP64
  * reproduces the reference.  MIXED (complex field only) runs them on bf16
  * MFMA with every FP64 operand split into hi + lo parts and FP32
- * accumulation, until the mixer's dR falls below mixed_switch; then FP64
- * residua refine the fixed point.  With refinement 0 every later pass is an
- * FP64 pass.  With refinement 1 (defect correction) a pass evaluates the FP64
- * residuum R64(T_k) and the split-bf16 one Rlo(T_k) at the same amplitudes
- * (an anchor pass, updated with R64), and the passes after it use the
- * corrected split-bf16 residuum Rlo(T) + R64(T_k) - Rlo(T_k) until dR falls
- * below refinement_ratio times the anchor's dR; only anchor passes can
- * declare convergence.  Denominators, energy, DIIS and every small
- * contraction stay FP64.  FP32 is accepted by the energy gradient only. */
+ * accumulation, until the mixer's dR falls below mixed_switch (or stops
+ * falling: three passes without a 1.5x gain, the rounding floor of the split
+ * residuum), then switches to FP64 for the remaining passes (iterative
+ * refinement of the same fixed point); only FP64 passes can declare
+ * convergence.  Denominators, energy, DIIS and every small contraction stay
+ * FP64.  FP32 is accepted by the energy gradient only. */
 enum { CC4S_PRECISION_FP64 = 0, CC4S_PRECISION_MIXED = 1, CC4S_PRECISION_FP32 = 2 };
 
 /* Step arguments of CoupledCluster (CoupledCluster.cxx:59-117,260-262,
@@ -72,8 +69,6 @@ typedef struct {
   int64_t integrals_slice_size;   /* integralsSliceSize (0 = No) */
   int precision;                  /* CC4S_PRECISION_* */
   double mixed_switch;            /* MIXED: dR below which FP64 takes over (1e-5) */
-  int refinement;                 /* MIXED: 0 FP64 passes, 1 defect correction (1) */
-  double refinement_ratio;        /* MIXED, refinement 1: dR reduction per anchor (1e-3) */
   int device;                     /* HIP device ordinal */
 } cc4s_ccsd_params;
 

@@ -45,7 +45,6 @@ def test_default_params_are_the_reference_defaults():
     # mixed-precision settings (no counterpart in the reference)
     assert p.precision == api.PRECISION_FP64
     assert p.mixed_switch == 1e-5
-    assert p.refinement == 1 and p.refinement_ratio == 1e-3
 
 
 def test_unsupported_precision_is_refused_before_touching_a_device():
@@ -54,10 +53,3 @@ def test_unsupported_precision_is_refused_before_touching_a_device():
     with pytest.raises(api.CcsdError, match="complex field"):
         api.CoupledCluster(2, 4, 3, field="real", precision=api.PRECISION_MIXED)
 
-
-def test_invalid_refinement_is_refused_before_touching_a_device():
-    with pytest.raises(api.CcsdError, match="refinement mode"):
-        api.CoupledCluster(2, 4, 3, precision=api.PRECISION_MIXED, refinement=3)
-    with pytest.raises(api.CcsdError, match="refinement_ratio"):
-        api.CoupledCluster(2, 4, 3, precision=api.PRECISION_MIXED,
-                           refinementRatio=1.5)

@@ -1,0 +1,150 @@
+"""The cc4s operation-file driver end to end on the GPU.
+
+A cc4s operation file (Read, DefineHolesAndParticles, SliceOperator,
+VertexCoulombIntegrals, CoupledCluster, Write) runs on cc4s-format input files;
+the report's energies must match the golden fixtures / the oracle to 1e-9 Eh,
+the written Amplitudes the oracle's to 1e-8 relative, and a restart from the
+written Amplitudes (initialAmplitudes) must reproduce the energy.
+"""
+
+import json
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+import yaml
+
+from cc4s_amd import synthetic as syn
+from oracle import ccsd_oracle as O
+from tests.conftest import rel_err
+from tests.test_driver import DRIVER, STEPS, read_tensor
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = pathlib.Path(__file__).parent / "golden"
+
+
+def ccsd_step(cfg, extra=""):
+    return f"""
+- name: CoupledCluster
+  in:
+    method: Ccsd
+    slicedEigenEnergies: EigenEnergies
+    coulombIntegrals: CoulombIntegrals
+    slicedCoulombVertex: CoulombVertex
+    maxIterations: {cfg['max_iterations']}
+    energyConvergence: {cfg['energy_convergence']}
+    amplitudesConvergence: {cfg['amplitudes_convergence']}
+    mixer:
+      type: DiisMixer
+      maxResidua: {cfg['max_residua']}
+{extra}  out:
+    energy: CcsdEnergy
+    amplitudes: Amplitudes
+"""
+
+
+def run(workdir, text):
+    (workdir / "cc4s.in").write_text(text)
+    r = subprocess.run([str(DRIVER), "-i", "cc4s.in"], cwd=workdir,
+                       capture_output=True, text=True, timeout=600)
+    print(r.stdout[-3000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    return yaml.safe_load((workdir / "cc4s.out.yaml").read_text())
+
+
+def ccsd_out(report):
+    for k in sorted(report["steps"]):
+        if report["steps"][k]["name"] == "CoupledCluster":
+            return report["steps"][k]["out"]
+    raise AssertionError("no CoupledCluster step in the report")
+
+
+def amplitudes(directory, name="Amplitudes"):
+    h = yaml.safe_load((directory / f"{name}.yaml").read_text())
+    assert h["type"] == "TensorSet"
+    c = h["components"]
+    return (read_tensor(directory, f"{name}.components.ph", c["ph"]),
+            read_tensor(directory, f"{name}.components.pphh", c["pphh"]))
+
+
+@pytest.mark.parametrize("golden,binary", [("config0", True), ("small_complex", False)],
+                         ids=["config0_real_binary", "small_complex_text"])
+def test_operation_file_matches_golden_and_oracle(tmp_path, golden, binary):
+    gold = json.loads((GOLDEN / f"{golden}.json").read_text())
+    c = gold["config"]
+    print(f"driver {golden}", flush=True)
+    inp = syn.generate(c["No"], c["Nv"], c["NG"], c["field"], c["seed"])
+    syn.write_cc4s_inputs(inp, tmp_path, binary=binary)
+    write = f"""
+- name: Write
+  in:
+    source: Amplitudes
+    fileName: "Amplitudes.yaml"
+    binary: {1 if binary else 0}
+  out: {{}}
+"""
+    rep = run(tmp_path, STEPS + ccsd_step(c) + write)
+    out = ccsd_out(rep)
+    assert out["convergenceReached"] == 1
+    for key in ("correlation", "direct", "exchange", "secondOrder"):
+        assert abs(out["energy"][key] - gold["energy"][key]) < 1e-9, key
+    assert out["energy"]["unit"] == 1.0
+    T1, T2 = amplitudes(tmp_path)
+    P = O.CcsdProblem(inp.gamma, inp.eps, c["No"], c["field"])
+    ref = O.coupled_cluster(P, max_iterations=c["max_iterations"],
+                            energy_convergence=c["energy_convergence"],
+                            amplitudes_convergence=c["amplitudes_convergence"],
+                            max_residua=c["max_residua"])
+    assert rel_err(T1, ref["amplitudes"][0]) < 1e-8
+    assert rel_err(T2, ref["amplitudes"][1]) < 1e-8
+
+    # restart from the written amplitudes (CoupledCluster.cxx:70-75)
+    restart = STEPS + """
+- name: Read
+  in:
+    fileName: "Amplitudes.yaml"
+  out:
+    destination: Start
+""" + ccsd_step(c, "    initialAmplitudes: Start\n")
+    rep2 = run(tmp_path, restart)
+    out2 = ccsd_out(rep2)
+    assert out2["convergenceReached"] == 1
+    assert abs(out2["energy"]["correlation"] - gold["energy"]["correlation"]) < 1e-9
+    assert rep2["steps"][6]["statistics"]["floatingPointOperations"] < \
+        rep["steps"][5]["statistics"]["floatingPointOperations"]
+
+
+def test_operation_file_mixed_precision(tmp_path):
+    gold = json.loads((GOLDEN / "small_complex.json").read_text())
+    c = gold["config"]
+    inp = syn.generate(c["No"], c["Nv"], c["NG"], c["field"], c["seed"])
+    syn.write_cc4s_inputs(inp, tmp_path)
+    rep = run(tmp_path, STEPS + ccsd_step(c, "    precision: mixed\n    mixedSwitch: 1.0e-5\n"))
+    out = ccsd_out(rep)
+    assert out["convergenceReached"] == 1
+    assert abs(out["energy"]["correlation"] - gold["energy"]["correlation"]) < 1e-8
+
+
+def test_vertex_with_more_states_than_the_energies(tmp_path):
+    """SliceOperator takes holes [0, No) and particles [Np - Nv, Np) of the
+    vertex (SliceOperator.cxx:112-118): a vertex with extra states between
+    them gives the energy of its hole/particle sub-vertex."""
+    No, Nv, NG = 4, 12, 30
+    inp = syn.generate(No, Nv, NG, "complex", 8)
+    extra = 3
+    Np = No + Nv
+    big = np.zeros((NG, Np + extra, Np + extra), dtype=np.complex128, order="F")
+    idx = list(range(No)) + list(range(No + extra, Np + extra))
+    big[np.ix_(range(NG), idx, idx)] = inp.gamma
+    wide = syn.CcsdInputs(gamma=big, eps=inp.eps, No=No, field="complex", scale=inp.scale)
+    syn.write_cc4s_inputs(wide, tmp_path)
+    # the energies tensor keeps No + Nv states
+    cfg = dict(max_iterations=30, energy_convergence=1e-10, amplitudes_convergence=1e-8,
+               max_residua=4)
+    rep = run(tmp_path, STEPS + ccsd_step(cfg))
+    P = O.CcsdProblem(inp.gamma, inp.eps, No, "complex")
+    ref = O.coupled_cluster(P, max_iterations=30, energy_convergence=1e-10,
+                            amplitudes_convergence=1e-8, max_residua=4)
+    assert abs(ccsd_out(rep)["energy"]["correlation"] - ref["energy"]["correlation"]) < 1e-9

@@ -66,17 +66,21 @@ def test_mixed_residuum_accuracy():
     assert rel_err(out[1][0], out[0][0]) < 1e-4
 
 
-@pytest.mark.parametrize("refinement", [0, 1], ids=["fp64_passes", "defect_correction"])
+@pytest.mark.parametrize("switch", [1e-5, 1e-7, 1e-14],
+                         ids=["written_1e-5", "tuned_1e-7", "floor_guard"])
 @pytest.mark.parametrize("shape", [(6, 24, 60), (12, 60, 300)])
-def test_mixed_run_reaches_fp64_energy(shape, refinement):
+def test_mixed_run_reaches_fp64_energy(shape, switch):
+    """The mixed run converges to the FP64 energy; once switched it stays
+    FP64; a switch below the split residuum's rounding floor (1e-14) is
+    ended by the stall guard instead of running to maxIterations."""
     No, Nv, NG = shape
-    print(f"mixed run {shape} refinement {refinement}", flush=True)
+    print(f"mixed run {shape} switch {switch}", flush=True)
     inp = syn.generate(No, Nv, NG, "complex", 42, scale_rule="mp2_per_occupied")
     kw = dict(maxIterations=60, energyConvergence=1e-10,
               amplitudesConvergence=1e-8, maxResidua=8)
     res = {}
     for prec in (api.PRECISION_FP64, api.PRECISION_MIXED):
-        with api.CoupledCluster(No, Nv, NG, precision=prec, refinement=refinement,
+        with api.CoupledCluster(No, Nv, NG, precision=prec, mixedSwitch=switch,
                                 **kw) as cc:
             cc.set_inputs(inp.gamma, inp.eps)
             res[prec] = cc.run()
@@ -85,15 +89,8 @@ def test_mixed_run_reaches_fp64_energy(shape, refinement):
     assert abs(mixed["energy"]["correlation"] - fp64["energy"]["correlation"]) < 1e-8
     used = [t["precision_used"] for t in mixed["trace"]]
     assert used[0] == api.PRECISION_MIXED and used[-1] == api.PRECISION_FP64
+    # once switched, FP64 to the end; only FP64 passes declare convergence
     first64 = used.index(api.PRECISION_FP64)
-    if refinement == 0:
-        # once switched, FP64 to the end
-        assert all(u == api.PRECISION_FP64 for u in used[first64:])
-    else:
-        # anchors (FP64 residuum) with corrected split-bf16 passes between
-        # them; the switch happened below the threshold
-        assert mixed["trace"][first64 - 1]["residuum_norm"] < 1e-5
-        assert sum(1 for u in used[first64:] if u == api.PRECISION_FP64) <= 4
-        # only FP64 passes declare convergence
-        assert all(not t["converged"] for t in mixed["trace"]
-                   if t["precision_used"] == api.PRECISION_MIXED)
+    assert all(u == api.PRECISION_FP64 for u in used[first64:])
+    assert not any(t["converged"] for t in mixed["trace"][:first64])
+    assert mixed["iterations"] <= fp64["iterations"] + 6
