@@ -21,6 +21,7 @@ FIELD_COMPLEX = 1
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 PRECISION_FP32 = 2
+METHODS = {"Ccsd": 0, "Drccd": 1}
 
 # every symbol declared in include/cc4s_ccsd.h
 EXPORTS = (
@@ -50,6 +51,9 @@ class Params(C.Structure):
         ("precision", C.c_int),
         ("mixed_switch", C.c_double),
         ("device", C.c_int),
+        ("method", C.c_int),
+        ("linearized", C.c_int),
+        ("adjacent_pairs_exchange", C.c_int),
     ]
 
 
@@ -156,7 +160,8 @@ class CoupledCluster:
                  energyConvergence=1e-7, amplitudesConvergence=1e-6,
                  levelShift=0.0, maxResidua=4, ppl=True,
                  integralsSliceSize=0, device=0, precision=PRECISION_FP64,
-                 mixedSwitch=1e-5):
+                 mixedSwitch=1e-5, method="Ccsd", linearized=False,
+                 adjacentPairsExchange=False):
         self.lib = load_library()
         p = default_params()
         p.field = FIELD_COMPLEX if field == "complex" else FIELD_REAL
@@ -171,6 +176,11 @@ class CoupledCluster:
         p.device = device
         p.precision = precision
         p.mixed_switch = mixedSwitch
+        if method not in METHODS:
+            raise CcsdError(f"Unknown method: '{method}'")
+        p.method = METHODS[method]
+        p.linearized = 1 if linearized else 0
+        p.adjacent_pairs_exchange = 1 if adjacentPairsExchange else 0
         self.params = p
         self.field = field
         self.No, self.Nv, self.NG = No, Nv, NG

@@ -82,6 +82,11 @@ int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
     CoupledClusterParams p;
     p.mixed = params->precision == CC4S_PRECISION_MIXED;
     p.mixed_switch = params->mixed_switch;
+    CC4S_ASSERT(params->method == CC4S_METHOD_CCSD || params->method == CC4S_METHOD_DRCCD,
+                "Unknown method");
+    p.ccsd.method = params->method == CC4S_METHOD_DRCCD ? Method::Drccd : Method::Ccsd;
+    p.ccsd.linearized = params->linearized != 0;
+    p.ccsd.adjacent_pairs_exchange = params->adjacent_pairs_exchange != 0;
     p.shape.field = params->field == CC4S_FIELD_REAL ? Field::Real : Field::Complex;
     p.shape.No = params->No;
     p.shape.Nv = params->Nv;

@@ -70,6 +70,8 @@ Ccsd::Ccsd(ContractionEngine &E, const CcsdShape &shape, const CcsdOptions &opt,
   if (f == Field::Complex) {
     w.Y.reset(f, "abij", {v, v, o, o});
     w.Zgoo.reset(f, "Gkj", {G, o, o});
+  } else if (opt_.method == Method::Drccd && opt_.adjacent_pairs_exchange) {
+    w.Y.reset(f, "abij", {v, v, o, o});
   }
 }
 
@@ -153,7 +155,9 @@ bool Ccsd::vertex_is_hermitian() const { return w_->hermitian; }
 
 void Ccsd::get_residuum(const void *T, void *R) {
   auto t0 = std::chrono::steady_clock::now();
-  if (s_.field == Field::Complex)
+  if (opt_.method == Method::Drccd)
+    residuum_drccd(T, R);
+  else if (s_.field == Field::Complex)
     residuum_complex(T, R);
   else
     residuum_real(T, R);
@@ -225,6 +229,57 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
     timings.ladder_calls += 1;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+  }
+}
+
+// Drccd::getResiduum (Drccd.cxx:31-109), both fields.  The phhp and hhpp
+// integrals are contracted through Gamma:
+//   2 V_akic T_cbkj = 2 sum_G L[G,a,i] Y[G,b,j],
+//       Y[G,b,j] = sum_kc Gamma_hp[G,k,c] T_cbkj;
+//   2 W_klcd T_acik T_dblj with W = 2 V: 4 sum_G A[G,a,i] Y[G,b,j],
+//       A[G,a,i] = sum_kc M[G,k,c] T_acik;
+// complex: L[G,a,i] = conj Gamma_hp[G,i,a], M[G,k,c] = conj Gamma_ph[G,c,k]
+// (V[p,q,s,r] = sum_G conj(Gamma[G,s,p]) Gamma[G,q,r]); real: L = Gamma_ph,
+// M = Gamma_hp over the Re/Im index (VertexCoulombIntegrals.cxx:68-102).
+// With adjacentPairsExchange the -V_lkcd part of W does not factorise and is
+// contracted from the explicit integrals.
+void Ccsd::residuum_drccd(const void *Tp, void *Rp) {
+  auto &w = *w_;
+  auto &E = E_;
+  const Field f = s_.field;
+  const bool cplx = f == Field::Complex;
+  const int64_t o = s_.No, v = s_.Nv;
+  const hipStream_t st = E.stream();
+  const View R1 = dense_view(Rp, f, "ai", {v, o});
+  const View R2 = dense_view(offset_ptr(Rp, s_.n1(), f), f, "abij", {v, v, o, o});
+  E.scale(0.0, R1);                                               // :61,76
+  if (!Tp) {                                                      // :62
+    E.sum(1.0, w.Vpphh("abij"), 0.0, R2);
+    return;
+  }
+  const View T2 = dense_view(offset_ptr(Tp, s_.n1(), f), f, "abij", {v, v, o, o});
+  const View Yg = w.Wgvo("Gbj");
+  E.contract(1.0, gam("hp", "Gkc"), T2.as("cbkj"), 0.0, Yg);
+  E.contract(2.0, gam("ph", "Gai", cplx), Yg, 0.0, R2);          // :77
+  symmetrize_pairs(st, f, R2.ptr, o, v);                          // :78
+  E.sum(1.0, w.Vpphh("abij"), 1.0, R2);                           // :79
+  if (opt_.linearized) return;                                    // :80-83
+  const View Ag = w.Zgvo("Gai");
+  E.contract(1.0, gam("hp", "Gkc", cplx), T2.as("acik"), 0.0, Ag);
+  E.contract(4.0, Ag, Yg, 1.0, R2);                               // :97-98
+  if (opt_.adjacent_pairs_exchange) {                             // :88-91
+    View Vlkcd;
+    if (cplx) {
+      Vlkcd = w.Vhhpp_view.as("lkcd");
+    } else {
+      // real V_klcd = sum_G' Gamma_hp[G',k,c] Gamma_hp[G',l,d], in X's buffer
+      E.contract(1.0, gam("hp", "Gkc"), gam("hp", "Gld"), 0.0,
+                 dense_view(w.X.buf.get(), f, "klcd", {o, o, v, v}));
+      Vlkcd = dense_view(w.X.buf.get(), f, "lkcd", {o, o, v, v});
+    }
+    const View Q = dense_view(w.Y.buf.get(), f, "aldi", {v, o, v, o});
+    E.contract(1.0, Vlkcd, T2.as("acik"), 0.0, Q);
+    E.contract(-2.0, Q, T2.as("dblj"), 1.0, R2);
   }
 }
 

@@ -20,9 +20,14 @@ struct CcsdShape {
   int64_t nset() const { return n1() + n2(); }  // elements of one [T1;T2] set
 };
 
+enum class Method { Ccsd, Drccd };
+
 struct CcsdOptions {
-  bool ppl = true;                 // Ccsd.cxx:95,402
+  Method method = Method::Ccsd;      // CoupledCluster.cxx:77-84
+  bool ppl = true;                   // Ccsd.cxx:95,402
   int64_t integrals_slice_size = 0;  // 0: No (Ccsd.cxx:287-289)
+  bool linearized = false;           // Drccd.cxx:53-55
+  bool adjacent_pairs_exchange = false;  // Drccd.cxx:56-58
 };
 
 // time per residuum stage, filled when profiling is on
@@ -56,6 +61,7 @@ class Ccsd {
  private:
   void residuum_complex(const void *T, void *R);
   void residuum_real(const void *T, void *R);
+  void residuum_drccd(const void *T, void *R);
   void ladder(const View &left, const View &right, const View &tauv,
               const View &R2);
   View gam(const char *part, const std::string &names, bool conjT = false) const;

@@ -8,7 +8,8 @@
 //   DefineHolesAndParticles DefineHolesAndParticles.cxx:38-96
 //   SliceOperator           SliceOperator.cxx:50-131
 //   VertexCoulombIntegrals  VertexCoulombIntegrals.cxx:36-52 (the field choice)
-//   CoupledCluster          CoupledCluster.cxx:55-192, method Ccsd, DiisMixer,
+//   CoupledCluster          CoupledCluster.cxx:55-192, methods Ccsd and Drccd,
+//                           DiisMixer,
 //                           through the C ABI of include/cc4s_ccsd.h
 //   Write                   Write.cxx:27-42 (Writer.hpp)
 // Any other algorithm stops the run with "unknown algorithm: <name>", as the
@@ -308,7 +309,8 @@ class Runner {
     if (ints->kind != Object::Kind::Integrals) throw std::runtime_error("expecting Coulomb integrals as 'coulombIntegrals'");
     if (vtx->kind != Object::Kind::SlicedVertex) throw std::runtime_error("expecting a sliced Coulomb vertex");
     const std::string method = yaml::get_text(args, "method", "Ccsd");
-    if (method != "Ccsd") throw std::runtime_error("Unknown method: '" + method + "'");
+    if (method != "Ccsd" && method != "Drccd")
+      throw std::runtime_error("Unknown method: '" + method + "'");
     auto mixer = args->get("mixer");
     const std::string mixer_type = yaml::get_text(mixer, "type", "DiisMixer");
     if (mixer_type != "DiisMixer") throw std::runtime_error("Unknown mixer type: '" + mixer_type + "'");
@@ -336,14 +338,22 @@ class Runner {
       throw std::runtime_error("unknown precision '" + precision + "' (fp64 | mixed)");
     p.mixed_switch = yaml::get_real(args, "mixedSwitch", p.mixed_switch);
     p.device = static_cast<int>(yaml::get_integer(args, "device", 0));
+    p.method = method == "Drccd" ? CC4S_METHOD_DRCCD : CC4S_METHOD_CCSD;
+    p.linearized = yaml::get_boolean(args, "linearized", false) ? 1 : 0;
+    p.adjacent_pairs_exchange = yaml::get_boolean(args, "adjacentPairsExchange", false) ? 1 : 0;
     // the report lists the defaults used, as the reference's argument map does
     in_report->set("method", method);
     in_report->set("maxIterations", p.max_iterations);
     in_report->set("energyConvergence", p.energy_convergence);
     in_report->set("amplitudesConvergence", p.amplitudes_convergence);
     in_report->set("levelShift", p.level_shift);
-    in_report->set("ppl", static_cast<int64_t>(p.ppl));
-    in_report->set("integralsSliceSize", p.integrals_slice_size);
+    if (p.method == CC4S_METHOD_DRCCD) {
+      in_report->set("linearized", static_cast<int64_t>(p.linearized));
+      in_report->set("adjacentPairsExchange", static_cast<int64_t>(p.adjacent_pairs_exchange));
+    } else {
+      in_report->set("ppl", static_cast<int64_t>(p.ppl));
+      in_report->set("integralsSliceSize", p.integrals_slice_size);
+    }
 
     ObjectPtr initial = symbol(args, "initialAmplitudes", false);
     if (args->has("initialAmplitudes") && !initial)
@@ -402,8 +412,8 @@ class Runner {
       check(cc4s_ccsd_set_amplitudes(ctx, t1.data.data(), t2.data.data()));
       std::printf("Using given initial amplitudes\n");
     }
-    std::printf("Using method Ccsd. integralsSliceSize: %lld\nUsing mixer DiisMixer. maxResidua: %lld\n",
-                static_cast<long long>(p.integrals_slice_size), static_cast<long long>(p.max_residua));
+    std::printf("Using method %s.\nUsing mixer DiisMixer. maxResidua: %lld\n", method.c_str(),
+                static_cast<long long>(p.max_residua));
     std::printf("Maximum number of iterations: %lld\nUnless reaching energy convergence dE: %g\n"
                 "and amplitudes convergence dR: %g\n",
                 static_cast<long long>(p.max_iterations), p.energy_convergence,
@@ -421,7 +431,7 @@ class Runner {
                   it.seconds > 0 ? it.flops / it.seconds / 1e12 : 0.0,
                   it.precision_used == CC4S_PRECISION_MIXED ? "  (mixed)" : "");
     }
-    std::printf("\nCcsd correlation energy:          %.12f\n", r.correlation);
+    std::printf("\n%s correlation energy:          %.12f\n", method.c_str(), r.correlation);
     std::printf("2nd-order correlation energy:     %.12f\n", r.second_order);
     if (!r.convergence_reached) std::printf("A converged solution was not found\n");
     std::fflush(stdout);

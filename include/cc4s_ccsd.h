@@ -55,6 +55,10 @@ enum { CC4S_FIELD_REAL = 0, CC4S_FIELD_COMPLEX = 1 };
  * FP64.  FP32 is accepted by the energy gradient only. */
 enum { CC4S_PRECISION_FP64 = 0, CC4S_PRECISION_MIXED = 1, CC4S_PRECISION_FP32 = 2 };
 
+/* Coupled-cluster method (CoupledCluster.cxx:77-84, registered names "Ccsd"
+ * Ccsd.cxx:25-32 and "Drccd" Drccd.cxx:25-28). */
+enum { CC4S_METHOD_CCSD = 0, CC4S_METHOD_DRCCD = 1 };
+
 /* Step arguments of CoupledCluster (CoupledCluster.cxx:59-117,260-262,
  * CoupledCluster.hpp:45-49, Ccsd.cxx:95,287-289, DiisMixer.cxx:33). */
 typedef struct {
@@ -70,6 +74,9 @@ typedef struct {
   int precision;                  /* CC4S_PRECISION_* */
   double mixed_switch;            /* MIXED: dR below which FP64 takes over (1e-5) */
   int device;                     /* HIP device ordinal */
+  int method;                     /* CC4S_METHOD_* (Ccsd) */
+  int linearized;                 /* Drccd: linearized (0), Drccd.cxx:53-55 */
+  int adjacent_pairs_exchange;    /* Drccd: adjacentPairsExchange (0), Drccd.cxx:56-58 */
 } cc4s_ccsd_params;
 
 /* One pass of the iteration loop (CoupledCluster.cxx:129-175). */

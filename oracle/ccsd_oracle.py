@@ -14,6 +14,7 @@ It restates, in numpy (einsum -> BLAS), the reference's algorithm:
 * ``energy``                CoupledCluster.cxx:194-253
 * ``DiisMixer``             mixers/DiisMixer.cxx:27-148 (+ TensorSet.hpp:209-224)
 * ``coupled_cluster``       CoupledCluster.cxx:55-192
+* ``residuum_drccd``        coupledcluster/Drccd.cxx:31-109
 * ``vertex_energy`` / ``vertex_gradient``  the energy of CoupledCluster.cxx:
   194-253 as a function of Gamma (V from VertexCoulombIntegrals.cxx:68-171)
   at fixed amplitudes, its gradient and per-G split (the structure factor of
@@ -114,7 +115,13 @@ class CcsdProblem:
     """Everything the Ccsd method reads from its step arguments."""
 
     def __init__(self, gamma, eps, No, field, ppl=True,
-                 integrals_slice_size=None):
+                 integrals_slice_size=None, method="Ccsd", linearized=False,
+                 adjacent_pairs_exchange=False):
+        if method not in ("Ccsd", "Drccd"):
+            raise ValueError(f"Unknown method: '{method}'")
+        self.method = method
+        self.linearized = linearized
+        self.adjacent_pairs_exchange = adjacent_pairs_exchange
         self.field = field
         self.No = No
         self.Nv = gamma.shape[1] - No
@@ -123,15 +130,45 @@ class CcsdProblem:
         self.eps_p = np.asarray(eps[No:], dtype=np.float64)
         self.slices = slice_vertex(gamma, No)
         names = COMPLEX_SLICES if field == "complex" else REAL_SLICES
+        if method == "Drccd":
+            names = ("pphh", "phhp", "hhpp")
         self.V = vertex_integrals(self.slices, field, names)
         self.ppl = ppl
         self.slice_size = integrals_slice_size or No   # Ccsd.cxx:287-289
         self.dtype = np.complex128 if field == "complex" else np.float64
 
     def residuum(self, amplitudes):
+        if self.method == "Drccd":
+            return residuum_drccd(self, amplitudes)
         if self.field == "complex":
             return residuum_complex(self, amplitudes)
         return residuum_real(self, amplitudes)
+
+
+# --------------------------------------------------------------------------
+# Drccd::getResiduum (both fields: the same statements, V from either rule)
+# --------------------------------------------------------------------------
+
+def residuum_drccd(P, amplitudes):
+    """Drccd.cxx:31-109: direct-ring CCD.  R1 = 0 always (:61,76);
+    first iteration R2 = V_abij (:62); then R2 = 2 V_akic T_cbkj (:77),
+    R2_abij += R2_baji (:78), R2 += V_abij (:79) and, unless linearized,
+    W_klcd = 2 V_klcd (- V_lkcd with adjacentPairsExchange),
+    R2 += 2 W_klcd T_acik T_dblj (:84-99)."""
+    V = P.V
+    R1 = np.zeros((P.Nv, P.No), dtype=P.dtype)
+    if amplitudes is None:
+        return R1, V["pphh"].copy()
+    T2 = amplitudes[1]
+    R2 = 2.0 * ein("akic,cbkj->abij", V["phhp"], T2)
+    R2 = R2 + R2.transpose(1, 0, 3, 2)
+    R2 += V["pphh"]
+    if not P.linearized:
+        W = 2.0 * V["hhpp"]
+        if P.adjacent_pairs_exchange:
+            W = W - V["hhpp"].transpose(1, 0, 2, 3)
+        R2 += 2.0 * ein("klcd,acik,dblj->abij", W, T2, T2)
+    return R1, R2
 
 
 # --------------------------------------------------------------------------

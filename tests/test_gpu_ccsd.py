@@ -219,3 +219,53 @@ def test_config1_properties():
     Tm = np.conj(V) / Dpphh
     mp2 = (-2.0 * np.sum(V * Tm) + np.sum(V.transpose(0, 1, 3, 2) * Tm)).real
     assert abs(it["energy"] - mp2) < 1e-13 * abs(mp2)
+
+
+@pytest.mark.parametrize("field", ["complex", "real"])
+@pytest.mark.parametrize("opts", [dict(), dict(linearized=True),
+                                  dict(adjacentPairsExchange=True)],
+                         ids=["quadratic", "linearized", "adjacentPairsExchange"])
+def test_drccd_parity(field, opts):
+    """Drccd (Drccd.cxx:31-109): one residuum against the oracle's explicit-V
+    statements, then a run to convergence."""
+    No, Nv, NG = 4, 12, 30
+    print(f"drccd {field} {opts}", flush=True)
+    inp = syn.generate(No, Nv, NG, field, 5)
+    kw = dict(maxIterations=40, energyConvergence=1e-10, amplitudesConvergence=1e-8)
+    P = O.CcsdProblem(inp.gamma, inp.eps, No, field, method="Drccd",
+                      linearized=opts.get("linearized", False),
+                      adjacent_pairs_exchange=opts.get("adjacentPairsExchange", False))
+    with api.CoupledCluster(No, Nv, NG, field=field, method="Drccd", **opts, **kw) as cc:
+        cc.set_inputs(inp.gamma, inp.eps)
+        T = _amps(No, Nv, field, 13)
+        T = (np.zeros_like(T[0]), T[1])
+        R1, R2 = cc.residuum(T)
+        Q1, Q2 = P.residuum(T)
+        assert np.all(R1 == 0)
+        assert rel_err(R2, Q2) < R_TOL
+        gpu = cc.run()
+        T_gpu = cc.amplitudes()
+    ref = O.coupled_cluster(P, max_iterations=40, energy_convergence=1e-10,
+                            amplitudes_convergence=1e-8)
+    _check_run(gpu, T_gpu, ref)
+
+
+def test_ueg_energies_of_the_reference_test_suite():
+    """Uniform electron gas rs=1, No=7, Nv=26 (inputs restated from
+    UegVertexGenerator.cxx): the GPU Drccd and Ccsd energies equal those the
+    reference recorded in test/tests/ueg/rs1.0-7occ-26virt/correct.out.yaml,
+    within 1e-9 Eh (the reference's own check uses 1e-7)."""
+    from oracle import ueg
+    gamma, eps, _ = ueg.generate(7, 26, 1.0)
+    ref = ueg.UEG_RS1_NO7_NV26
+    s = ref["settings"]
+    kw = dict(maxIterations=s["maxIterations"], energyConvergence=s["energyConvergence"],
+              amplitudesConvergence=s["amplitudesConvergence"], maxResidua=s["maxResidua"])
+    for method, extra in (("Drccd", {}), ("Ccsd", dict(integralsSliceSize=s["integralsSliceSize"]))):
+        with api.CoupledCluster(7, 26, gamma.shape[0], method=method, **kw, **extra) as cc:
+            cc.set_inputs(gamma, eps)
+            out = cc.run()
+        print(method, out["energy"], out["iterations"], flush=True)
+        assert out["convergenceReached"]
+        for key in ("correlation", "direct", "exchange", "secondOrder"):
+            assert abs(out["energy"][key] - ref[method][key]) < 1e-9, (method, key)

@@ -148,3 +148,48 @@ def test_vertex_with_more_states_than_the_energies(tmp_path):
     ref = O.coupled_cluster(P, max_iterations=30, energy_convergence=1e-10,
                             amplitudes_convergence=1e-8, max_residua=4)
     assert abs(ccsd_out(rep)["energy"]["correlation"] - ref["energy"]["correlation"]) < 1e-9
+
+
+def test_reference_ueg_operation_file(tmp_path):
+    """The reference's own operation file of test/tests/ueg/rs1.0-7occ-26virt
+    (Drccd, then Ccsd; its last step, (T), is outside this drop-in) on the
+    uniform-electron-gas inputs restated from UegVertexGenerator.cxx: both
+    CoupledCluster steps report the energies of its correct.out.yaml within
+    1e-9 Eh (the reference's check uses 1e-7)."""
+    from oracle import ueg
+    gamma, eps, fermi = ueg.generate(7, 26, 1.0)
+    inp = syn.CcsdInputs(gamma=gamma, eps=eps, No=7, field="complex", scale=1.0,
+                         fermi_energy=fermi)
+    syn.write_cc4s_inputs(inp, tmp_path)
+    s = ueg.UEG_RS1_NO7_NV26["settings"]
+    common = f"""    slicedEigenEnergies: EigenEnergies
+    coulombIntegrals: CoulombIntegrals
+    slicedCoulombVertex: CoulombVertex
+    maxIterations: {s['maxIterations']}
+    energyConvergence: 1.0E-8
+    amplitudesConvergence: 1.0E-8
+    mixer:
+      type: DiisMixer
+      maxResidua: {s['maxResidua']}
+"""
+    ops = STEPS + f"""
+- name: CoupledCluster
+  in:
+    method: Drccd
+{common}  out:
+    energy: CcsdEnergy
+
+- name: CoupledCluster
+  in:
+    method: Ccsd
+    integralsSliceSize: 100
+{common}  out:
+    energy: CcsdEnergy
+    amplitudes: Amplitudes
+"""
+    rep = run(tmp_path, ops)
+    for step, method in ((5, "Drccd"), (6, "Ccsd")):
+        out = rep["steps"][step]["out"]
+        assert out["convergenceReached"] == 1
+        for key in ("correlation", "direct", "exchange", "secondOrder"):
+            assert abs(out["energy"][key] - ueg.UEG_RS1_NO7_NV26[method][key]) < 1e-9, (method, key)

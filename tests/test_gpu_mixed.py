@@ -88,9 +88,13 @@ def test_mixed_run_reaches_fp64_energy(shape, switch):
     assert fp64["convergenceReached"] and mixed["convergenceReached"]
     assert abs(mixed["energy"]["correlation"] - fp64["energy"]["correlation"]) < 1e-8
     used = [t["precision_used"] for t in mixed["trace"]]
+    for t in mixed["trace"]:
+        print(f"{t['iteration']:3d} {'mixed' if t['precision_used'] else 'fp64 '} "
+              f"E={t['energy']:.13f} dR={t['residuum_norm']:.3e}", flush=True)
     assert used[0] == api.PRECISION_MIXED and used[-1] == api.PRECISION_FP64
     # once switched, FP64 to the end; only FP64 passes declare convergence
     first64 = used.index(api.PRECISION_FP64)
     assert all(u == api.PRECISION_FP64 for u in used[first64:])
     assert not any(t["converged"] for t in mixed["trace"][:first64])
-    assert mixed["iterations"] <= fp64["iterations"] + 6
+    # the stall guard costs the passes spent at the rounding floor
+    assert mixed["iterations"] <= fp64["iterations"] + (10 if switch < 1e-12 else 4)

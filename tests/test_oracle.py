@@ -190,3 +190,36 @@ def test_golden_config0_generator_bytes():
     he = hashlib.sha256(inp.eps.tobytes()).hexdigest()
     assert h == gold["gamma_sha256"]
     assert he == gold["eps_sha256"]
+
+
+def test_ueg_known_answer_from_the_reference_test_suite():
+    """The reference's own recorded energies: restating UegVertexGenerator.cxx
+    for rs=1, No=7, Nv=26 reproduces the inputs of
+    test/tests/ueg/rs1.0-7occ-26virt, and the oracle's MP2 and CCSD energies
+    equal correct.out.yaml:125-157 (the reference's test tolerance is 1e-7)."""
+    from oracle import ueg
+    gamma, eps, fermi = ueg.generate(7, 26, 1.0)
+    assert gamma.shape == (257, 33, 33)
+    No = O.define_holes_and_particles(eps, fermi)[0]
+    assert No == 7
+    ref = ueg.UEG_RS1_NO7_NV26
+    s = ref["settings"]
+    P = O.CcsdProblem(gamma, eps, No, "complex",
+                      integrals_slice_size=s["integralsSliceSize"])
+    assert abs(O.mp2_energy(P) - ref["Ccsd"]["secondOrder"]) < 1e-12
+    r = O.coupled_cluster(P, max_iterations=s["maxIterations"],
+                          energy_convergence=s["energyConvergence"],
+                          amplitudes_convergence=s["amplitudesConvergence"],
+                          max_residua=s["maxResidua"])
+    assert r["convergenceReached"]
+    for key in ("correlation", "direct", "exchange", "secondOrder"):
+        assert abs(r["energy"][key] - ref["Ccsd"][key]) < 1e-10, key
+    # Drccd, the step before Ccsd in the same operation file
+    P = O.CcsdProblem(gamma, eps, No, "complex", method="Drccd")
+    r = O.coupled_cluster(P, max_iterations=s["maxIterations"],
+                          energy_convergence=s["energyConvergence"],
+                          amplitudes_convergence=s["amplitudesConvergence"],
+                          max_residua=s["maxResidua"])
+    assert r["convergenceReached"]
+    for key in ("correlation", "direct", "exchange", "secondOrder"):
+        assert abs(r["energy"][key] - ref["Drccd"][key]) < 1e-10, key
