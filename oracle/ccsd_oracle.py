@@ -15,6 +15,7 @@ It restates, in numpy (einsum -> BLAS), the reference's algorithm:
 * ``DiisMixer``             mixers/DiisMixer.cxx:27-148 (+ TensorSet.hpp:209-224)
 * ``coupled_cluster``       CoupledCluster.cxx:55-192
 * ``residuum_drccd``        coupledcluster/Drccd.cxx:31-109
+* ``perturbative_triples``  PerturbativeTriplesReference.cxx:55-180
 * ``vertex_energy`` / ``vertex_gradient``  the energy of CoupledCluster.cxx:
   194-253 as a function of Gamma (V from VertexCoulombIntegrals.cxx:68-171)
   at fixed amplitudes, its gradient and per-G split (the structure factor of
@@ -592,3 +593,40 @@ def vertex_gradient(gamma, amplitudes, No, field):
         g[:, No:, :No] = ein("abij,Gbj->Gai", Z, ph)
     S = 0.5 * np.real(np.sum(np.conj(gamma) * g, axis=(1, 2)))
     return g, S
+
+
+# ---------------------------------------------------------------------------
+# PerturbativeTriplesReference (the (T) energy of converged amplitudes)
+# ---------------------------------------------------------------------------
+
+def perturbative_triples(gamma, eps, No, field, amplitudes):
+    """(T) correlation energy as PerturbativeTriplesReference.cxx:55-180
+    forms it (cT off):
+      T'_abcijk = V_bcdk T_adij - conj(V_jklc) T_abil             (:106-107)
+      W = T' + T'_bacjik + T'_acbikj + T'_cbakji + T'_cabkij + T'_bcajki
+      S = W + T1_ai V_bcjk + T1_bj V_acik + T1_ck V_abij          (:118-121)
+      Z = 4/3 S_abcijk - 2 S_acbijk + 2/3 S_bcaijk                 (:123-125)
+      D = e_i + e_j + e_k - e_a - e_b - e_c                        (:128-134)
+      E = sum conj(W)/D * conj(Z)/D * D = sum conj(W Z) / D       (:136-140,163)
+    with V_ppph, V_hhhp, V_pphh from VertexCoulombIntegrals.  Returns Re E.
+    Memory O(o^3 v^3): small cases only."""
+    slices = slice_vertex(gamma, No)
+    V = vertex_integrals(slices, field, ("pphh", "ppph", "hhhp"))
+    T1, T2 = amplitudes
+    Tp = (ein("bcdk,adij->abcijk", V["ppph"], T2)
+          - ein("jklc,abil->abcijk", np.conj(V["hhhp"]), T2))
+    W = (Tp + Tp.transpose(1, 0, 2, 4, 3, 5) + Tp.transpose(0, 2, 1, 3, 5, 4)
+         + Tp.transpose(2, 1, 0, 5, 4, 3) + Tp.transpose(1, 2, 0, 4, 5, 3)
+         + Tp.transpose(2, 0, 1, 5, 3, 4))
+    R = (ein("ai,bcjk->abcijk", T1, V["pphh"])
+         + ein("bj,acik->abcijk", T1, V["pphh"])
+         + ein("ck,abij->abcijk", T1, V["pphh"]))
+    S = W + R
+    Z = (4.0 / 3.0) * S - 2.0 * S.transpose(0, 2, 1, 3, 4, 5) \
+        + (2.0 / 3.0) * S.transpose(2, 0, 1, 3, 4, 5)
+    eh = np.asarray(eps[:No], dtype=np.float64)
+    ep = np.asarray(eps[No:], dtype=np.float64)
+    D = (eh[None, None, None, :, None, None] + eh[None, None, None, None, :, None]
+         + eh[None, None, None, None, None, :] - ep[:, None, None, None, None, None]
+         - ep[None, :, None, None, None, None] - ep[None, None, :, None, None, None])
+    return float(np.real(np.sum(np.conj(W * Z) / D)))

@@ -107,12 +107,14 @@ def generate(No: int, Nv: int, rs: float, hartree_fock: bool = True):
 
 
 # recorded by the reference for rs = 1, No = 7, Nv = 26
-# (test/tests/ueg/rs1.0-7occ-26virt/correct.out.yaml:120-157)
+# (test/tests/ueg/rs1.0-7occ-26virt/correct.out.yaml:120-170)
 UEG_RS1_NO7_NV26 = {
     "Drccd": {"correlation": -0.2758298986953282, "direct": -0.4128084206966558,
               "exchange": 0.1369785220013276, "secondOrder": -0.36143028565969504},
     "Ccsd": {"correlation": -0.39269658954585018, "direct": -0.5953888037963122,
              "exchange": 0.20269221425046202, "secondOrder": -0.36143028565969504},
+    # PerturbativeTriplesReference on the Ccsd amplitudes (:166-170)
+    "Triples": {"correlation": -0.0063019625641725016},
     "settings": {"maxIterations": 20, "energyConvergence": 1e-8,
                  "amplitudesConvergence": 1e-8, "maxResidua": 4,
                  "integralsSliceSize": 100},

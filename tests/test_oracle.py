@@ -214,6 +214,9 @@ def test_ueg_known_answer_from_the_reference_test_suite():
     assert r["convergenceReached"]
     for key in ("correlation", "direct", "exchange", "secondOrder"):
         assert abs(r["energy"][key] - ref["Ccsd"][key]) < 1e-10, key
+    # (T) of the converged CCSD amplitudes (PerturbativeTriplesReference)
+    e_t = O.perturbative_triples(gamma, eps, No, "complex", r["amplitudes"])
+    assert abs(e_t - ref["Triples"]["correlation"]) < 1e-12
     # Drccd, the step before Ccsd in the same operation file
     P = O.CcsdProblem(gamma, eps, No, "complex", method="Drccd")
     r = O.coupled_cluster(P, max_iterations=s["maxIterations"],
