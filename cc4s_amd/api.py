@@ -30,6 +30,7 @@ EXPORTS = (
     "cc4s_ccsd_iterate", "cc4s_ccsd_run", "cc4s_ccsd_get_amplitudes",
     "cc4s_ccsd_residuum", "cc4s_ccsd_energy",
     "cc4s_ccsd_energy_grad_forward", "cc4s_ccsd_energy_grad_backward",
+    "cc4s_ccsd_triples",
     "cc4s_ccsd_set_profiling",
     "cc4s_ccsd_ladder_stats", "cc4s_ccsd_kernel_stats",
     "cc4s_ccsd_last_error", "cc4s_ccsd_destroy",
@@ -112,6 +113,7 @@ def load_library(path: pathlib.Path | str | None = None):
     lib.cc4s_ccsd_residuum.argtypes = [ctxp, vp, vp, vp, vp]
     lib.cc4s_ccsd_energy.argtypes = [ctxp, vp, vp, dp]
     lib.cc4s_ccsd_energy_grad_forward.argtypes = [ctxp, vp, C.c_int, dp]
+    lib.cc4s_ccsd_triples.argtypes = [ctxp, dp]
     lib.cc4s_ccsd_energy_grad_backward.argtypes = [ctxp, vp, C.c_int, vp, C.c_int,
                                                    vp, C.c_int]
     lib.cc4s_ccsd_set_profiling.argtypes = [ctxp, C.c_int]
@@ -285,6 +287,12 @@ class CoupledCluster:
         if g.shape != (self.NG, Np, Np):
             raise ValueError(f"gamma shape {g.shape} != {(self.NG, Np, Np)}")
         return _ptr(g), 0, g
+
+    def triples(self):
+        """(T) correlation energy of the current amplitudes."""
+        e = C.c_double()
+        self._check(self.lib.cc4s_ccsd_triples(self._ctx, C.byref(e)))
+        return e.value
 
     def energy_forward(self, gamma=None):
         """Correlation energy at the current amplitudes with the vertex

@@ -20,7 +20,7 @@ LIB = HERE / "libcc4s_ccsd.so"
 BENCH = HERE / "gett_bench"
 DRIVER = HERE / "cc4s_ccsd"
 DRIVER_SOURCES = ["main.cpp", "yaml.cpp", "tensor_io.cpp"]
-SOURCES = ["contract.hip", "mixed.hip", "elementwise.hip", "adjoint.hip", "ccsd.hip",
+SOURCES = ["contract.hip", "mixed.hip", "elementwise.hip", "adjoint.hip", "triples.hip", "ccsd.hip",
            "coupled_cluster.hip", "abi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -189,6 +189,14 @@ int cc4s_ccsd_energy(cc4s_ccsd_context *ctx, const double *T1, const double *T2,
   });
 }
 
+int cc4s_ccsd_triples(cc4s_ccsd_context *ctx, double *energy) {
+  if (!ctx) return 1;
+  return guarded(ctx, [&] {
+    CC4S_ASSERT(energy, "null output");
+    *energy = ctx->cc->triples();
+  });
+}
+
 int cc4s_ccsd_energy_grad_forward(cc4s_ccsd_context *ctx, const double *gamma,
                                   int gamma_on_device, double *energy) {
   if (!ctx) return 1;

@@ -5,6 +5,7 @@
 // ladder forms V_xycd slabs from the dressed vertices and never holds the
 // full v^4 tensor.
 #include "ccsd.hpp"
+#include "triples.hpp"
 
 #include "elementwise.hpp"
 
@@ -111,26 +112,31 @@ View Ccsd::gam(const char *part, const std::string &names, bool conjT) const {
   return v.as(names);
 }
 
-void Ccsd::form_integrals() {
-  auto &w = *w_;
+// Coulomb integral slice `name` (four of h/p, LO RO LI RI) into V
+void Ccsd::form_slice(Tensor &V, const char *name) {
   const Field f = s_.field;
   const int64_t o = s_.No, v = s_.Nv;
   auto len = [&](char c) { return c == 'h' ? o : v; };
-  auto make = [&](Tensor &V, const char *name) {
-    V.reset(f, "pqsr", {len(name[0]), len(name[1]), len(name[2]), len(name[3])});
-    const std::string LI{name[0], name[2]}, RI{name[1], name[3]};
-    if (f == Field::Complex) {
-      // V[p,q,s,r] = sum_G conj(Gamma_{LI LO}[G,s,p]) Gamma_{RO RI}[G,q,r]
-      // (VertexCoulombIntegrals.cxx:143-171)
-      E_.contract(1.0, gam(LI.c_str(), "Gps", true), gam(RI.c_str(), "Gqr"), 0.0,
-                  V("pqsr"));
-    } else {
-      // V[p,q,s,r] = sum_G ReGamma_{LO LI}[G,p,s] ReGamma_{RO RI}[G,q,r] + Im Im
-      // (VertexCoulombIntegrals.cxx:87-102)
-      E_.contract(1.0, gam(LI.c_str(), "Gps"), gam(RI.c_str(), "Gqr"), 0.0,
-                  V("pqsr"));
-    }
-  };
+  V.reset(f, "pqsr", {len(name[0]), len(name[1]), len(name[2]), len(name[3])});
+  const std::string LI{name[0], name[2]}, RI{name[1], name[3]};
+  if (f == Field::Complex) {
+    // V[p,q,s,r] = sum_G conj(Gamma_{LI LO}[G,s,p]) Gamma_{RO RI}[G,q,r]
+    // (VertexCoulombIntegrals.cxx:143-171)
+    E_.contract(1.0, gam(LI.c_str(), "Gps", true), gam(RI.c_str(), "Gqr"), 0.0,
+                V("pqsr"));
+  } else {
+    // V[p,q,s,r] = sum_G ReGamma_{LO LI}[G,p,s] ReGamma_{RO RI}[G,q,r] + Im Im
+    // (VertexCoulombIntegrals.cxx:87-102)
+    E_.contract(1.0, gam(LI.c_str(), "Gps"), gam(RI.c_str(), "Gqr"), 0.0,
+                V("pqsr"));
+  }
+}
+
+void Ccsd::form_integrals() {
+  auto &w = *w_;
+  const Field f = s_.field;
+  const int64_t v = s_.Nv;
+  auto make = [&](Tensor &V, const char *name) { form_slice(V, name); };
   make(w.Vpphh, "pphh");
   make(w.Vhhhh, "hhhh");
   make(w.Vhhhp, "hhhp");
@@ -485,6 +491,91 @@ void Ccsd::residuum_real(const void *Tp, void *Rp) {
   C(-1.0, gam("pp", "Gda"), w.Wgvo("Gdi"), 1.0, R1);
   C(-2.0, Vhhhp("klic"), X("ackl"), 1.0, R1);                    // :372
   C(1.0, Vhhhp("lkic"), X("ackl"), 1.0, R1);                     // :373
+}
+
+}  // namespace cc4s_amd
+
+namespace cc4s_amd {
+
+namespace {
+
+// `v` with index `c` fixed at `value` (the dimension dropped)
+View fixed(const View &v, char c, int64_t value) {
+  const int d = v.find(c);
+  CC4S_ASSERT(d >= 0 && value >= 0 && value < v.lens[d], "fixed: bad index");
+  View r = v;
+  r.ptr = static_cast<char *>(v.ptr) + value * v.strides[d] * static_cast<int64_t>(scalar_bytes(v.field));
+  r.idx.erase(d, 1);
+  r.lens.erase(r.lens.begin() + d);
+  r.strides.erase(r.strides.begin() + d);
+  return r;
+}
+
+}  // namespace
+
+// (T): for every hole pair (i, j) and chunk of k, the connected amplitudes
+// W_abcijk = sum over the six pair permutations of
+//   T'_{x1x2x3,y1y2y3} = V_{x2 x3 d y3} T_{x1 d y1 y2} - conj(V_{y2 y3 l x3}) T_{x1 x2 y1 l}
+// (PerturbativeTriplesReference.cxx:106-116) are accumulated by twelve
+// contractions into W[a,b,c,k], each reading its operands in place through
+// index-permuted views (no transposes); the energy kernel then adds the
+// disconnected part and the denominator (:118-163).  Per pair the particle
+// contractions cost 6 No Nv^4 MACs, the hole ones 6 No^2 Nv^3.
+double Ccsd::triples(const void *Tp, const double *eps_h, const double *eps_p) {
+  auto &w = *w_;
+  auto &E = E_;
+  const Field f = s_.field;
+  const bool cplx = f == Field::Complex;
+  const int64_t o = s_.No, v = s_.Nv;
+  const size_t es = scalar_bytes(f);
+  Tensor Vppph;
+  form_slice(Vppph, "ppph");                                     // V[b,c,d,k]
+  const View T1 = dense_view(const_cast<void *>(Tp), f, "ai", {v, o});
+  const View T2 = dense_view(offset_ptr(Tp, s_.n1(), f), f, "abij", {v, v, o, o});
+  auto t2 = [&](const char *names) { return T2.as(names); };
+  auto vp = [&](const char *names) { return Vppph.v.as(names); };
+  const View Vh0 = cplx ? w.Vhhhp.v.conjugated() : w.Vhhhp.v;   // conj V_hhhp
+  auto vh = [&](const char *names) { return Vh0.as(names); };
+
+  const size_t budget = size_t(4) << 30;
+  const int64_t per_k = v * v * v * static_cast<int64_t>(es);
+  const int64_t kc = std::max<int64_t>(1, std::min<int64_t>(o, static_cast<int64_t>(budget) / per_k));
+  Tensor Wt(f, "abck", {v, v, v, kc});
+  DeviceBuffer acc(kTriplesBlocks * sizeof(double));
+  CC4S_HIP(hipMemsetAsync(acc.get(), 0, kTriplesBlocks * sizeof(double), E.stream()));
+  for (int64_t i = 0; i < o; ++i) {
+    for (int64_t j = 0; j < o; ++j) {
+      for (int64_t k0 = 0; k0 < o; k0 += kc) {
+        const int64_t kn = std::min(kc, o - k0);
+        const View W = Wt.v.sliced(3, 0, kn);
+        auto K = [&](const View &x) { return x.sliced(x.find('k'), k0, k0 + kn); };
+        // particle terms, T'_{abc,ijk}, T'_{bac,jik}, T'_{acb,ikj}, T'_{cba,kji},
+        // T'_{cab,kij}, T'_{bca,jki}
+        E.contract(1.0, fixed(fixed(t2("adij"), 'i', i), 'j', j), K(vp("bcdk")), 0.0, W);
+        E.contract(1.0, fixed(fixed(t2("bdji"), 'j', j), 'i', i), K(vp("acdk")), 1.0, W);
+        E.contract(1.0, K(fixed(t2("adik"), 'i', i)), fixed(vp("cbdj"), 'j', j), 1.0, W);
+        E.contract(1.0, K(fixed(t2("cdkj"), 'j', j)), fixed(vp("badi"), 'i', i), 1.0, W);
+        E.contract(1.0, K(fixed(t2("cdki"), 'i', i)), fixed(vp("abdj"), 'j', j), 1.0, W);
+        E.contract(1.0, K(fixed(t2("bdjk"), 'j', j)), fixed(vp("cadi"), 'i', i), 1.0, W);
+        // hole terms in the same order
+        E.contract(-1.0, fixed(t2("abil"), 'i', i), K(fixed(vh("jklc"), 'j', j)), 1.0, W);
+        E.contract(-1.0, fixed(t2("bajl"), 'j', j), K(fixed(vh("iklc"), 'i', i)), 1.0, W);
+        E.contract(-1.0, fixed(t2("acil"), 'i', i), K(fixed(vh("kjlb"), 'j', j)), 1.0, W);
+        E.contract(-1.0, K(t2("cbkl")), fixed(fixed(vh("jila"), 'j', j), 'i', i), 1.0, W);
+        E.contract(-1.0, K(t2("cakl")), fixed(fixed(vh("ijlb"), 'i', i), 'j', j), 1.0, W);
+        E.contract(-1.0, fixed(t2("bcjl"), 'j', j), K(fixed(vh("kila"), 'i', i)), 1.0, W);
+        triples_energy(E.stream(), f, W.ptr, w.Vpphh.v.ptr, T1.ptr, eps_h, eps_p, o, v, i, j,
+                       k0, kn, acc.as<double>());
+      }
+    }
+  }
+  std::vector<double> h(kTriplesBlocks);
+  CC4S_HIP(hipMemcpyAsync(h.data(), acc.get(), kTriplesBlocks * sizeof(double),
+                          hipMemcpyDeviceToHost, E.stream()));
+  CC4S_HIP(hipStreamSynchronize(E.stream()));
+  double e = 0.0;
+  for (double x : h) e += x;
+  return e;
 }
 
 }  // namespace cc4s_amd

@@ -53,6 +53,11 @@ class Ccsd {
   void get_residuum(const void *T, void *R);
 
   const View &Vpphh() const;
+
+  // (T) correlation energy of the amplitudes T (PerturbativeTriplesReference.
+  // cxx:55-180, cT off); eps_h, eps_p on the device.  V_ppph (No*Nv^3) is
+  // formed for the call and released afterwards.
+  double triples(const void *T, const double *eps_h, const double *eps_p);
   // Gamma^G_qp == conj(Gamma^G_pq) bit for bit (checked on the device)
   bool vertex_is_hermitian() const;
   CcsdTimings timings;
@@ -62,6 +67,7 @@ class Ccsd {
   void residuum_complex(const void *T, void *R);
   void residuum_real(const void *T, void *R);
   void residuum_drccd(const void *T, void *R);
+  void form_slice(Tensor &V, const char *name);
   void ladder(const View &left, const View &right, const View &tauv,
               const View &R2);
   View gam(const char *part, const std::string &names, bool conjT = false) const;

@@ -380,6 +380,14 @@ void CoupledCluster::energy_of(const double *T1, const double *T2, cplx *E,
 }
 
 
+double CoupledCluster::triples() {
+  CC4S_ASSERT(inputs_set_, "inputs not set");
+  CC4S_ASSERT(have_amplitudes_, "no amplitudes yet");
+  CC4S_HIP(hipSetDevice(device_));
+  return ccsd_->triples(T_.get(), static_cast<const double *>(eps_h_.get()),
+                        static_cast<const double *>(eps_p_.get()));
+}
+
 // --- energy adjoint ----------------------------------------------------------
 
 const void *CoupledCluster::adjoint_vertex(const double *gamma, bool on_device) {

@@ -96,6 +96,9 @@ class CoupledCluster {
   // energy_forward: Re E of CoupledCluster.cxx:194-253 with the pphh
   // integrals formed from `gamma` (VertexCoulombIntegrals.cxx:68-171).
   double energy_forward(const double *gamma, bool gamma_on_device);
+  // (T) correlation energy of the current amplitudes
+  // (PerturbativeTriplesReference.cxx:55-180)
+  double triples();
   // grad = dE/dRe Gamma + i dE/dIm Gamma ([NG, Np, Np] complex, host or
   // device memory, may be null) and S[G] = 1/2 sum_pq Re(conj Gamma^G_pq
   // grad^G_pq) (host, NG values, may be null); fp32: gradient products in

@@ -12,6 +12,7 @@
 //                           DiisMixer,
 //                           through the C ABI of include/cc4s_ccsd.h
 //   Write                   Write.cxx:27-42 (Writer.hpp)
+//   PerturbativeTriples(Reference)  PerturbativeTriplesReference.cxx:55-180
 // Any other algorithm stops the run with "unknown algorithm: <name>", as the
 // reference does for names it does not know.  Step arguments that are unquoted
 // names of stored objects are replaced by the objects (Cc4s.cxx:177-195), the
@@ -167,6 +168,8 @@ class Runner {
       flops = coupled_cluster(args, in_report, results, out_report);
     } else if (name == "Write") {
       write(args, out_report);
+    } else if (name == "PerturbativeTriplesReference" || name == "PerturbativeTriples") {
+      triples(args, results);
     } else {
       throw std::runtime_error("unknown algorithm: " + name);
     }
@@ -301,6 +304,86 @@ class Runner {
     results["coulombIntegrals"] = o;
   }
 
+  // Gamma [NG, No+Nv, No+Nv] -- the vertex itself, or its hole and particle
+  // states when it has more states (SliceOperator.cxx:112-118) -- and
+  // eps = [eps_h; eps_p]
+  static const double *step_inputs(const ObjectPtr &vtx, const ObjectPtr &eps,
+                                    std::vector<double> &compact, std::vector<double> &e) {
+    const HostTensor &V = *vtx->vertex;
+    const int64_t No = vtx->No, Nv = vtx->Nv;
+    const int64_t NG = V.lens[0], NpV = V.lens[1], Np = No + Nv;
+    const double *gamma = V.data.data();
+    if (NpV != Np) {
+      compact.resize(static_cast<size_t>(NG) * Np * Np * 2);
+      auto state = [&](int64_t q) { return q < No ? q : NpV - Nv + (q - No); };
+      for (int64_t q = 0; q < Np; ++q)
+        for (int64_t r = 0; r < Np; ++r)
+          std::memcpy(&compact[2 * NG * (r + Np * q)],
+                      &V.data[2 * NG * (state(r) + NpV * state(q))], 2 * NG * sizeof(double));
+      gamma = compact.data();
+    }
+    e.resize(Np);
+    const auto &h = eps->set->at("h").data;
+    const auto &pp = eps->set->at("p").data;
+    std::copy(h.begin(), h.end(), e.begin());
+    std::copy(pp.begin(), pp.end(), e.begin() + No);
+    return gamma;
+  }
+
+  // PerturbativeTriplesReference / PerturbativeTriples: the (T) energy of the
+  // amplitudes (PerturbativeTriplesReference.cxx:55-180; both names give the
+  // same energy, PerturbativeTriples.cxx being the distributed variant)
+  void triples(const NodePtr &args, std::map<std::string, ObjectPtr> &results) {
+    auto ints = symbol(args, "coulombIntegrals");
+    auto amps = symbol(args, "amplitudes");
+    auto eps = symbol(args, "slicedEigenEnergies");
+    if (ints->kind != Object::Kind::Integrals) throw std::runtime_error("expecting Coulomb integrals as 'coulombIntegrals'");
+    if (yaml::get_integer(args, "cT", 0) != 0) throw std::runtime_error("cT (complete triples) is not supported");
+    auto vtx = ints->sliced_vertex;
+    int64_t No, Nv;
+    sliced_counts(eps, No, Nv);
+    if (amps->kind != Object::Kind::TensorSet || !amps->set->count("ph") || !amps->set->count("pphh"))
+      throw std::runtime_error("unsupported tensor type as 'amplitudes'");
+    const bool cplx = !ints->real_field;
+    const HostTensor &t1 = amps->set->at("ph"), &t2 = amps->set->at("pphh");
+    if (t1.complex != cplx || t2.complex != cplx || t1.elements() != Nv * No ||
+        t2.elements() != Nv * Nv * No * No)
+      throw std::runtime_error("amplitudes do not match the field or the sizes");
+    std::vector<double> compact, e;
+    const double *gamma = step_inputs(vtx, eps, compact, e);
+    double et = 0.0;
+    if (!opt_.dry_run) {
+      cc4s_ccsd_params p;
+      cc4s_ccsd_default_params(&p);
+      p.field = cplx ? CC4S_FIELD_COMPLEX : CC4S_FIELD_REAL;
+      p.No = No;
+      p.Nv = Nv;
+      p.NG = vtx->vertex->lens[0];
+      p.max_residua = 1;
+      p.device = static_cast<int>(yaml::get_integer(args, "device", 0));
+      cc4s_ccsd_context *ctx = nullptr;
+      auto check = [&](int rc) {
+        if (rc != 0) {
+          std::string msg = cc4s_ccsd_last_error(ctx);
+          if (ctx) cc4s_ccsd_destroy(ctx);
+          throw std::runtime_error(msg);
+        }
+      };
+      check(cc4s_ccsd_create(&p, &ctx));
+      check(cc4s_ccsd_set_inputs(ctx, gamma, e.data()));
+      check(cc4s_ccsd_set_amplitudes(ctx, t1.data.data(), t2.data.data()));
+      check(cc4s_ccsd_triples(ctx, &et));
+      cc4s_ccsd_destroy(ctx);
+      std::printf("(T) correlation energy: %22.10f\n", et);
+    }
+    auto energy = Node::map();
+    energy->set("correlation", et);
+    energy->set("unit", eps->set->at("h").unit);
+    auto o = make_object(Object::Kind::Data);
+    o->data = energy;
+    results["energy"] = o;
+  }
+
   double coupled_cluster(const NodePtr &args, NodePtr &in_report,
                          std::map<std::string, ObjectPtr> &results, NodePtr &out_report) {
     auto ints = symbol(args, "coulombIntegrals");
@@ -359,26 +442,10 @@ class Runner {
     if (args->has("initialAmplitudes") && !initial)
       throw std::runtime_error("'initialAmplitudes' does not name a stored object");
 
-    // Gamma [NG, No+Nv, No+Nv]: the vertex itself, or its hole and particle
-    // states when it has more states (SliceOperator.cxx:112-118)
     const HostTensor &V = *vtx->vertex;
-    const int64_t NG = V.lens[0], NpV = V.lens[1], Np = No + Nv;
-    std::vector<double> compact;
-    const double *gamma = V.data.data();
-    if (NpV != Np) {
-      compact.resize(static_cast<size_t>(NG) * Np * Np * 2);
-      auto state = [&](int64_t q) { return q < No ? q : NpV - Nv + (q - No); };
-      for (int64_t q = 0; q < Np; ++q)
-        for (int64_t r = 0; r < Np; ++r)
-          std::memcpy(&compact[2 * NG * (r + Np * q)],
-                      &V.data[2 * NG * (state(r) + NpV * state(q))], 2 * NG * sizeof(double));
-      gamma = compact.data();
-    }
-    std::vector<double> e(Np);
-    const auto &h = eps->set->at("h").data;
-    const auto &pp = eps->set->at("p").data;
-    std::copy(h.begin(), h.end(), e.begin());
-    std::copy(pp.begin(), pp.end(), e.begin() + No);
+    const int64_t NG = V.lens[0];
+    std::vector<double> compact, e;
+    const double *gamma = step_inputs(vtx, eps, compact, e);
 
     if (opt_.dry_run) {
       std::printf("dry run: CCSD %s field, No=%lld Nv=%lld NG=%lld\n",

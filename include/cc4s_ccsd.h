@@ -145,6 +145,12 @@ int cc4s_ccsd_residuum(cc4s_ccsd_context *ctx, const double *T1,
 int cc4s_ccsd_energy(cc4s_ccsd_context *ctx, const double *T1,
                      const double *T2, double *out);
 
+/* (T) correlation energy of the context's current amplitudes (after run /
+ * iterate, or set_amplitudes): PerturbativeTriplesReference::run
+ * (PerturbativeTriplesReference.cxx:55-180, cT off), with the V_ppph slice
+ * formed for the call.  *energy = Re E. */
+int cc4s_ccsd_triples(cc4s_ccsd_context *ctx, double *energy);
+
 /* Energy adjoint at the context's current amplitudes (after iterate/run or
  * set_amplitudes).  The reference has no such entry point: its finite-size
  * analysis (FiniteSizeCorrection.cxx:50-133) contracts the converged

@@ -153,7 +153,7 @@ def test_tensor_round_trip_text_and_binary(built, tmp_path):
 
 
 @pytest.mark.parametrize("case,message", [
-    ("unknown", "unknown algorithm: PerturbativeTriples"),
+    ("unknown", "unknown algorithm: FiniteSizeCorrection"),
     ("fermi_high", "Fermi energy above all eigen energies."),
     ("fermi_low", "Fermi energy below all eigen energies."),
     ("version", "serialization versions mismatch"),
@@ -165,7 +165,7 @@ def test_reference_errors(built, tmp_path, case, message):
     syn.write_cc4s_inputs(inp, tmp_path)
     steps = STEPS + CCSD
     if case == "unknown":
-        steps += "\n- name: PerturbativeTriples\n  in: {}\n  out: {}\n"
+        steps += "\n- name: FiniteSizeCorrection\n  in: {}\n  out: {}\n"
     if case in ("fermi_high", "fermi_low"):
         h = yaml.safe_load((tmp_path / "EigenEnergies.yaml").read_text())
         h["metaData"]["fermiEnergy"] = 10.0 if case == "fermi_high" else -10.0

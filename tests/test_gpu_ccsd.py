@@ -269,3 +269,39 @@ def test_ueg_energies_of_the_reference_test_suite():
         assert out["convergenceReached"]
         for key in ("correlation", "direct", "exchange", "secondOrder"):
             assert abs(out["energy"][key] - ref[method][key]) < 1e-9, (method, key)
+
+
+@pytest.mark.parametrize("field", ["complex", "real"])
+@pytest.mark.parametrize("shape", [(3, 7, 5), (4, 12, 30)], ids=["o3v7G5", "o4v12G30"])
+def test_triples_parity(field, shape):
+    """(T) of given amplitudes (PerturbativeTriplesReference.cxx:55-180)
+    against the oracle's restatement, 1e-12 relative."""
+    No, Nv, NG = shape
+    print(f"triples {field} {shape}", flush=True)
+    inp = syn.generate(No, Nv, NG, field, 23)
+    T = _amps(No, Nv, field, 29, scale=0.1)
+    with api.CoupledCluster(No, Nv, NG, field=field) as cc:
+        cc.set_inputs(inp.gamma, inp.eps)
+        cc.set_amplitudes(*T)
+        e = cc.triples()
+    ref = O.perturbative_triples(inp.gamma, inp.eps, No, field, T)
+    assert abs(e - ref) <= 1e-12 * abs(ref)
+
+
+def test_ueg_triples_of_the_reference_test_suite():
+    """(T) of the GPU CCSD amplitudes of the uniform electron gas rs=1, No=7,
+    Nv=26 equals the reference's recorded PerturbativeTriplesReference
+    energy (correct.out.yaml:166-170) within 1e-10 Eh."""
+    from oracle import ueg
+    gamma, eps, _ = ueg.generate(7, 26, 1.0)
+    ref = ueg.UEG_RS1_NO7_NV26
+    s = ref["settings"]
+    with api.CoupledCluster(7, 26, gamma.shape[0], maxIterations=s["maxIterations"],
+                            energyConvergence=s["energyConvergence"],
+                            amplitudesConvergence=s["amplitudesConvergence"],
+                            maxResidua=s["maxResidua"],
+                            integralsSliceSize=s["integralsSliceSize"]) as cc:
+        cc.set_inputs(gamma, eps)
+        assert cc.run()["convergenceReached"]
+        e = cc.triples()
+    assert abs(e - ref["Triples"]["correlation"]) < 1e-10

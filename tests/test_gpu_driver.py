@@ -152,10 +152,10 @@ def test_vertex_with_more_states_than_the_energies(tmp_path):
 
 def test_reference_ueg_operation_file(tmp_path):
     """The reference's own operation file of test/tests/ueg/rs1.0-7occ-26virt
-    (Drccd, then Ccsd; its last step, (T), is outside this drop-in) on the
-    uniform-electron-gas inputs restated from UegVertexGenerator.cxx: both
-    CoupledCluster steps report the energies of its correct.out.yaml within
-    1e-9 Eh (the reference's check uses 1e-7)."""
+    (Drccd, Ccsd, PerturbativeTriplesReference) on the uniform-electron-gas
+    inputs restated from UegVertexGenerator.cxx: every step reports the
+    energies of its correct.out.yaml within 1e-9 Eh (the reference's check
+    uses 1e-7)."""
     from oracle import ueg
     gamma, eps, fermi = ueg.generate(7, 26, 1.0)
     inp = syn.CcsdInputs(gamma=gamma, eps=eps, No=7, field="complex", scale=1.0,
@@ -186,6 +186,14 @@ def test_reference_ueg_operation_file(tmp_path):
 {common}  out:
     energy: CcsdEnergy
     amplitudes: Amplitudes
+
+- name: PerturbativeTriplesReference
+  in:
+    coulombIntegrals: CoulombIntegrals
+    amplitudes: Amplitudes
+    slicedEigenEnergies: EigenEnergies
+  out:
+    {{}}
 """
     rep = run(tmp_path, ops)
     for step, method in ((5, "Drccd"), (6, "Ccsd")):
@@ -193,3 +201,7 @@ def test_reference_ueg_operation_file(tmp_path):
         assert out["convergenceReached"] == 1
         for key in ("correlation", "direct", "exchange", "secondOrder"):
             assert abs(out["energy"][key] - ueg.UEG_RS1_NO7_NV26[method][key]) < 1e-9, (method, key)
+    triples = rep["steps"][7]
+    assert triples["name"] == "PerturbativeTriplesReference"
+    assert abs(triples["out"]["energy"]["correlation"]
+               - ueg.UEG_RS1_NO7_NV26["Triples"]["correlation"]) < 1e-10
