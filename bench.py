@@ -49,6 +49,19 @@ VARIANTS = [f"gett_{t}_{a}_{b}" for t in ("c64", "f64") for a in ("ak", "am")
             for b in ("bk", "bn")]
 
 
+def measured_traffic(kernel, cfg):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated against a
+    plain copy kernel), or None."""
+    import re
+    name = f"profiles/{kernel}_hbm_traffic_no{cfg.No}_nv{cfg.Nv}.md"
+    path = os.path.join(ROOT, name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -117,13 +130,13 @@ def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
     }
 
 
-def make_context(cfg, precision, switch):
+def make_context(cfg, precision, switch, restart=None):
     return api.CoupledCluster(cfg.No, cfg.Nv, cfg.NG, field=cfg.field,
                               maxIterations=cfg.max_iterations,
                               energyConvergence=cfg.energy_convergence,
                               amplitudesConvergence=cfg.amplitudes_convergence,
                               maxResidua=cfg.max_residua, precision=precision,
-                              mixedSwitch=switch)
+                              mixedSwitch=switch, mixedRestart=restart)
 
 
 def converge(cc, label, first_passes):
@@ -196,6 +209,8 @@ def main():
                     help="comma-separated dR thresholds of the mixed -> FP64 "
                          "switch; one mixed run per value (the first is the "
                          "workload's written 1e-5)")
+    ap.add_argument("--mixed-restart", default="on", choices=("on", "off", "both"),
+                    help="restart the DIIS history at the mixed -> FP64 switch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint", default="auto", choices=("auto", "on", "off"),
                     help="energy adjoint at the FP64 fixed point (auto: on "
@@ -259,9 +274,11 @@ def main():
             if want and passes[-1]["converged"]:
                 adjoint = adjoint_report(cc, inp.gamma, passes[-1]["energy"])
         cc.close()
-    for n, sw in enumerate(switches if args.converge in ("both", "mixed") else []):
-        key = "mixed" if n == 0 else f"mixed_switch_{sw:g}"
-        cc = make_context(cfg, api.PRECISION_MIXED, sw)
+    restarts = {"on": [True], "off": [False], "both": [True, False]}[args.mixed_restart]
+    runs = [(sw, rs) for sw in switches for rs in restarts]
+    for n, (sw, rs) in enumerate(runs if args.converge in ("both", "mixed") else []):
+        key = "mixed" if n == 0 else f"mixed_switch_{sw:g}" + ("" if rs else "_norestart")
+        cc = make_context(cfg, api.PRECISION_MIXED, sw, rs)
         cc.set_inputs(inp.gamma, inp.eps)
         passes = converge(cc, key, [])
         mixed_times = [p["seconds"] for p in passes[1:] if p["precision_used"]]
@@ -278,6 +295,7 @@ def main():
             "energy": passes[-1]["energy"],
             "converged": bool(passes[-1]["converged"]),
             "switch": sw,
+            "diis_restart_at_switch": rs,
         }
         cc.close()
         if "fp64" in out_conv:
@@ -329,7 +347,7 @@ def main():
             "peak": FP64_MFMA_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_MFMA_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(dom, cfg),
             "launches": d["launches"],
             "avg_launch_ms": d["ms"] / max(1, d["launches"]),
             "flops_per_launch": d["flops"] / max(1, d["launches"]),

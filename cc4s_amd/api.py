@@ -55,6 +55,7 @@ class Params(C.Structure):
         ("method", C.c_int),
         ("linearized", C.c_int),
         ("adjacent_pairs_exchange", C.c_int),
+        ("mixed_restart", C.c_int),
     ]
 
 
@@ -163,7 +164,7 @@ class CoupledCluster:
                  levelShift=0.0, maxResidua=4, ppl=True,
                  integralsSliceSize=0, device=0, precision=PRECISION_FP64,
                  mixedSwitch=1e-5, method="Ccsd", linearized=False,
-                 adjacentPairsExchange=False):
+                 adjacentPairsExchange=False, mixedRestart=None):
         self.lib = load_library()
         p = default_params()
         p.field = FIELD_COMPLEX if field == "complex" else FIELD_REAL
@@ -183,6 +184,8 @@ class CoupledCluster:
         p.method = METHODS[method]
         p.linearized = 1 if linearized else 0
         p.adjacent_pairs_exchange = 1 if adjacentPairsExchange else 0
+        if mixedRestart is not None:
+            p.mixed_restart = 1 if mixedRestart else 0
         self.params = p
         self.field = field
         self.No, self.Nv, self.NG = No, Nv, NG

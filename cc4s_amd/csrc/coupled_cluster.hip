@@ -100,6 +100,12 @@ void DiisMixer::write_next(const void *R, const void *T, const double *eps_h,
   CC4S_HIP(hipStreamSynchronize(s_));
 }
 
+void DiisMixer::restart() {
+  next_ = 0;
+  count_ = 0;
+  filled_.assign(static_cast<size_t>(N_), false);
+}
+
 void DiisMixer::append(void *out) {
   const size_t N = static_cast<size_t>(N_);
   filled_[next_] = true;                          // DiisMixer.cxx:65-66
@@ -325,8 +331,10 @@ IterationReport CoupledCluster::iterate() {
     } else {
       ++stalled_passes_;
     }
-    if (rep.residuum_norm < p_.mixed_switch || stalled_passes_ >= p_.mixed_stall_passes)
+    if (rep.residuum_norm < p_.mixed_switch || stalled_passes_ >= p_.mixed_stall_passes) {
       mixed_phase_ = false;
+      if (p_.mixed_restart) mixer_->restart();
+    }
   }
   previous_e_ = rep.energy;
   iterations_done += 1;

@@ -28,6 +28,9 @@ struct CoupledClusterParams {
   // residuum's rounding floor (dR ~1e-8 at No=32, Nv=256) was reached
   int mixed_stall_passes = 3;
   double mixed_stall_gain = 1.5;
+  // restart the DIIS history at the switch: the split-bf16 residua in it
+  // carry rounding of ~ the floor, which a late switch would extrapolate with
+  bool mixed_restart = true;
 };
 
 // DIIS over (amplitudes, amplitude change) pairs held as ring-buffer slots
@@ -47,6 +50,8 @@ class DiisMixer {
   // overlaps, bordered solve, and `out` (full layout) = sum_j c_j A_j
   void append(void *out);
   double residuum_norm() const { return norm_; }
+  // forget every stored pair: the next append starts a new history
+  void restart();
   const std::vector<double> &coefficients() const { return coeffs_; }
   bool amplitudes_on_host() const { return on_host_; }
   // device bytes a mixer with these parameters allocates
