@@ -26,8 +26,8 @@ struct CoupledClusterParams {
   // the mixed phase also ends after `mixed_stall_passes` passes without a
   // `mixed_stall_gain`-fold drop of dR below its best value: the split
   // residuum's rounding floor (dR ~1e-8 at No=32, Nv=256) was reached
-  int mixed_stall_passes = 3;
-  double mixed_stall_gain = 1.5;
+  int mixed_stall_passes = 2;
+  double mixed_stall_gain = 2.0;
   // restart the DIIS history at the switch: the split-bf16 residua in it
   // carry rounding of ~ the floor, which a late switch would extrapolate with
   bool mixed_restart = true;

@@ -3,12 +3,17 @@
 // amplitude W_abcijk (PerturbativeTriplesReference.cxx:106-116, formed by
 // contractions on the MFMA engine); this kernel adds the disconnected part,
 // the spin-adapted combination and the denominator (:118-163):
-//   S_abc = W_abc + T1_ai V_bcjk + T1_bj V_acik + T1_ck V_abij
+//   S_x   = W_x + T1_{x1 i} V_{x2 x3 j k} + T1_{x2 j} V_{x1 x3 i k} + T1_{x3 k} V_{x1 x2 i j}
 //   Z     = 4/3 S_abc - 2 S_acb + 2/3 S_bca
-//   E    += Re(W_abc Z) / (e_i + e_j + e_k - e_a - e_b - e_c)
-// HBM-bound: W is read at three permutations per element.  Each block keeps
-// its running sum across launches in `acc[block]` (one stream: a fixed order),
-// so the energy is reproducible run to run.
+//   E    += Re(W_abc Z) / (e_i + e_j + e_k - e_a - e_b - e_c).
+// W and S are unchanged by a simultaneous permutation of the (a,i), (b,j),
+// (c,k) pairs, so the six orderings of a hole triple contribute together
+//   Re(W_abc Zs) / D,  Zs = 8 S_abc - 4 (S_bac + S_cba + S_acb) + 2 (S_bca + S_cab)
+// (the sum of Z over the six orderings, written at the ordering i <= j <= k);
+// i <= j <= k runs once, weighted 1 for distinct holes, 1/2 for two equal,
+// 1/6 for three equal.  HBM-bound: W is read at six permutations per
+// element.  Each block keeps its running sum across launches in
+// `acc[block]` (one stream: a fixed order), so the energy is reproducible.
 #include "triples.hpp"
 
 namespace cc4s_amd {
@@ -62,24 +67,19 @@ __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
     const int64_t a = e % v, b = (e / v) % v, c = (e / (v * v)) % v, kl = e / (v * v * v);
     const int64_t k = k0 + kl;
     const int64_t base = v * v * v * kl;
-    const T wabc = W[e];
-    const T wacb = W[base + a + v * (c + v * b)];
-    const T wbca = W[base + b + v * (c + v * a)];
-    const T t_ai = T1[a + v * i], t_bj = T1[b + v * j], t_ck = T1[c + v * k];
-    const T t_cj = T1[c + v * j], t_bk = T1[b + v * k];
-    const T t_bi = T1[b + v * i], t_ak = T1[a + v * k];
-    const T s_abc = S::plus(wabc, S::plus(S::mul(t_ai, Vx(b, c, j, k)),
-                                          S::plus(S::mul(t_bj, Vx(a, c, i, k)),
-                                                  S::mul(t_ck, Vx(a, b, i, j)))));
-    const T s_acb = S::plus(wacb, S::plus(S::mul(t_ai, Vx(c, b, j, k)),
-                                          S::plus(S::mul(t_cj, Vx(a, b, i, k)),
-                                                  S::mul(t_bk, Vx(a, c, i, j)))));
-    const T s_bca = S::plus(wbca, S::plus(S::mul(t_bi, Vx(c, a, j, k)),
-                                          S::plus(S::mul(t_cj, Vx(b, a, i, k)),
-                                                  S::mul(t_ak, Vx(b, c, i, j)))));
-    const T z = S::lin(4.0 / 3.0, s_abc, -2.0, s_acb, 2.0 / 3.0, s_bca);
+    // S at the particle tuple (x1, x2, x3) and the holes (i, j, k)
+    auto Sx = [&](int64_t x1, int64_t x2, int64_t x3) {
+      const T w = W[base + x1 + v * (x2 + v * x3)];
+      return S::plus(w, S::plus(S::mul(T1[x1 + v * i], Vx(x2, x3, j, k)),
+                                S::plus(S::mul(T1[x2 + v * j], Vx(x1, x3, i, k)),
+                                        S::mul(T1[x3 + v * k], Vx(x1, x2, i, j)))));
+    };
+    const T zs = S::lin(8.0, Sx(a, b, c),
+                        -4.0, S::plus(Sx(b, a, c), S::plus(Sx(c, b, a), Sx(a, c, b))),
+                        2.0, S::plus(Sx(b, c, a), Sx(c, a, b)));
+    const double weight = (i == j && j == k) ? 1.0 / 6.0 : (i == j || j == k) ? 0.5 : 1.0;
     const double d = ((eh[i] + eh[j]) + eh[k]) - ((ep[a] + ep[b]) + ep[c]);
-    sum += S::re_mul(wabc, z) / d;
+    sum += weight * S::re_mul(W[e], zs) / d;
   }
   __shared__ double sh[kBlock];
   sh[threadIdx.x] = sum;
@@ -97,8 +97,8 @@ void triples_energy(hipStream_t s, Field f, const void *W, const void *Vpphh,
                     const void *T1, const double *eps_h, const double *eps_p,
                     int64_t No, int64_t Nv, int64_t i, int64_t j, int64_t k0,
                     int64_t kc, double *acc) {
-  CC4S_ASSERT(i >= 0 && i < No && j >= 0 && j < No && k0 >= 0 && kc > 0 && k0 + kc <= No,
-              "triples: hole indices out of range");
+  CC4S_ASSERT(i >= 0 && i <= j && j < No && k0 >= j && kc > 0 && k0 + kc <= No,
+              "triples: hole indices out of range (needs i <= j <= k)");
   if (f == Field::Complex)
     triples_energy_kernel<true><<<kTriplesBlocks, kBlock, 0, s>>>(
         static_cast<const double2 *>(W), static_cast<const double2 *>(Vpphh),

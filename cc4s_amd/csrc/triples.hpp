@@ -10,8 +10,9 @@ namespace cc4s_amd {
 // fixed grid of the energy kernel; `acc` holds one running sum per block
 constexpr int kTriplesBlocks = 1024;
 
-// acc[block] += partial sums of Re(W Z)/D over W[a,b,c,k - k0], k in
-// [k0, k0 + kc), for the hole pair (i, j); Vpphh [v,v,o,o], T1 [v,o]
+// acc[block] += partial sums of the weighted Re(W Zs)/D over W[a,b,c,k - k0],
+// k in [k0, k0 + kc), for the hole pair i <= j <= k0; Vpphh [v,v,o,o],
+// T1 [v,o]
 void triples_energy(hipStream_t s, Field f, const void *W, const void *Vpphh,
                     const void *T1, const double *eps_h, const double *eps_p,
                     int64_t No, int64_t Nv, int64_t i, int64_t j, int64_t k0,

@@ -48,8 +48,8 @@ enum { CC4S_FIELD_REAL = 0, CC4S_FIELD_COMPLEX = 1 };
  * reproduces the reference.  MIXED (complex field only) runs them on bf16
  * MFMA with every FP64 operand split into hi + lo parts and FP32
  * accumulation, until the mixer's dR falls below mixed_switch (or stops
- * falling: three passes without a 1.5x gain, the rounding floor of the split
- * residuum), then switches to FP64 for the remaining passes (iterative
+ * falling: two passes without halving its best value, the rounding floor of
+ * the split residuum), then switches to FP64 for the remaining passes (iterative
  * refinement of the same fixed point); only FP64 passes can declare
  * convergence.  With mixed_restart the DIIS history restarts at the switch
  * (its split-bf16 residua carry rounding at the floor).  Denominators, energy, DIIS and every small contraction stay
