@@ -185,7 +185,7 @@ class CoupledCluster:
         p.linearized = 1 if linearized else 0
         p.adjacent_pairs_exchange = 1 if adjacentPairsExchange else 0
         if mixedRestart is not None:
-            p.mixed_restart = 1 if mixedRestart else 0
+            p.mixed_restart = {False: 0, True: 1, "auto": 2}[mixedRestart]
         self.params = p
         self.field = field
         self.No, self.Nv, self.NG = No, Nv, NG

@@ -65,7 +65,7 @@ void cc4s_ccsd_default_params(cc4s_ccsd_params *p) {
   p->precision = CC4S_PRECISION_FP64;
   p->mixed_switch = 1e-5;
   p->device = 0;
-  p->mixed_restart = 1;
+  p->mixed_restart = 2;
 }
 
 int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
@@ -88,7 +88,9 @@ int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
     p.ccsd.method = params->method == CC4S_METHOD_DRCCD ? Method::Drccd : Method::Ccsd;
     p.ccsd.linearized = params->linearized != 0;
     p.ccsd.adjacent_pairs_exchange = params->adjacent_pairs_exchange != 0;
-    p.mixed_restart = params->mixed_restart != 0;
+    CC4S_ASSERT(params->mixed_restart >= 0 && params->mixed_restart <= 2,
+                "mixed_restart must be 0, 1 or 2");
+    p.mixed_restart = params->mixed_restart;
     p.shape.field = params->field == CC4S_FIELD_REAL ? Field::Real : Field::Complex;
     p.shape.No = params->No;
     p.shape.Nv = params->Nv;

@@ -331,9 +331,12 @@ IterationReport CoupledCluster::iterate() {
     } else {
       ++stalled_passes_;
     }
-    if (rep.residuum_norm < p_.mixed_switch || stalled_passes_ >= p_.mixed_stall_passes) {
+    const bool stalled = stalled_passes_ >= p_.mixed_stall_passes;
+    if (rep.residuum_norm < p_.mixed_switch || stalled) {
       mixed_phase_ = false;
-      if (p_.mixed_restart) mixer_->restart();
+      if (p_.mixed_restart == 1 ||
+          (p_.mixed_restart == 2 && (stalled || rep.residuum_norm < 2e-6)))
+        mixer_->restart();
     }
   }
   previous_e_ = rep.energy;

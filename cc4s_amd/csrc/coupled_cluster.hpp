@@ -28,9 +28,12 @@ struct CoupledClusterParams {
   // residuum's rounding floor (dR ~1e-8 at No=32, Nv=256) was reached
   int mixed_stall_passes = 2;
   double mixed_stall_gain = 2.0;
-  // restart the DIIS history at the switch: the split-bf16 residua in it
-  // carry rounding of ~ the floor, which a late switch would extrapolate with
-  bool mixed_restart = true;
+  // DIIS history at the switch: its split-bf16 residua carry rounding at the
+  // floor, which a late switch would extrapolate with.  0 keeps it, 1
+  // restarts it, 2 restarts it when the switch comes at dR < 2e-6 or from
+  // the stall guard (measured: restarting at dR ~3e-6 costs passes, at
+  // dR < 1e-6 it saves them)
+  int mixed_restart = 2;
 };
 
 // DIIS over (amplitudes, amplitude change) pairs held as ring-buffer slots

@@ -51,8 +51,8 @@ enum { CC4S_FIELD_REAL = 0, CC4S_FIELD_COMPLEX = 1 };
  * falling: two passes without halving its best value, the rounding floor of
  * the split residuum), then switches to FP64 for the remaining passes (iterative
  * refinement of the same fixed point); only FP64 passes can declare
- * convergence.  With mixed_restart the DIIS history restarts at the switch
- * (its split-bf16 residua carry rounding at the floor).  Denominators, energy, DIIS and every small contraction stay
+ * convergence.  mixed_restart decides whether the DIIS history restarts at
+ * the switch (its split-bf16 residua carry rounding at the floor).  Denominators, energy, DIIS and every small contraction stay
  * FP64.  FP32 is accepted by the energy gradient only. */
 enum { CC4S_PRECISION_FP64 = 0, CC4S_PRECISION_MIXED = 1, CC4S_PRECISION_FP32 = 2 };
 
@@ -78,7 +78,8 @@ typedef struct {
   int method;                     /* CC4S_METHOD_* (Ccsd) */
   int linearized;                 /* Drccd: linearized (0), Drccd.cxx:53-55 */
   int adjacent_pairs_exchange;    /* Drccd: adjacentPairsExchange (0), Drccd.cxx:56-58 */
-  int mixed_restart;              /* MIXED: restart the DIIS history at the switch (1) */
+  int mixed_restart;              /* MIXED: DIIS history at the switch: 0 keep, 1 restart,
+                                     2 restart when dR < 2e-6 or stalled (2) */
 } cc4s_ccsd_params;
 
 /* One pass of the iteration loop (CoupledCluster.cxx:129-175). */
