@@ -131,13 +131,14 @@ def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
     }
 
 
-def make_context(cfg, precision, switch, restart=None):
+def make_context(cfg, precision, switch, restart=None, keep=None):
     return api.CoupledCluster(cfg.No, cfg.Nv, cfg.NG, field=cfg.field,
                               maxIterations=cfg.max_iterations,
                               energyConvergence=cfg.energy_convergence,
                               amplitudesConvergence=cfg.amplitudes_convergence,
                               maxResidua=cfg.max_residua, precision=precision,
-                              mixedSwitch=switch, mixedRestart=restart)
+                              mixedSwitch=switch, mixedRestart=restart,
+                              mixedRestartKeep=keep)
 
 
 def converge(cc, label, first_passes):
@@ -213,6 +214,8 @@ def main():
     ap.add_argument("--mixed-restart", default="auto", choices=("auto", "on", "off", "both"),
                     help="DIIS history at the mixed -> FP64 switch (auto: restart "
                          "when the switch comes at dR < 2e-6)")
+    ap.add_argument("--mixed-restart-keep", default="0",
+                    help="comma-separated numbers of recent DIIS pairs a restart keeps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--triples", action="store_true",
                     help="time the (T) correction at the FP64 fixed point")
@@ -294,11 +297,12 @@ def main():
         cc.close()
     restarts = {"auto": ["auto"], "on": [True], "off": [False],
                 "both": [True, False]}[args.mixed_restart]
-    runs = [(sw, rs) for sw in switches for rs in restarts]
-    for n, (sw, rs) in enumerate(runs if args.converge in ("both", "mixed") else []):
+    keeps = [int(x) for x in str(args.mixed_restart_keep).split(",") if x]
+    runs = [(sw, rs, kp) for sw in switches for rs in restarts for kp in keeps]
+    for n, (sw, rs, kp) in enumerate(runs if args.converge in ("both", "mixed") else []):
         key = "mixed" if n == 0 else f"mixed_switch_{sw:g}" + (
-            "" if rs in (True, "auto") else "_norestart")
-        cc = make_context(cfg, api.PRECISION_MIXED, sw, rs)
+            "" if rs in (True, "auto") else "_norestart") + (f"_keep{kp}" if kp else "")
+        cc = make_context(cfg, api.PRECISION_MIXED, sw, rs, kp)
         cc.set_inputs(inp.gamma, inp.eps)
         passes = converge(cc, key, [])
         mixed_times = [p["seconds"] for p in passes[1:] if p["precision_used"]]
@@ -316,6 +320,7 @@ def main():
             "converged": bool(passes[-1]["converged"]),
             "switch": sw,
             "diis_restart_at_switch": rs,
+            "diis_pairs_kept": kp,
         }
         cc.close()
         if "fp64" in out_conv:

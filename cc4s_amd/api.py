@@ -56,6 +56,7 @@ class Params(C.Structure):
         ("linearized", C.c_int),
         ("adjacent_pairs_exchange", C.c_int),
         ("mixed_restart", C.c_int),
+        ("mixed_restart_keep", C.c_int),
     ]
 
 
@@ -164,7 +165,7 @@ class CoupledCluster:
                  levelShift=0.0, maxResidua=4, ppl=True,
                  integralsSliceSize=0, device=0, precision=PRECISION_FP64,
                  mixedSwitch=1e-5, method="Ccsd", linearized=False,
-                 adjacentPairsExchange=False, mixedRestart=None):
+                 adjacentPairsExchange=False, mixedRestart=None, mixedRestartKeep=None):
         self.lib = load_library()
         p = default_params()
         p.field = FIELD_COMPLEX if field == "complex" else FIELD_REAL
@@ -186,6 +187,8 @@ class CoupledCluster:
         p.adjacent_pairs_exchange = 1 if adjacentPairsExchange else 0
         if mixedRestart is not None:
             p.mixed_restart = {False: 0, True: 1, "auto": 2}[mixedRestart]
+        if mixedRestartKeep is not None:
+            p.mixed_restart_keep = int(mixedRestartKeep)
         self.params = p
         self.field = field
         self.No, self.Nv, self.NG = No, Nv, NG

@@ -91,6 +91,9 @@ int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
     CC4S_ASSERT(params->mixed_restart >= 0 && params->mixed_restart <= 2,
                 "mixed_restart must be 0, 1 or 2");
     p.mixed_restart = params->mixed_restart;
+    CC4S_ASSERT(params->mixed_restart_keep >= 0 && params->mixed_restart_keep < params->max_residua,
+                "mixed_restart_keep must lie in [0, maxResidua)");
+    p.mixed_restart_keep = params->mixed_restart_keep;
     p.shape.field = params->field == CC4S_FIELD_REAL ? Field::Real : Field::Complex;
     p.shape.No = params->No;
     p.shape.Nv = params->Nv;

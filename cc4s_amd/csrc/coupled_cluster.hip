@@ -2,6 +2,7 @@
 
 #include "adjoint.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -100,10 +101,45 @@ void DiisMixer::write_next(const void *R, const void *T, const double *eps_h,
   CC4S_HIP(hipStreamSynchronize(s_));
 }
 
-void DiisMixer::restart() {
-  next_ = 0;
-  count_ = 0;
-  filled_.assign(static_cast<size_t>(N_), false);
+void DiisMixer::restart(int keep) {
+  const size_t N = static_cast<size_t>(N_);
+  const size_t k = std::min<size_t>(static_cast<size_t>(std::max(keep, 0)), count_);
+  // slots of the k most recent pairs, oldest first
+  std::vector<size_t> src(k);
+  for (size_t a = 0; a < k; ++a) src[a] = (next_ + N - k + a) % N;
+  std::vector<double> B = B_;
+  std::vector<DeviceBuffer> R(N), A;
+  std::vector<HostBuffer> Ah;
+  for (size_t i = 0; i < N; ++i) R[i] = std::move(R_[i]);
+  if (on_host_) {
+    for (size_t i = 0; i < N; ++i) Ah.push_back(std::move(Ahost_[i]));
+  } else {
+    A.resize(N);
+    for (size_t i = 0; i < N; ++i) A[i] = std::move(A_[i]);
+  }
+  // kept pairs to slots 0..k-1, every other buffer after them
+  std::vector<bool> used(N, false);
+  std::vector<size_t> order;
+  for (size_t a = 0; a < k; ++a) {
+    order.push_back(src[a]);
+    used[src[a]] = true;
+  }
+  for (size_t i = 0; i < N; ++i)
+    if (!used[i]) order.push_back(i);
+  for (size_t i = 0; i < N; ++i) {
+    R_[i] = std::move(R[order[i]]);
+    if (on_host_)
+      Ahost_[i] = std::move(Ah[order[i]]);
+    else
+      A_[i] = std::move(A[order[i]]);
+  }
+  for (size_t a = 0; a < k; ++a)
+    for (size_t b = 0; b < k; ++b)
+      B_[(a + 1) * (N + 1) + b + 1] = B[(src[a] + 1) * (N + 1) + src[b] + 1];
+  filled_.assign(N, false);
+  for (size_t a = 0; a < k; ++a) filled_[a] = true;
+  count_ = k;
+  next_ = k % N;
 }
 
 void DiisMixer::append(void *out) {
@@ -336,7 +372,7 @@ IterationReport CoupledCluster::iterate() {
       mixed_phase_ = false;
       if (p_.mixed_restart == 1 ||
           (p_.mixed_restart == 2 && (stalled || rep.residuum_norm < 2e-6)))
-        mixer_->restart();
+        mixer_->restart(p_.mixed_restart_keep);
     }
   }
   previous_e_ = rep.energy;

@@ -34,6 +34,7 @@ struct CoupledClusterParams {
   // the stall guard (measured: restarting at dR ~3e-6 costs passes, at
   // dR < 1e-6 it saves them)
   int mixed_restart = 2;
+  int mixed_restart_keep = 0;  // most recent pairs kept by a restart
 };
 
 // DIIS over (amplitudes, amplitude change) pairs held as ring-buffer slots
@@ -53,8 +54,9 @@ class DiisMixer {
   // overlaps, bordered solve, and `out` (full layout) = sum_j c_j A_j
   void append(void *out);
   double residuum_norm() const { return norm_; }
-  // forget every stored pair: the next append starts a new history
-  void restart();
+  // forget the stored pairs except the `keep` most recent ones, which move
+  // to the first slots: the next append continues a history of keep + 1
+  void restart(int keep = 0);
   const std::vector<double> &coefficients() const { return coeffs_; }
   bool amplitudes_on_host() const { return on_host_; }
   // device bytes a mixer with these parameters allocates

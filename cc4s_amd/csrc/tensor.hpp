@@ -81,6 +81,16 @@ class HostBuffer {
     o.p_ = nullptr;
     o.bytes_ = 0;
   }
+  HostBuffer &operator=(HostBuffer &&o) noexcept {
+    if (this != &o) {
+      if (p_) (void)hipHostFree(p_);
+      p_ = o.p_;
+      bytes_ = o.bytes_;
+      o.p_ = nullptr;
+      o.bytes_ = 0;
+    }
+    return *this;
+  }
   void *get() const { return p_; }
   size_t bytes() const { return bytes_; }
 

@@ -80,6 +80,7 @@ typedef struct {
   int adjacent_pairs_exchange;    /* Drccd: adjacentPairsExchange (0), Drccd.cxx:56-58 */
   int mixed_restart;              /* MIXED: DIIS history at the switch: 0 keep, 1 restart,
                                      2 restart when dR < 2e-6 or stalled (2) */
+  int mixed_restart_keep;         /* MIXED: most recent pairs a restart keeps (0) */
 } cc4s_ccsd_params;
 
 /* One pass of the iteration loop (CoupledCluster.cxx:129-175). */

@@ -98,3 +98,21 @@ def test_mixed_run_reaches_fp64_energy(shape, switch):
     assert not any(t["converged"] for t in mixed["trace"][:first64])
     # the stall guard costs the passes spent at the rounding floor
     assert mixed["iterations"] <= fp64["iterations"] + (12 if switch < 1e-12 else 4)
+
+
+def test_restart_keeping_recent_pairs():
+    """A DIIS restart at the switch that keeps the two most recent pairs
+    (moved to the first slots) still converges to the FP64 energy."""
+    No, Nv, NG = 12, 60, 300
+    inp = syn.generate(No, Nv, NG, "complex", 42, scale_rule="mp2_per_occupied")
+    kw = dict(maxIterations=60, energyConvergence=1e-10, amplitudesConvergence=1e-8,
+              maxResidua=8)
+    with api.CoupledCluster(No, Nv, NG, **kw) as cc:
+        cc.set_inputs(inp.gamma, inp.eps)
+        fp64 = cc.run()
+    with api.CoupledCluster(No, Nv, NG, precision=api.PRECISION_MIXED, mixedSwitch=1e-6,
+                            mixedRestart=True, mixedRestartKeep=2, **kw) as cc:
+        cc.set_inputs(inp.gamma, inp.eps)
+        mixed = cc.run()
+    assert mixed["convergenceReached"]
+    assert abs(mixed["energy"]["correlation"] - fp64["energy"]["correlation"]) < 1e-8
