@@ -30,6 +30,7 @@ Per-pass cost does not depend on the scale.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -154,6 +155,34 @@ def converge(cc, label, first_passes):
     return passes
 
 
+def device_memory():
+    """(free, total) GB of the current device, from the HIP runtime the
+    library loaded."""
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        return None
+    return round(free.value / 1e9, 2), round(total.value / 1e9, 2)
+
+
+PLACEMENTS = {0: "hbm", 1: "amplitude slots in pinned host memory",
+              2: "every slot in pinned host memory"}
+
+
+def phase_report(passes):
+    """Mean wall time per pass of the residuum and of the amplitude update +
+    DIIS step, and where the DIIS history lives."""
+    if not passes:
+        return None
+    return {
+        "passes": len(passes),
+        "seconds_per_iteration": float(np.mean([p["seconds"] for p in passes])),
+        "residuum_seconds": float(np.mean([p["residuum_seconds"] for p in passes])),
+        "update_and_diis_seconds": float(np.mean([p["mixer_seconds"] for p in passes])),
+        "diis_history": PLACEMENTS[passes[-1]["diis_placement"]],
+    }
+
+
 def adjoint_report(cc, gamma, energy, n_fd=20, h=1e-5, seed=2024):
     """Energy adjoint at the converged amplitudes: forward energy, FP64 and
     FP32 backward (gradient + structure factor S(G)), sum_G S(G) against E,
@@ -216,6 +245,9 @@ def main():
                          "when the switch comes at dR < 2e-6)")
     ap.add_argument("--mixed-restart-keep", default="0",
                     help="comma-separated numbers of recent DIIS pairs a restart keeps")
+    ap.add_argument("--mixed-steps", type=int, default=0,
+                    help="also time this many mixed-precision passes (after "
+                         "--warmup untimed ones) without converging")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--triples", action="store_true",
                     help="time the (T) correction at the FP64 fixed point")
@@ -228,6 +260,8 @@ def main():
                           "--gpus must be 1"}))
         sys.exit(2)
     switches = [float(x) for x in str(args.mixed_switch).split(",") if x]
+    if args.steps < 1 and args.converge != "none":
+        ap.error("--steps must be at least 1 unless --converge none")
     cfg = syn.CONFIGS[args.config]
     log(f"generating inputs: {cfg.name}, scale rule {args.scale}")
     t = time.perf_counter()
@@ -244,11 +278,12 @@ def main():
     lad = (0.0, 0.0, 0)
     value = None
     iter_flops = None
-    if args.converge in ("both", "fp64", "none"):
+    if args.converge in ("both", "fp64") or (args.converge == "none" and args.steps > 0):
         cc = make_context(cfg, api.PRECISION_FP64, switches[0])
         t = time.perf_counter()
         cc.set_inputs(inp.gamma, inp.eps)
-        log(f"inputs resident, V slices formed: {time.perf_counter() - t:.2f} s")
+        log(f"inputs resident, V slices formed: {time.perf_counter() - t:.2f} s; "
+            f"device memory free/total after setup: {device_memory()}")
         passes = []
         for w in range(args.warmup):
             r = cc.iterate()
@@ -266,6 +301,7 @@ def main():
                 f"{r['flops'] / r['seconds'] / 1e12:.2f} TFLOP/s")
         value = (time.perf_counter() - t0) / args.steps
         iter_flops = float(np.mean([r["flops"] for r in reps]))
+        phases = phase_report(reps)
         stats = {v: cc.kernel_stats(v) for v in VARIANTS}
         lad = cc.ladder_stats()
         cc.set_profiling(False)
@@ -295,6 +331,28 @@ def main():
                            "algorithmic_flops": fl, "tflops_per_s": fl / dt / 1e12}
                 log(f"(T): {json.dumps(triples)}")
         cc.close()
+    mixed_iter = None
+    if args.mixed_steps > 0:
+        cc = make_context(cfg, api.PRECISION_MIXED, switches[0])
+        cc.set_inputs(inp.gamma, inp.eps)
+        log(f"mixed context: device memory free/total after setup: {device_memory()}")
+        for w in range(args.warmup):
+            r = cc.iterate()
+            log(f"mixed warmup {r['iteration']}: E={r['energy']:.12f} "
+                f"dR={r['residuum_norm']:.3e} {r['seconds']:.3f} s")
+        mreps = []
+        for k in range(args.mixed_steps):
+            r = cc.iterate()
+            mreps.append(r)
+            log(f"mixed step {r['iteration']}: E={r['energy']:.12f} "
+                f"dR={r['residuum_norm']:.3e} {r['seconds']:.3f} s "
+                f"({'mixed' if r['precision_used'] else 'fp64'})")
+        cc.close()
+        mixed_iter = phase_report([r for r in mreps if r["precision_used"]])
+        if mixed_iter and value is not None:
+            mixed_iter["speedup_vs_fp64_iteration"] = value / mixed_iter["seconds_per_iteration"]
+            mixed_iter["algorithmic_tflops_per_s"] = (
+                iter_flops / mixed_iter["seconds_per_iteration"] / 1e12)
     restarts = {"auto": ["auto"], "on": [True], "off": [False],
                 "both": [True, False]}[args.mixed_restart]
     keeps = [int(x) for x in str(args.mixed_restart_keep).split(",") if x]
@@ -350,6 +408,10 @@ def main():
             "time_to_convergence": out_conv,
         },
     }
+    if value is not None:
+        out["config"]["fp64_iteration_phases"] = phases
+    if mixed_iter is not None:
+        out["config"]["mixed_iteration"] = mixed_iter
     if adjoint is not None:
         out["config"]["adjoint"] = adjoint
     if triples is not None:

@@ -68,6 +68,8 @@ class Iteration(C.Structure):
         ("direct", C.c_double), ("exchange", C.c_double),
         ("seconds", C.c_double), ("flops", C.c_double),
         ("converged", C.c_int), ("precision_used", C.c_int),
+        ("diis_placement", C.c_int),
+        ("residuum_seconds", C.c_double), ("mixer_seconds", C.c_double),
     ]
 
     def as_dict(self):

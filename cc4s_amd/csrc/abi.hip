@@ -45,6 +45,9 @@ void fill(cc4s_ccsd_iteration *out, const IterationReport &r) {
   out->flops = r.flops;
   out->converged = r.converged ? 1 : 0;
   out->precision_used = r.mixed ? CC4S_PRECISION_MIXED : CC4S_PRECISION_FP64;
+  out->diis_placement = r.diis_placement;
+  out->residuum_seconds = r.residuum_seconds;
+  out->mixer_seconds = r.mixer_seconds;
 }
 
 }  // namespace

@@ -287,19 +287,24 @@ __global__ __launch_bounds__(kThreads, 2) void gett_kernel(KArgs p) {
   }
 }
 
-// Complex GETT, second generation: 8 waves (2 x 4) on a BM x BN complex tile,
+// Complex GETT, second generation: NT/64 waves on a BM x BN complex tile,
 // one workgroup per CU.  LDS holds two K-stages (double buffer) with (re, im)
 // interleaved, so an MFMA operand pair is one 128-bit LDS read and a K-stage
 // needs one barrier.  Global loads for stage s+2 are issued while stage s is
 // multiplied; their K offsets were fetched a stage earlier still.
+//
+// M3 selects the three-product complex multiply (Gauss/Karatsuba):
+//   P1 = ar br, P2 = ai bi, P3 = (ar + ai)(br + bi),
+//   re = P1 - P2, im = P3 - P1 - P2,
+// three MFMA products per complex MAC instead of four.  Its error bound is
+// the four-product one with |ar|+|ai| and |br|+|bi| in place of |a|, |b|.
 constexpr int kZThreads = 512;
 
-template <int BM, int BN, int BK, int WM, int WN, bool AKC, bool BKC>
-__global__ __launch_bounds__(kZThreads, 1) void zgett_kernel(KArgs p) {
-  constexpr int NT = kZThreads;
+template <int NT, int BM, int BN, int BK, int WM, int WN, bool AKC, bool BKC, bool M3>
+__global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
   constexpr int MB = WM / 16, NB = WN / 16;
   constexpr int WAVES_N = BN / WN;
-  static_assert((BM / WM) * (BN / WN) * 64 == NT, "eight waves per workgroup");
+  static_assert((BM / WM) * (BN / WN) * 64 == NT, "one wave per WM x WN block");
   static_assert(BK % 4 == 0, "BK multiple of the MFMA k");
   constexpr int EA = BM * BK / NT;
   constexpr int EB = BN * BK / NT;
@@ -410,13 +415,15 @@ __global__ __launch_bounds__(kZThreads, 1) void zgett_kernel(KArgs p) {
   const int lane = t & 63;
   const int w = t >> 6;
   const int wm = w / WAVES_N, wn = w % WAVES_N;
-  dbl4 accR[MB][NB], accI[MB][NB];
+  // four-product: accR = re, accI = im; M3: accR = P1, accI = P2, accS = P3
+  dbl4 accR[MB][NB], accI[MB][NB], accS[M3 ? MB : 1][M3 ? NB : 1];
 #pragma unroll
   for (int i = 0; i < MB; ++i)
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       accR[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
       accI[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+      if constexpr (M3) accS[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
     }
 
   // prologue: stage 0 in LDS buffer 0, stage 1 in registers
@@ -442,28 +449,51 @@ __global__ __launch_bounds__(kZThreads, 1) void zgett_kernel(KArgs p) {
       for (int i = 0; i < MB; ++i) a[i] = As[buf][kr][wm * WM + i * 16 + (lane & 15)];
 #pragma unroll
       for (int j = 0; j < NB; ++j) b[j] = Bs[buf][kr][wn * WN + j * 16 + (lane & 15)];
+      if constexpr (M3) {
 #pragma unroll
-      for (int i = 0; i < MB; ++i)
+        for (int i = 0; i < MB; ++i)
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, accR[i][j], 0, 0, 0);
+          for (int j = 0; j < NB; ++j)
+            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, accR[i][j], 0, 0, 0);
 #pragma unroll
-      for (int i = 0; i < MB; ++i)
+        for (int i = 0; i < MB; ++i)
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
-          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].y, accI[i][j], 0, 0, 0);
+          for (int j = 0; j < NB; ++j)
+            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, accI[i][j], 0, 0, 0);
+        double bs[NB];
 #pragma unroll
-      for (int i = 0; i < MB; ++i) {
-        const double nai = -a[i].y;
+        for (int j = 0; j < NB; ++j) bs[j] = b[j].x + b[j].y;
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[j].y, accR[i][j], 0, 0, 0);
+        for (int i = 0; i < MB; ++i) {
+          const double as = a[i].x + a[i].y;
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            accS[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(as, bs[j], accS[i][j], 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, accR[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].y, accI[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+          const double nai = -a[i].y;
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[j].y, accR[i][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].x, accI[i][j], 0, 0, 0);
       }
-#pragma unroll
-      for (int i = 0; i < MB; ++i)
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].x, accI[i][j], 0, 0, 0);
     }
     if (s + 1 < nstages) {
       store_lds(buf ^ 1);  // stage s+1, loaded during stage s-1
@@ -495,8 +525,13 @@ __global__ __launch_bounds__(kZThreads, 1) void zgett_kernel(KArgs p) {
       for (int j = 0; j < NB; ++j) {
         if (!cColOk[j]) continue;
         const int64_t n = n0 + wn * WN + j * 16 + (lane & 15);
-        const double vr = accR[i][j][r];
-        const double vi = accI[i][j][r];
+        double vr = accR[i][j][r];
+        double vi = accI[i][j][r];
+        if constexpr (M3) {
+          const double p1 = vr, p2 = vi;
+          vr = p1 - p2;
+          vi = accS[i][j][r] - (p1 + p2);
+        }
         if (p.W) {
           const int64_t o = (static_cast<int64_t>(blockIdx.y) * p.M + m) * p.N + n;
           reinterpret_cast<double2 *>(p.W)[o] = make_double2(vr, vi);
@@ -607,22 +642,24 @@ void dispatch_gett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_
     gett_kernel<CPLX, BM, BN, BK, WM, WN, false, false><<<grid, kThreads, 0, s>>>(args);
 }
 
-template <int BM, int BN, int BK, int WM, int WN>
+template <int NT, int BM, int BN, int BK, int WM, int WN, bool M3>
 void dispatch_zgett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_t s) {
   if (L.a_kcontig && L.b_kcontig)
-    zgett_kernel<BM, BN, BK, WM, WN, true, true><<<grid, kZThreads, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, true, true, M3><<<grid, NT, 0, s>>>(args);
   else if (L.a_kcontig)
-    zgett_kernel<BM, BN, BK, WM, WN, true, false><<<grid, kZThreads, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, true, false, M3><<<grid, NT, 0, s>>>(args);
   else if (L.b_kcontig)
-    zgett_kernel<BM, BN, BK, WM, WN, false, true><<<grid, kZThreads, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, false, true, M3><<<grid, NT, 0, s>>>(args);
   else
-    zgett_kernel<BM, BN, BK, WM, WN, false, false><<<grid, kZThreads, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, false, false, M3><<<grid, NT, 0, s>>>(args);
 }
 
-// tile shapes: complex first generation 128x64, second generation 128x128
-// (complex elements); real 128x128
+// tile shapes (complex elements): first generation 128x64; second 128x128;
+// three-product kernels 128x64 with 8 waves of 32x32 (generation 3) or 4
+// waves of 64x32 (generation 4); real 128x128
 constexpr int kCBM = 128, kCBN = 64, kCBK = 16, kCWM = 64, kCWN = 32;
 constexpr int kZBM = 128, kZBN = 128, kZBK = 16, kZWM = 64, kZWN = 32;
+constexpr int kZ3BM = 128, kZ3BN = 64, kZ3BK = 16;
 constexpr int kRBM = 128, kRBN = 128, kRBK = 16, kRWM = 64, kRWN = 64;
 
 int g_complex_generation = 2;
@@ -632,6 +669,8 @@ void tile_of(Field f, int &BM, int &BN, int &BK) {
     BM = kRBM; BN = kRBN; BK = kRBK;
   } else if (g_complex_generation == 1) {
     BM = kCBM; BN = kCBN; BK = kCBK;
+  } else if (g_complex_generation >= 3) {
+    BM = kZ3BM; BN = kZ3BN; BK = kZ3BK;
   } else {
     BM = kZBM; BN = kZBN; BK = kZBK;
   }
@@ -643,7 +682,9 @@ inline int grid_for(int64_t n) {
 
 }  // namespace
 
-void set_complex_gett_generation(int g) { g_complex_generation = g == 1 ? 1 : 2; }
+void set_complex_gett_generation(int g) {
+  g_complex_generation = (g >= 1 && g <= 4) ? g : 2;
+}
 
 std::string gett_variant(const GettLaunch &L) {
   std::string n = L.field == Field::Complex ? "gett_c64" : "gett_f64";
@@ -677,7 +718,11 @@ void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start,
   dim3 grid(a.tilesM * a.tilesN, L.splits);
   if (start) CC4S_HIP(hipEventRecord(start, s));
   if (cplx && g_complex_generation == 2)
-    dispatch_zgett<kZBM, kZBN, kZBK, kZWM, kZWN>(L, a, grid, s);
+    dispatch_zgett<kZThreads, kZBM, kZBN, kZBK, kZWM, kZWN, false>(L, a, grid, s);
+  else if (cplx && g_complex_generation == 3)
+    dispatch_zgett<512, kZ3BM, kZ3BN, kZ3BK, 32, 32, true>(L, a, grid, s);
+  else if (cplx && g_complex_generation == 4)
+    dispatch_zgett<256, kZ3BM, kZ3BN, kZ3BK, 64, 32, true>(L, a, grid, s);
   else if (cplx)
     dispatch_gett<true, kCBM, kCBN, kCBK, kCWM, kCWN>(L, a, grid, s);
   else
@@ -862,7 +907,7 @@ GettPlan &ContractionEngine::plan_for(const View &A0, const View &B0,
   // split K when the tile grid cannot fill the 256 CUs twice over
   int BM, BN, BK;
   tile_of(C.field, BM, BN, BK);
-  const int slots = 256 * ((C.field == Field::Complex && g_complex_generation == 2) ? 1 : 2);
+  const int slots = 256 * ((C.field == Field::Complex && g_complex_generation >= 2) ? 1 : 2);
   const int64_t tiles = ((plan->M + BM - 1) / BM) * ((plan->N + BN - 1) / BN);
   const int64_t kstages = (plan->K + BK - 1) / BK;
   int64_t splits = 1;

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 namespace cc4s_amd {
 
@@ -16,29 +17,59 @@ namespace cc4s_amd {
 // ---------------------------------------------------------------------------
 
 namespace {
-constexpr size_t kStagingBytes = size_t(1) << 30;  // host-slot transfer chunk
+constexpr size_t kStagingBytes = size_t(1) << 30;   // amplitude-slot transfer chunk
+constexpr size_t kSlotChunkBytes = size_t(1) << 29; // per-slot chunk, all on host
 }
 
 size_t DiisMixer::device_bytes(Field f, int64_t No, int64_t Nv, int64_t N,
-                               bool on_host) {
+                               DiisPlacement where) {
   const size_t set = static_cast<size_t>(packed_set_size(No, Nv)) * scalar_bytes(f);
-  return on_host ? N * set + std::min(set, kStagingBytes) : 2 * N * set;
+  switch (where) {
+    case DiisPlacement::Device: return 2 * N * set;
+    case DiisPlacement::AmplitudesOnHost: return N * set + std::min(set, kStagingBytes);
+    case DiisPlacement::AllOnHost: return set + N * std::min(set, kSlotChunkBytes);
+  }
+  return 0;
 }
 
-DiisMixer::DiisMixer(Field f, int64_t No, int64_t Nv, int64_t N,
-                     bool amplitudes_on_host, hipStream_t s)
-    : f_(f), No_(No), Nv_(Nv), n_(packed_set_size(No, Nv)), N_(N),
-      on_host_(amplitudes_on_host), s_(s) {
+DiisMixer::DiisMixer(Field f, int64_t No, int64_t Nv, int64_t N, DiisPlacement where,
+                     hipStream_t s)
+    : f_(f), No_(No), Nv_(Nv), n_(packed_set_size(No, Nv)), N_(N), where_(where), s_(s) {
   CC4S_ASSERT(N >= 1 && N <= 16, "maxResidua must be in [1, 16]");
-  const size_t set = static_cast<size_t>(n_) * scalar_bytes(f);
-  R_.resize(N);
-  for (int64_t i = 0; i < N; ++i) R_[i].allocate(set);
-  if (on_host_) {
-    for (int64_t i = 0; i < N; ++i) Ahost_.emplace_back(set);
-    staging_.allocate(std::min(set, kStagingBytes));
-  } else {
-    A_.resize(N);
-    for (int64_t i = 0; i < N; ++i) A_[i].allocate(set);
+  const size_t es = scalar_bytes(f);
+  const size_t set = static_cast<size_t>(n_) * es;
+  // host-slot transfer chunk (CC4S_DIIS_CHUNK_BYTES lowers it)
+  size_t chunk_cap = where_ == DiisPlacement::AllOnHost ? kSlotChunkBytes : kStagingBytes;
+  if (const char *c = std::getenv("CC4S_DIIS_CHUNK_BYTES")) {
+    const long long v = std::atoll(c);
+    CC4S_ASSERT(v >= static_cast<long long>(es), "CC4S_DIIS_CHUNK_BYTES too small");
+    chunk_cap = std::min(chunk_cap, static_cast<size_t>(v));
+  }
+  switch (where_) {
+    case DiisPlacement::Device:
+      A_.resize(N);
+      R_.resize(N);
+      for (int64_t i = 0; i < N; ++i) {
+        A_[i].allocate(set);
+        R_[i].allocate(set);
+      }
+      break;
+    case DiisPlacement::AmplitudesOnHost:
+      R_.resize(N);
+      for (int64_t i = 0; i < N; ++i) R_[i].allocate(set);
+      for (int64_t i = 0; i < N; ++i) Ahost_.emplace_back(set);
+      chunk_ = static_cast<int64_t>(std::min(set, chunk_cap) / es);
+      staging_.allocate(chunk_ * es);
+      break;
+    case DiisPlacement::AllOnHost:
+      for (int64_t i = 0; i < N; ++i) {
+        Ahost_.emplace_back(set);
+        Rhost_.emplace_back(set);
+      }
+      current_.allocate(set);
+      chunk_ = static_cast<int64_t>(std::min(set, chunk_cap) / es);
+      staging_.allocate(static_cast<size_t>(N) * chunk_ * es);
+      break;
   }
   filled_.assign(N, false);
   const size_t M = N + 1;
@@ -83,21 +114,21 @@ std::vector<double> DiisMixer::solve(std::vector<double> a, size_t n) const {
 void DiisMixer::write_next(const void *R, const void *T, const double *eps_h,
                            const double *eps_p, double shift) {
   const size_t set = static_cast<size_t>(n_) * scalar_bytes(f_);
-  if (!on_host_) {
+  if (where_ == DiisPlacement::Device) {
     amplitude_update_packed(s_, f_, R, T, eps_h, eps_p, shift, A_[next_].get(),
                             R_[next_].get(), No_, Nv_);
     return;
   }
-  // T' goes through the residuum slot to the host: the update kernel runs
-  // for T' alone, the slot is copied out, then it runs again for dT (on the
-  // first pass dT = T' is already in place)
-  amplitude_update_packed(s_, f_, R, T, eps_h, eps_p, shift, R_[next_].get(),
-                          nullptr, No_, Nv_);
-  CC4S_HIP(hipMemcpyAsync(Ahost_[next_].get(), R_[next_].get(), set,
-                          hipMemcpyDeviceToHost, s_));
+  // T' goes through a device set to the host: the update kernel runs for T'
+  // alone, the set is copied out, then it runs again for dT (on the first
+  // pass dT = T' is already in place)
+  void *dev = where_ == DiisPlacement::AllOnHost ? current_.get() : R_[next_].get();
+  amplitude_update_packed(s_, f_, R, T, eps_h, eps_p, shift, dev, nullptr, No_, Nv_);
+  CC4S_HIP(hipMemcpyAsync(Ahost_[next_].get(), dev, set, hipMemcpyDeviceToHost, s_));
   if (T)
-    amplitude_update_packed(s_, f_, R, T, eps_h, eps_p, shift, nullptr,
-                            R_[next_].get(), No_, Nv_);
+    amplitude_update_packed(s_, f_, R, T, eps_h, eps_p, shift, nullptr, dev, No_, Nv_);
+  if (where_ == DiisPlacement::AllOnHost)
+    CC4S_HIP(hipMemcpyAsync(Rhost_[next_].get(), dev, set, hipMemcpyDeviceToHost, s_));
   CC4S_HIP(hipStreamSynchronize(s_));
 }
 
@@ -107,16 +138,6 @@ void DiisMixer::restart(int keep) {
   // slots of the k most recent pairs, oldest first
   std::vector<size_t> src(k);
   for (size_t a = 0; a < k; ++a) src[a] = (next_ + N - k + a) % N;
-  std::vector<double> B = B_;
-  std::vector<DeviceBuffer> R(N), A;
-  std::vector<HostBuffer> Ah;
-  for (size_t i = 0; i < N; ++i) R[i] = std::move(R_[i]);
-  if (on_host_) {
-    for (size_t i = 0; i < N; ++i) Ah.push_back(std::move(Ahost_[i]));
-  } else {
-    A.resize(N);
-    for (size_t i = 0; i < N; ++i) A[i] = std::move(A_[i]);
-  }
   // kept pairs to slots 0..k-1, every other buffer after them
   std::vector<bool> used(N, false);
   std::vector<size_t> order;
@@ -126,13 +147,17 @@ void DiisMixer::restart(int keep) {
   }
   for (size_t i = 0; i < N; ++i)
     if (!used[i]) order.push_back(i);
-  for (size_t i = 0; i < N; ++i) {
-    R_[i] = std::move(R[order[i]]);
-    if (on_host_)
-      Ahost_[i] = std::move(Ah[order[i]]);
-    else
-      A_[i] = std::move(A[order[i]]);
-  }
+  auto permute = [&](auto &slots) {
+    if (slots.empty()) return;
+    std::remove_reference_t<decltype(slots)> old(N);
+    for (size_t i = 0; i < N; ++i) old[i] = std::move(slots[i]);
+    for (size_t i = 0; i < N; ++i) slots[i] = std::move(old[order[i]]);
+  };
+  permute(A_);
+  permute(R_);
+  permute(Ahost_);
+  permute(Rhost_);
+  const std::vector<double> B = B_;
   for (size_t a = 0; a < k; ++a)
     for (size_t b = 0; b < k; ++b)
       B_[(a + 1) * (N + 1) + b + 1] = B[(src[a] + 1) * (N + 1) + src[b] + 1];
@@ -144,19 +169,43 @@ void DiisMixer::restart(int keep) {
 
 void DiisMixer::append(void *out) {
   const size_t N = static_cast<size_t>(N_);
+  const size_t es = scalar_bytes(f_);
   filled_[next_] = true;                          // DiisMixer.cxx:65-66
   // overlaps 2 Re <R_i|R_new> with every stored residuum (:69-77)
-  std::vector<const void *> xs;
   std::vector<size_t> which;
   for (size_t i = 0; i < N; ++i)
-    if (filled_[i]) {
-      xs.push_back(R_[i].get());
-      which.push_back(i);
+    if (filled_[i]) which.push_back(i);
+  const int nw = static_cast<int>(which.size());
+  std::vector<cplx> dots(nw, cplx(0.0, 0.0));
+  if (where_ != DiisPlacement::AllOnHost) {
+    std::vector<const void *> xs;
+    for (size_t i : which) xs.push_back(R_[i].get());
+    multi_dot_packed(s_, f_, xs.data(), nw, R_[next_].get(), No_, Nv_,
+                     partials_.get(), dots);
+  } else {
+    // every stored change streams through its staging chunk; the newest is
+    // resident.  Chunk sums are added in a fixed order.
+    std::vector<const void *> xs(nw);
+    std::vector<cplx> part;
+    for (int64_t t0 = 0; t0 < n_; t0 += chunk_) {
+      const int64_t len = std::min(chunk_, n_ - t0);
+      const char *y = static_cast<const char *>(current_.get()) + t0 * es;
+      for (int k = 0; k < nw; ++k) {
+        if (which[k] == next_) {
+          xs[k] = y;
+          continue;
+        }
+        char *dst = static_cast<char *>(staging_.get()) + k * chunk_ * es;
+        CC4S_HIP(hipMemcpyAsync(dst, static_cast<const char *>(Rhost_[which[k]].get()) + t0 * es,
+                                len * es, hipMemcpyHostToDevice, s_));
+        xs[k] = dst;
+      }
+      multi_dot_packed_range(s_, f_, xs.data(), nw, y, t0, len, No_, Nv_,
+                             partials_.get(), part);
+      for (int k = 0; k < nw; ++k) dots[k] += part[k];
     }
-  std::vector<cplx> dots;
-  multi_dot_packed(s_, f_, xs.data(), static_cast<int>(xs.size()), R_[next_].get(),
-                   No_, Nv_, partials_.get(), dots);
-  for (size_t k = 0; k < which.size(); ++k) {
+  }
+  for (int k = 0; k < nw; ++k) {
     const size_t i = which[k];
     const double overlap = 2.0 * dots[k].real();
     B_[(i + 1) * (N + 1) + next_ + 1] = overlap;
@@ -169,36 +218,57 @@ void DiisMixer::append(void *out) {
     for (size_t n = 0; n < dim; ++n) matrix[n + dim * m] = B_[n + (N + 1) * m];
   const std::vector<double> column = solve(matrix, dim);   // :89
   // next = sum_j c_j A_(next-j), residuum = sum_j c_j R_(next-j) (:92-104)
-  std::vector<const void *> As, Rs;
   std::vector<size_t> slots;
   coeffs_.clear();
   for (size_t j = 0; j < count_; ++j) {
     const size_t i = (next_ + N - j) % N;
     slots.push_back(i);
-    if (!on_host_) As.push_back(A_[i].get());
-    Rs.push_back(R_[i].get());
     coeffs_.push_back(column[i + 1]);
   }
-  double rr;
-  if (!on_host_) {
+  double rr = 0.0;
+  if (where_ == DiisPlacement::Device) {
+    std::vector<const void *> As, Rs;
+    for (size_t i : slots) {
+      As.push_back(A_[i].get());
+      Rs.push_back(R_[i].get());
+    }
     rr = combine_packed(s_, f_, As.data(), Rs.data(), coeffs_, No_, Nv_, out,
                         partials_.get());
   } else {
-    // stream every host slot through the staging buffer, accumulating in
-    // the same order (most recent first)
-    const size_t es = scalar_bytes(f_);
-    const int64_t chunk = static_cast<int64_t>(staging_.bytes() / es);
+    // stream every host amplitude slot through the staging buffer,
+    // accumulating in the same order (most recent first)
     for (size_t j = 0; j < slots.size(); ++j) {
       const char *src = static_cast<const char *>(Ahost_[slots[j]].get());
-      for (int64_t t0 = 0; t0 < n_; t0 += chunk) {
-        const int64_t len = std::min(chunk, n_ - t0);
+      for (int64_t t0 = 0; t0 < n_; t0 += chunk_) {
+        const int64_t len = std::min(chunk_, n_ - t0);
         CC4S_HIP(hipMemcpyAsync(staging_.get(), src + t0 * es, len * es,
                                 hipMemcpyHostToDevice, s_));
         unpack_accumulate(s_, f_, staging_.get(), t0, len, coeffs_[j], j == 0, out,
                           No_, Nv_);
       }
     }
-    rr = norm_packed(s_, f_, Rs.data(), coeffs_, No_, Nv_, partials_.get());
+    if (where_ == DiisPlacement::AmplitudesOnHost) {
+      std::vector<const void *> Rs;
+      for (size_t i : slots) Rs.push_back(R_[i].get());
+      rr = norm_packed(s_, f_, Rs.data(), coeffs_, No_, Nv_, partials_.get());
+    } else {
+      std::vector<const void *> Rs(slots.size());
+      for (int64_t t0 = 0; t0 < n_; t0 += chunk_) {
+        const int64_t len = std::min(chunk_, n_ - t0);
+        for (size_t j = 0; j < slots.size(); ++j) {
+          if (slots[j] == next_) {
+            Rs[j] = static_cast<const char *>(current_.get()) + t0 * es;
+            continue;
+          }
+          char *dst = static_cast<char *>(staging_.get()) + j * chunk_ * es;
+          CC4S_HIP(hipMemcpyAsync(dst, static_cast<const char *>(Rhost_[slots[j]].get()) + t0 * es,
+                                  len * es, hipMemcpyHostToDevice, s_));
+          Rs[j] = dst;
+        }
+        rr += norm_packed_range(s_, f_, Rs.data(), coeffs_, t0, len, No_, Nv_,
+                                partials_.get());
+      }
+    }
   }
   norm_ = std::sqrt(rr);                          // :106
   next_ = (next_ + 1) % N;                        // :105
@@ -227,9 +297,9 @@ CoupledCluster::CoupledCluster(const CoupledClusterParams &p, int device)
 }
 
 // The DIIS history is the largest allocation; it is placed once the vertex,
-// the integrals and the intermediates are resident.  Amplitude slots go to
-// pinned host memory when HBM cannot hold both kinds of slot (override with
-// CC4S_DIIS_PLACEMENT=device|host).
+// the integrals and the intermediates are resident: in HBM if it fits, else
+// with the amplitude slots in pinned host memory, else with every slot there
+// (override with CC4S_DIIS_PLACEMENT=device|host|host_all).
 void CoupledCluster::create_mixer() {
   const auto &s = p_.shape;
   size_t free_b = 0, total_b = 0;
@@ -245,17 +315,28 @@ void CoupledCluster::create_mixer() {
   const size_t avail = free_b > reserve ? free_b - reserve : 0;
   const char *env = std::getenv("CC4S_DIIS_PLACEMENT");
   const std::string placement = env ? env : "auto";
-  bool on_host;
-  if (placement == "host") {
-    on_host = true;
-  } else if (placement == "device") {
-    on_host = false;
+  auto need = [&](DiisPlacement w) {
+    return DiisMixer::device_bytes(s.field, s.No, s.Nv, p_.max_residua, w);
+  };
+  DiisPlacement where;
+  if (placement == "device") {
+    where = DiisPlacement::Device;
+  } else if (placement == "host") {
+    where = DiisPlacement::AmplitudesOnHost;
+  } else if (placement == "host_all") {
+    where = DiisPlacement::AllOnHost;
   } else {
-    on_host = DiisMixer::device_bytes(s.field, s.No, s.Nv, p_.max_residua, false) > avail;
-    CC4S_ASSERT(DiisMixer::device_bytes(s.field, s.No, s.Nv, p_.max_residua, true) <= avail,
-                "out of device memory for the DIIS history");
+    CC4S_ASSERT(placement == "auto",
+                "CC4S_DIIS_PLACEMENT must be auto, device, host or host_all");
+    if (need(DiisPlacement::Device) <= avail)
+      where = DiisPlacement::Device;
+    else if (need(DiisPlacement::AmplitudesOnHost) <= avail)
+      where = DiisPlacement::AmplitudesOnHost;
+    else
+      where = DiisPlacement::AllOnHost;
+    CC4S_ASSERT(need(where) <= avail, "out of device memory for the DIIS history");
   }
-  mixer_.reset(new DiisMixer(s.field, s.No, s.Nv, p_.max_residua, on_host, stream_));
+  mixer_.reset(new DiisMixer(s.field, s.No, s.Nv, p_.max_residua, where, stream_));
 }
 
 CoupledCluster::~CoupledCluster() {
@@ -341,12 +422,19 @@ IterationReport CoupledCluster::iterate() {
   engine_->set_mixed(mixed_phase_);
   ccsd_->get_residuum(T, R_.get());                     // :136
   engine_->set_mixed(false);
+  CC4S_HIP(hipStreamSynchronize(stream_));
+  const auto t1 = std::chrono::steady_clock::now();
   // :137-145: T' = -R/(Delta+s) and dT = T' - T, written into the mixer slot
   mixer_->write_next(R_.get(), T, static_cast<const double *>(eps_h_.get()),
                      static_cast<const double *>(eps_p_.get()), p_.level_shift);
   const bool is_second_order = !have_amplitudes_;
   mixer_->append(T_.get());                             // :145-147
   have_amplitudes_ = true;
+  CC4S_HIP(hipStreamSynchronize(stream_));
+  const auto t2 = std::chrono::steady_clock::now();
+  rep.residuum_seconds = std::chrono::duration<double>(t1 - t0).count();
+  rep.mixer_seconds = std::chrono::duration<double>(t2 - t1).count();
+  rep.diis_placement = static_cast<int>(mixer_->placement());
   rep.residuum_norm = mixer_->residuum_norm();          // :148
   energy(T_.get(), &rep.energy, &rep.direct, &rep.exchange);   // :149
   CC4S_HIP(hipStreamSynchronize(stream_));

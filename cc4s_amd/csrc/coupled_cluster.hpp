@@ -37,16 +37,19 @@ struct CoupledClusterParams {
   int mixed_restart_keep = 0;  // most recent pairs kept by a restart
 };
 
+// Where the DIIS history lives: both kinds of slot in HBM; the amplitude
+// slots in pinned host memory (they are read once per pass, to form the
+// extrapolated amplitudes); or every slot in pinned host memory, streamed
+// through HBM in chunks for the overlaps and the combination.
+enum class DiisPlacement { Device, AmplitudesOnHost, AllOnHost };
+
 // DIIS over (amplitudes, amplitude change) pairs held as ring-buffer slots
 // (DiisMixer.cxx:58-107).  Slots are pair-packed sets: T1 plus the pairs
 // (a,i) <= (b,j) of T2, half the size of a full set, exact because
-// T2_abij = T2_baji.  The amplitude-change slots stay in HBM (they take part
-// in every overlap); the amplitude slots are in HBM too, or in pinned host
-// memory when HBM cannot hold them (they are read once per pass, to form the
-// extrapolated amplitudes).
+// T2_abij = T2_baji.
 class DiisMixer {
  public:
-  DiisMixer(Field f, int64_t No, int64_t Nv, int64_t N, bool amplitudes_on_host,
+  DiisMixer(Field f, int64_t No, int64_t Nv, int64_t N, DiisPlacement where,
             hipStream_t s);
   // T' = -(R - shift T)/(Delta + shift) and dT = T' - T into slot(next)
   void write_next(const void *R, const void *T, const double *eps_h,
@@ -58,19 +61,23 @@ class DiisMixer {
   // to the first slots: the next append continues a history of keep + 1
   void restart(int keep = 0);
   const std::vector<double> &coefficients() const { return coeffs_; }
-  bool amplitudes_on_host() const { return on_host_; }
+  DiisPlacement placement() const { return where_; }
   // device bytes a mixer with these parameters allocates
-  static size_t device_bytes(Field f, int64_t No, int64_t Nv, int64_t N, bool on_host);
+  static size_t device_bytes(Field f, int64_t No, int64_t Nv, int64_t N,
+                             DiisPlacement where);
 
  private:
   std::vector<double> solve(std::vector<double> matrix, size_t dim) const;
+  void append_streamed(void *out);
   Field f_;
   int64_t No_, Nv_, n_, N_;
-  bool on_host_;
+  DiisPlacement where_;
   hipStream_t s_;
-  std::vector<DeviceBuffer> A_, R_;
-  std::vector<HostBuffer> Ahost_;
-  DeviceBuffer staging_;
+  std::vector<DeviceBuffer> A_, R_;     // HBM slots
+  std::vector<HostBuffer> Ahost_, Rhost_;  // host slots
+  DeviceBuffer current_;  // AllOnHost: the newest amplitude change
+  DeviceBuffer staging_;  // host-slot chunks, N_ of them for AllOnHost
+  int64_t chunk_ = 0;     // elements per staged chunk
   std::vector<bool> filled_;
   std::vector<double> B_;
   size_t next_ = 0, count_ = 0;
@@ -85,6 +92,8 @@ struct IterationReport {
   double delta_energy = 0, residuum_norm = 0, seconds = 0, flops = 0;
   bool converged = false;
   bool mixed = false;  // pass ran its large contractions in split bf16
+  int diis_placement = 0;  // DiisPlacement of the mixer
+  double residuum_seconds = 0, mixer_seconds = 0;
 };
 
 class CoupledCluster {

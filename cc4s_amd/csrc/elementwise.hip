@@ -416,29 +416,31 @@ __global__ __launch_bounds__(kBlock) void update_packed_kernel(
   }
 }
 
+// packed elements [t0, t0 + len) of one set; x[k] and y point at element t0
 template <bool CPLX>
 __global__ __launch_bounds__(kBlock) void dots_packed_kernel(DotArgs p, int count,
                                                              const typename Sc<CPLX>::T *y,
-                                                             int64_t n1, double2 *partials) {
+                                                             int64_t n1, int64_t t0,
+                                                             int64_t len, double2 *partials) {
   using S = Sc<CPLX>;
   __shared__ double2 sh[kBlock];
-  const int64_t np = n1 * (n1 + 1) / 2;
   double2 acc[kMaxDiis];
 #pragma unroll
   for (int k = 0; k < kMaxDiis; ++k) acc[k] = make_double2(0.0, 0.0);
-  for (int64_t t = blockIdx.x * int64_t(kBlock) + threadIdx.x; t < n1 + np;
-       t += int64_t(gridDim.x) * kBlock) {
+  for (int64_t u = blockIdx.x * int64_t(kBlock) + threadIdx.x; u < len;
+       u += int64_t(gridDim.x) * kBlock) {
+    const int64_t t = t0 + u;
     double w = 1.0;
     if (t >= n1) {
       int64_t P, Q;
       pair_of(t - n1, P, Q);
       w = P == Q ? 1.0 : 2.0;  // an off-diagonal pair stands for two elements
     }
-    const typename S::T yv = y[t];
+    const typename S::T yv = y[u];
 #pragma unroll
     for (int k = 0; k < kMaxDiis; ++k) {
       if (k < count) {
-        const double2 v = S::cdot(static_cast<const typename S::T *>(p.x[k])[t], yv);
+        const double2 v = S::cdot(static_cast<const typename S::T *>(p.x[k])[u], yv);
         acc[k].x += w * v.x;
         acc[k].y += w * v.y;
       }
@@ -514,16 +516,18 @@ __global__ __launch_bounds__(kBlock) void unpack_accumulate_kernel(
   }
 }
 
+// packed elements [t0, t0 + len); p.R[k] point at element t0
 template <bool CPLX>
 __global__ __launch_bounds__(kBlock) void norm_packed_kernel(DotArgs p, int count,
-                                                             int64_t n1,
+                                                             int64_t n1, int64_t t0,
+                                                             int64_t len,
                                                              double2 *partials) {
   using S = Sc<CPLX>;
   __shared__ double2 sh[kBlock];
-  const int64_t np = n1 * (n1 + 1) / 2;
   double nrm = 0.0;
-  for (int64_t t = blockIdx.x * int64_t(kBlock) + threadIdx.x; t < n1 + np;
-       t += int64_t(gridDim.x) * kBlock) {
+  for (int64_t u = blockIdx.x * int64_t(kBlock) + threadIdx.x; u < len;
+       u += int64_t(gridDim.x) * kBlock) {
+    const int64_t t = t0 + u;
     double w = 1.0;
     if (t >= n1) {
       int64_t P, Q;
@@ -532,7 +536,7 @@ __global__ __launch_bounds__(kBlock) void norm_packed_kernel(DotArgs p, int coun
     }
     typename S::T rv = S::zero();
     for (int k = 0; k < count; ++k)
-      rv = S::add(rv, S::scale(static_cast<const typename S::T *>(p.R[k])[t], p.c[k]));
+      rv = S::add(rv, S::scale(static_cast<const typename S::T *>(p.R[k])[u], p.c[k]));
     nrm += w * S::cdot(rv, rv).x;
   }
   const double2 s = block_sum(make_double2(nrm, 0.0), sh);
@@ -558,8 +562,16 @@ void unpack_accumulate(hipStream_t s, Field f, const void *chunk, int64_t t0,
 double norm_packed(hipStream_t s, Field f, const void *const *R,
                    const std::vector<double> &c, int64_t No, int64_t Nv,
                    void *partials) {
+  return norm_packed_range(s, f, R, c, 0, packed_set_size(No, Nv), No, Nv, partials);
+}
+
+double norm_packed_range(hipStream_t s, Field f, const void *const *R,
+                         const std::vector<double> &c, int64_t t0, int64_t len,
+                         int64_t No, int64_t Nv, void *partials) {
   const int count = static_cast<int>(c.size());
   CC4S_ASSERT(count >= 1 && count <= kMaxDiis, "too many DIIS vectors");
+  CC4S_ASSERT(t0 >= 0 && len >= 0 && t0 + len <= packed_set_size(No, Nv),
+              "packed range out of bounds");
   DotArgs a;
   for (int k = 0; k < count; ++k) {
     a.R[k] = R[k];
@@ -568,9 +580,11 @@ double norm_packed(hipStream_t s, Field f, const void *const *R,
   double2 *part = static_cast<double2 *>(partials);
   double2 *res = part + kReduceBlocks;
   if (f == Field::Complex)
-    norm_packed_kernel<true><<<kReduceBlocks, kBlock, 0, s>>>(a, count, Nv * No, part);
+    norm_packed_kernel<true><<<kReduceBlocks, kBlock, 0, s>>>(a, count, Nv * No, t0,
+                                                              len, part);
   else
-    norm_packed_kernel<false><<<kReduceBlocks, kBlock, 0, s>>>(a, count, Nv * No, part);
+    norm_packed_kernel<false><<<kReduceBlocks, kBlock, 0, s>>>(a, count, Nv * No, t0,
+                                                               len, part);
   CC4S_HIP(hipGetLastError());
   finish_kernel<<<1, kBlock, 0, s>>>(part, 1, kReduceBlocks, res);
   CC4S_HIP(hipGetLastError());
@@ -603,17 +617,26 @@ void amplitude_update_packed(hipStream_t s, Field f, const void *R, const void *
 void multi_dot_packed(hipStream_t s, Field f, const void *const *x, int count,
                       const void *y, int64_t No, int64_t Nv, void *partials,
                       std::vector<cplx> &out) {
+  multi_dot_packed_range(s, f, x, count, y, 0, packed_set_size(No, Nv), No, Nv,
+                         partials, out);
+}
+
+void multi_dot_packed_range(hipStream_t s, Field f, const void *const *x, int count,
+                            const void *y, int64_t t0, int64_t len, int64_t No,
+                            int64_t Nv, void *partials, std::vector<cplx> &out) {
   CC4S_ASSERT(count >= 1 && count <= kMaxDiis, "too many DIIS vectors");
+  CC4S_ASSERT(t0 >= 0 && len >= 0 && t0 + len <= packed_set_size(No, Nv),
+              "packed range out of bounds");
   DotArgs a;
   for (int k = 0; k < count; ++k) a.x[k] = x[k];
   double2 *part = static_cast<double2 *>(partials);
   double2 *res = part + static_cast<size_t>(kReduceBlocks) * count;
   if (f == Field::Complex)
     dots_packed_kernel<true><<<kReduceBlocks, kBlock, 0, s>>>(
-        a, count, static_cast<const double2 *>(y), Nv * No, part);
+        a, count, static_cast<const double2 *>(y), Nv * No, t0, len, part);
   else
     dots_packed_kernel<false><<<kReduceBlocks, kBlock, 0, s>>>(
-        a, count, static_cast<const double *>(y), Nv * No, part);
+        a, count, static_cast<const double *>(y), Nv * No, t0, len, part);
   CC4S_HIP(hipGetLastError());
   finish_kernel<<<1, kBlock, 0, s>>>(part, count, kReduceBlocks, res);
   CC4S_HIP(hipGetLastError());

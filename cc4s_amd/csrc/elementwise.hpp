@@ -74,6 +74,15 @@ void unpack_accumulate(hipStream_t s, Field f, const void *chunk, int64_t t0,
 double norm_packed(hipStream_t s, Field f, const void *const *R,
                    const std::vector<double> &c, int64_t No, int64_t Nv,
                    void *partials);
+// the two reductions above restricted to the packed elements [t0, t0 + len)
+// of a set: x[k], y and R[k] point at element t0 (chunks of slots streamed
+// from host memory); summing the chunks gives the whole-set value
+void multi_dot_packed_range(hipStream_t s, Field f, const void *const *x, int count,
+                            const void *y, int64_t t0, int64_t len, int64_t No,
+                            int64_t Nv, void *partials, std::vector<cplx> &out);
+double norm_packed_range(hipStream_t s, Field f, const void *const *R,
+                         const std::vector<double> &c, int64_t t0, int64_t len,
+                         int64_t No, int64_t Nv, void *partials);
 
 // true when the complex vertex [NG, Np, Np] satisfies
 // Gamma[G,q,p] == conj(Gamma[G,p,q]) exactly for every G, p, q

@@ -1,8 +1,11 @@
 // Contraction-kernel benchmark on the CCSD statement shapes.
 //
 //   gett_bench [generation=2] [repetitions=5] [No Nv NG slice]
-//   (generation 1, 2: FP64 kernels; 3: split-bf16 mixed precision;
-//    0: a plain 1 GiB device copy kernel, the reference for HBM counters)
+//   (generation 1, 2: FP64 four-product kernels; 3: split-bf16 mixed
+//    precision; 4, 5: FP64 three-product kernels (contraction-engine
+//    generations 3 and 4), each checked against generation 2 on the same
+//    operands; 0: a plain 1 GiB device copy kernel, the reference for HBM
+//    counters)
 //
 // Runs the complex GETT on device buffers for the shapes of the ladder
 // (formation and contraction, Ccsd.cxx:588-590), the ring and crossed-ring
@@ -11,6 +14,7 @@
 // (8 flop per complex MAC).  Default sizes: No=32, Nv=256, NG=1500, slice 32.
 #include "contract.hpp"
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -77,7 +81,8 @@ int main(int argc, char **argv) {
   const int64_t G = argc > 5 ? std::atoll(argv[5]) : 1500;
   const int64_t s = argc > 6 ? std::atoll(argv[6]) : o;
   // generation 3: the split-bf16 mixed-precision path (packing included)
-  set_complex_gett_generation(gen == 1 ? 1 : 2);
+  const int engine_gen = gen == 1 ? 1 : gen == 4 ? 3 : gen == 5 ? 4 : 2;
+  const bool check = gen == 4 || gen == 5;
   const std::vector<Case> cases = {
       {"ladder-form  Gxc,Gyd->cxdy", "Gxc", "Gyd", "cxdy", {G, s, v}, {G, s, v}, {v, s, v, s}},
       {"ladder-ctr   cxdy,cdij->xyij", "cxdy", "cdij", "xyij", {v, s, v, s}, {v, v, o, o}, {s, s, o, o}},
@@ -89,6 +94,9 @@ int main(int argc, char **argv) {
   try {
     hipStream_t st;
     CC4S_HIP(hipStreamCreate(&st));
+    set_complex_gett_generation(2);
+    ContractionEngine Eref(st);
+    set_complex_gett_generation(engine_gen);
     ContractionEngine E(st);
     E.set_profiling(true);
     if (gen == 3) {
@@ -101,6 +109,25 @@ int main(int argc, char **argv) {
       fill<<<4096, 256, 0, st>>>(A.buf.as<double>(), A.size() * 2, 1.0);
       fill<<<4096, 256, 0, st>>>(B.buf.as<double>(), B.size() * 2, 1.0);
       fill<<<4096, 256, 0, st>>>(C.buf.as<double>(), C.size() * 2, 1.0);
+      double rel = -1.0;
+      if (check) {
+        // relative Frobenius difference to the four-product kernel
+        const size_t n = C.size() * 2;
+        std::vector<double> ref(n), got(n);
+        set_complex_gett_generation(2);
+        Eref.contract(1.0, A.v, B.v, 0.0, C.v);
+        CC4S_HIP(hipMemcpyAsync(ref.data(), C.buf.get(), n * 8, hipMemcpyDeviceToHost, st));
+        set_complex_gett_generation(engine_gen);
+        E.contract(1.0, A.v, B.v, 0.0, C.v);
+        CC4S_HIP(hipMemcpyAsync(got.data(), C.buf.get(), n * 8, hipMemcpyDeviceToHost, st));
+        CC4S_HIP(hipStreamSynchronize(st));
+        double num = 0, den = 0;
+        for (size_t i = 0; i < n; ++i) {
+          num += (got[i] - ref[i]) * (got[i] - ref[i]);
+          den += ref[i] * ref[i];
+        }
+        rel = den > 0 ? std::sqrt(num / den) : 0.0;
+      }
       E.contract(1.0, A.v, B.v, 0.0, C.v);  // plan and warm up
       CC4S_HIP(hipStreamSynchronize(st));
       E.reset_timings();
@@ -121,9 +148,11 @@ int main(int argc, char **argv) {
         kms += kv.second.ms;
         var += kv.first + " ";
       }
-      std::printf("gen%d %-32s %8.3f ms/call  %6.2f TFLOP/s (kernel only %6.2f)  %s\n",
+      std::printf("gen%d %-32s %8.3f ms/call  %6.2f TFLOP/s (kernel only %6.2f)  %s",
                   gen, c.name, ms / reps, fl / (ms * 1e-3) / 1e12,
                   fl / (kms * 1e-3) / 1e12, var.c_str());
+      if (check) std::printf(" rel.diff vs gen2 %.2e", rel);
+      std::printf("\n");
       std::fflush(stdout);
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);

@@ -94,6 +94,10 @@ typedef struct {
   double flops;            /* algorithmic flops of the pass (8 per complex MAC) */
   int converged;           /* |dE| < energyConvergence and dR < amplitudesConvergence */
   int precision_used;      /* CC4S_PRECISION_* */
+  int diis_placement;      /* DIIS history: 0 in HBM, 1 amplitude slots in pinned
+                              host memory, 2 every slot in pinned host memory */
+  double residuum_seconds; /* wall time of the residuum */
+  double mixer_seconds;    /* wall time of the amplitude update and DIIS step */
 } cc4s_ccsd_iteration;
 
 /* Result map of CoupledCluster::run (CoupledCluster.cxx:184-191,245-248). */

@@ -53,3 +53,21 @@ def test_unsupported_precision_is_refused_before_touching_a_device():
     with pytest.raises(api.CcsdError, match="complex field"):
         api.CoupledCluster(2, 4, 3, field="real", precision=api.PRECISION_MIXED)
 
+
+
+@pytest.mark.parametrize("cname,pyclass", [("cc4s_ccsd_params", "Params"),
+                                           ("cc4s_ccsd_iteration", "Iteration"),
+                                           ("cc4s_ccsd_result", "Result")])
+def test_ctypes_structs_follow_the_header(cname, pyclass):
+    """The ctypes mirrors list the header's struct members in its order."""
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "cc4s_ccsd.h").read_text(),
+                  flags=re.S)
+    body = re.search(r"typedef struct \{([^{}]*)\}\s*" + cname + ";", text).group(1)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        parts = decl.split(None, 1)
+        names += [n.strip().lstrip("*") for n in parts[1].split(",")]
+    assert [f for f, _ in getattr(api, pyclass)._fields_] == names

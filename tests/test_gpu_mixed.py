@@ -100,9 +100,14 @@ def test_mixed_run_reaches_fp64_energy(shape, switch):
     assert mixed["iterations"] <= fp64["iterations"] + (12 if switch < 1e-12 else 4)
 
 
-def test_restart_keeping_recent_pairs():
+@pytest.mark.parametrize("placement", ["device", "host", "host_all"])
+def test_restart_keeping_recent_pairs(placement, monkeypatch):
     """A DIIS restart at the switch that keeps the two most recent pairs
-    (moved to the first slots) still converges to the FP64 energy."""
+    (moved to the first slots, wherever the slots live) still converges to
+    the FP64 energy."""
+    print(f"restart keep 2, DIIS slots {placement}", flush=True)
+    monkeypatch.setenv("CC4S_DIIS_PLACEMENT", placement)
+    monkeypatch.setenv("CC4S_DIIS_CHUNK_BYTES", "1000000")
     No, Nv, NG = 12, 60, 300
     inp = syn.generate(No, Nv, NG, "complex", 42, scale_rule="mp2_per_occupied")
     kw = dict(maxIterations=60, energyConvergence=1e-10, amplitudesConvergence=1e-8,
