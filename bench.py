@@ -132,14 +132,22 @@ def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
     }
 
 
-def make_context(cfg, precision, switch, restart=None, keep=None):
+MIXED_OPTIONS = {}
+
+
+def make_context(cfg, precision, switch, restart=None, keep=None, refine=None):
+    """A context of the workload; mixed contexts take the split settings of
+    the command line (MIXED_OPTIONS)."""
+    extra = dict(MIXED_OPTIONS) if precision == api.PRECISION_MIXED else {}
+    if refine is not None and precision == api.PRECISION_MIXED:
+        extra["mixedRefine"] = refine
     return api.CoupledCluster(cfg.No, cfg.Nv, cfg.NG, field=cfg.field,
                               maxIterations=cfg.max_iterations,
                               energyConvergence=cfg.energy_convergence,
                               amplitudesConvergence=cfg.amplitudes_convergence,
                               maxResidua=cfg.max_residua, precision=precision,
                               mixedSwitch=switch, mixedRestart=restart,
-                              mixedRestartKeep=keep)
+                              mixedRestartKeep=keep, **extra)
 
 
 def converge(cc, label, first_passes):
@@ -149,7 +157,8 @@ def converge(cc, label, first_passes):
             len(passes) < cc.params.max_iterations:
         r = cc.iterate()
         passes.append(r)
-        log(f"{label} {r['iteration']:3d} {'mixed' if r['precision_used'] else 'fp64 '} "
+        kind = ("fp64 ", "mixed", "anchr", "corr ")[r["pass_kind"]]
+        log(f"{label} {r['iteration']:3d} {kind} "
             f"E={r['energy']:.12f} dE={r['delta_energy']:.3e} "
             f"dR={r['residuum_norm']:.3e} {r['seconds']:.3f} s")
     return passes
@@ -245,6 +254,14 @@ def main():
                          "when the switch comes at dR < 2e-6)")
     ap.add_argument("--mixed-restart-keep", default="0",
                     help="comma-separated numbers of recent DIIS pairs a restart keeps")
+    ap.add_argument("--mixed-refine", default=None,
+                    help="comma-separated refinement modes after the split passes "
+                         "(0 FP64 passes, 1/2 FP64 anchor + defect-corrected split "
+                         "passes); default: the library's")
+    ap.add_argument("--mixed-split", default=None, choices=("bf16", "half"),
+                    help="split format of the mixed passes (default: the library's)")
+    ap.add_argument("--mixed-max-k", type=int, default=None,
+                    help="longest FP32 sum of a split contraction (default: the library's)")
     ap.add_argument("--mixed-steps", type=int, default=0,
                     help="also time this many mixed-precision passes (after "
                          "--warmup untimed ones) without converging")
@@ -260,6 +277,12 @@ def main():
                           "--gpus must be 1"}))
         sys.exit(2)
     switches = [float(x) for x in str(args.mixed_switch).split(",") if x]
+    if args.mixed_split is not None:
+        MIXED_OPTIONS["mixedSplit"] = args.mixed_split
+    if args.mixed_max_k is not None:
+        MIXED_OPTIONS["mixedMaxK"] = args.mixed_max_k
+    refines = ([int(x) for x in str(args.mixed_refine).split(",") if x]
+               if args.mixed_refine is not None else [None])
     if args.steps < 1 and args.converge != "none":
         ap.error("--steps must be at least 1 unless --converge none")
     cfg = syn.CONFIGS[args.config]
@@ -356,11 +379,13 @@ def main():
     restarts = {"auto": ["auto"], "on": [True], "off": [False],
                 "both": [True, False]}[args.mixed_restart]
     keeps = [int(x) for x in str(args.mixed_restart_keep).split(",") if x]
-    runs = [(sw, rs, kp) for sw in switches for rs in restarts for kp in keeps]
-    for n, (sw, rs, kp) in enumerate(runs if args.converge in ("both", "mixed") else []):
+    runs = [(sw, rs, kp, rf) for sw in switches for rs in restarts for kp in keeps
+            for rf in refines]
+    for n, (sw, rs, kp, rf) in enumerate(runs if args.converge in ("both", "mixed") else []):
         key = "mixed" if n == 0 else f"mixed_switch_{sw:g}" + (
-            "" if rs in (True, "auto") else "_norestart") + (f"_keep{kp}" if kp else "")
-        cc = make_context(cfg, api.PRECISION_MIXED, sw, rs, kp)
+            "" if rs in (True, "auto") else "_norestart") + (f"_keep{kp}" if kp else "") + (
+            f"_refine{rf}" if rf else "")
+        cc = make_context(cfg, api.PRECISION_MIXED, sw, rs, kp, rf)
         cc.set_inputs(inp.gamma, inp.eps)
         passes = converge(cc, key, [])
         mixed_times = [p["seconds"] for p in passes[1:] if p["precision_used"]]
@@ -379,6 +404,10 @@ def main():
             "switch": sw,
             "diis_restart_at_switch": rs,
             "diis_pairs_kept": kp,
+            "refine": cc.params.mixed_refine,
+            "split": {0: "bf16", 1: "half"}[cc.params.mixed_split],
+            "max_k": cc.params.mixed_max_k,
+            "pass_kinds": "".join("FSAC"[p["pass_kind"]] for p in passes),
         }
         cc.close()
         if "fp64" in out_conv:

@@ -57,6 +57,9 @@ class Params(C.Structure):
         ("adjacent_pairs_exchange", C.c_int),
         ("mixed_restart", C.c_int),
         ("mixed_restart_keep", C.c_int),
+        ("mixed_refine", C.c_int),
+        ("mixed_split", C.c_int),
+        ("mixed_max_k", C.c_int64),
     ]
 
 
@@ -70,6 +73,7 @@ class Iteration(C.Structure):
         ("converged", C.c_int), ("precision_used", C.c_int),
         ("diis_placement", C.c_int),
         ("residuum_seconds", C.c_double), ("mixer_seconds", C.c_double),
+        ("pass_kind", C.c_int),
     ]
 
     def as_dict(self):
@@ -158,6 +162,9 @@ def _fortran(a, dtype):
     return np.asfortranarray(np.asarray(a, dtype=dtype))
 
 
+MIXED_SPLITS = {"bf16": 0, "half": 1}
+
+
 class CoupledCluster:
     """One CCSD run on the GPU: the CoupledCluster step with method Ccsd and
     DiisMixer, arguments named as in a cc4s operation file."""
@@ -167,7 +174,8 @@ class CoupledCluster:
                  levelShift=0.0, maxResidua=4, ppl=True,
                  integralsSliceSize=0, device=0, precision=PRECISION_FP64,
                  mixedSwitch=1e-5, method="Ccsd", linearized=False,
-                 adjacentPairsExchange=False, mixedRestart=None, mixedRestartKeep=None):
+                 adjacentPairsExchange=False, mixedRestart=None, mixedRestartKeep=None,
+                 mixedRefine=None, mixedSplit=None, mixedMaxK=None):
         self.lib = load_library()
         p = default_params()
         p.field = FIELD_COMPLEX if field == "complex" else FIELD_REAL
@@ -191,6 +199,14 @@ class CoupledCluster:
             p.mixed_restart = {False: 0, True: 1, "auto": 2}[mixedRestart]
         if mixedRestartKeep is not None:
             p.mixed_restart_keep = int(mixedRestartKeep)
+        if mixedRefine is not None:
+            p.mixed_refine = int(mixedRefine)
+        if mixedSplit is not None:
+            if mixedSplit not in MIXED_SPLITS:
+                raise CcsdError(f"unknown split format {mixedSplit!r}")
+            p.mixed_split = MIXED_SPLITS[mixedSplit]
+        if mixedMaxK is not None:
+            p.mixed_max_k = int(mixedMaxK)
         self.params = p
         self.field = field
         self.No, self.Nv, self.NG = No, Nv, NG

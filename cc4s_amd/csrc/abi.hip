@@ -48,6 +48,7 @@ void fill(cc4s_ccsd_iteration *out, const IterationReport &r) {
   out->diis_placement = r.diis_placement;
   out->residuum_seconds = r.residuum_seconds;
   out->mixer_seconds = r.mixer_seconds;
+  out->pass_kind = r.kind;
 }
 
 }  // namespace
@@ -69,6 +70,9 @@ void cc4s_ccsd_default_params(cc4s_ccsd_params *p) {
   p->mixed_switch = 1e-5;
   p->device = 0;
   p->mixed_restart = 2;
+  p->mixed_refine = 0;
+  p->mixed_split = 0;
+  p->mixed_max_k = 16384;
 }
 
 int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
@@ -97,6 +101,14 @@ int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
     CC4S_ASSERT(params->mixed_restart_keep >= 0 && params->mixed_restart_keep < params->max_residua,
                 "mixed_restart_keep must lie in [0, maxResidua)");
     p.mixed_restart_keep = params->mixed_restart_keep;
+    CC4S_ASSERT(params->mixed_refine >= 0 && params->mixed_refine <= 2,
+                "mixed_refine must be 0, 1 or 2");
+    p.mixed_refine = params->mixed_refine;
+    CC4S_ASSERT(params->mixed_split == 0 || params->mixed_split == 1,
+                "mixed_split must be 0 (bf16) or 1 (half)");
+    p.mixed_half = params->mixed_split == 1;
+    CC4S_ASSERT(params->mixed_max_k >= 16, "mixed_max_k must be at least 16");
+    p.mixed_max_k = params->mixed_max_k;
     p.shape.field = params->field == CC4S_FIELD_REAL ? Field::Real : Field::Complex;
     p.shape.No = params->No;
     p.shape.Nv = params->Nv;

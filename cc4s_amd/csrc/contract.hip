@@ -2,6 +2,7 @@
 #include "contract.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <sstream>
@@ -760,7 +761,25 @@ struct GettPlan {
   const int64_t *rowA, *colA, *rowB, *colB, *rowC, *colC;
 };
 
-ContractionEngine::ContractionEngine(hipStream_t s) : stream_(s) {}
+ContractionEngine::ContractionEngine(hipStream_t s) : stream_(s) { apply_env_overrides(); }
+
+void ContractionEngine::apply_env_overrides() {
+  // CC4S_MIXED_SPLIT=half|bf16 and CC4S_MIXED_MAX_K override the split
+  // format and the longest FP32 sum of the mixed path
+  if (const char *e = std::getenv("CC4S_MIXED_SPLIT")) {
+    const std::string v = e;
+    CC4S_ASSERT(v == "half" || v == "bf16", "CC4S_MIXED_SPLIT must be half or bf16");
+    mixed_half = v == "half";
+  }
+  // CC4S_GETT_GENERATION selects the complex FP64 kernel (plans made later)
+  if (const char *e = std::getenv("CC4S_GETT_GENERATION"))
+    set_complex_gett_generation(std::atoi(e));
+  if (const char *e = std::getenv("CC4S_MIXED_MAX_K")) {
+    const long long v = std::atoll(e);
+    CC4S_ASSERT(v >= 16, "CC4S_MIXED_MAX_K must be at least 16");
+    mixed_max_k = v;
+  }
+}
 
 ContractionEngine::~ContractionEngine() {
   for (auto &p : pending_) {
@@ -993,20 +1012,44 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
   // an operand marked constant is packed once over the full K range
   const bool cacheA = cache_ptr_ && Aop.ptr == cache_ptr_;
   const bool cacheB = !cacheA && cache_ptr_ && Bop.ptr == cache_ptr_;
+  // half split: per-operand scales (A, B, cached operand) and reduction room
+  double *scA = nullptr, *scB = nullptr, *scC = nullptr, *scPart = nullptr;
+  if (mixed_half) {
+    scales_.reserve(4 * sizeof(double) + split_scale_partials_bytes());
+    scA = scales_.as<double>();
+    scB = scA + 1;
+    scC = scA + 2;
+    scPart = scA + 4;
+  }
   if (cacheA || cacheB) {
     const int64_t rows = cacheA ? Mp : Np;
     if (cache_plan_ != &P || cache_is_a_ != cacheA) {
       cache_.reserve(static_cast<size_t>(rows) * Kfull * 8);
-      if (cacheA)
+      if (cacheA) {
+        if (mixed_half)
+          launch_split_scale(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M,
+                             P.K, P.a_kc, scPart, scC, stream_);
         launch_pack(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K, 0,
-                    Kfull, Mp, Aop.conj, P.a_kc, cache_.get(), stream_);
-      else
+                    Kfull, Mp, Aop.conj, P.a_kc, cache_.get(), scC, stream_);
+      } else {
+        if (mixed_half)
+          launch_split_scale(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N,
+                             P.K, P.b_kc, scPart, scC, stream_);
         launch_pack(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K, 0,
-                    Kfull, Np, Bop.conj, P.b_kc, cache_.get(), stream_);
+                    Kfull, Np, Bop.conj, P.b_kc, cache_.get(), scC, stream_);
+      }
       cache_plan_ = &P;
       cache_is_a_ = cacheA;
     }
   }
+  if (mixed_half && !cacheA)
+    launch_split_scale(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K,
+                       P.a_kc, scPart, scA, stream_);
+  if (mixed_half && !cacheB)
+    launch_split_scale(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K,
+                       P.b_kc, scPart, scB, stream_);
+  const double *useScA = mixed_half ? (cacheA ? scC : scA) : nullptr;
+  const double *useScB = mixed_half ? (cacheB ? scC : scB) : nullptr;
   const int64_t packed_rows = (cacheA ? 0 : Mp) + (cacheB ? 0 : Np);
   int64_t Kc = packed_rows > 0
                    ? static_cast<int64_t>(pack_budget / (static_cast<size_t>(packed_rows) * 8))
@@ -1049,13 +1092,13 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
                               : Bpk;
     if (!cacheA)
       launch_pack(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K, k0,
-                  kc, Mp, Aop.conj, P.a_kc, Apk, stream_);
+                  kc, Mp, Aop.conj, P.a_kc, Apk, useScA, stream_);
     if (!cacheB)
       launch_pack(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K, k0,
-                  kc, Np, Bop.conj, P.b_kc, Bpk, stream_);
+                  kc, Np, Bop.conj, P.b_kc, Bpk, useScB, stream_);
     const double bc = c == 0 ? beta : 1.0;
     launch_bgemm(Ause, Buse, static_cast<double *>(C.ptr), W, P.rowC, P.colC, P.M,
-                 P.N, Mp, Np, kc, splits, alpha, bc, stream_);
+                 P.N, Mp, Np, kc, splits, alpha, bc, useScA, useScB, stream_);
     if (splits > 1)
       launch_splitk_reduce(Field::Complex, W, static_cast<double *>(C.ptr), P.rowC,
                            P.colC, P.M, P.N, splits, alpha, bc, stream_);

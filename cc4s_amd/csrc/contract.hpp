@@ -69,6 +69,12 @@ class ContractionEngine {
   // longest FP32 sum of the split-bf16 kernel (K is chunked beyond it and
   // the chunk results are added in FP64)
   int64_t mixed_max_k = 16384;
+  // split parts in IEEE half with per-operand power-of-two scaling (about
+  // 22 significant bits per real part) instead of bf16 (about 16)
+  bool mixed_half = false;
+  // CC4S_MIXED_SPLIT, CC4S_MIXED_MAX_K and CC4S_GETT_GENERATION, when set,
+  // replace the settings above and the complex kernel generation
+  void apply_env_overrides();
   // While set, the split-bf16 packing of an operand whose memory starts at
   // `ptr` is kept for the whole K range and reused by later contractions of
   // the same plan; the caller guarantees that the memory does not change
@@ -89,6 +95,7 @@ class ContractionEngine {
   bool mixed_ = false;
   DeviceBuffer pack_;
   DeviceBuffer cache_;
+  DeviceBuffer scales_;  // half split: [A, B, cached operand] scales, partials
   const void *cache_ptr_ = nullptr;
   const GettPlan *cache_plan_ = nullptr;
   bool cache_is_a_ = true;
@@ -142,14 +149,25 @@ struct MixedTile {
 };
 MixedTile mixed_tile();
 // packs k in [k0, k0+Kc) of an operand (FP64 complex, addressed by
-// offMN[mn] + offK[k]) into out[Kc][MNp] split-bf16 words, zero-padded
+// offMN[mn] + offK[k]) into out[Kc][MNp] split words, zero-padded: bf16
+// parts, or (scale != null) half parts of the operand times scale[0]
 void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
                  int64_t MN, int64_t K, int64_t k0, int64_t Kc, int64_t MNp,
-                 bool conj, bool kcontig, void *out, hipStream_t s);
-// C (or split-K partials W) from packed operands [Kc][Mp] and [Kc][Np]
+                 bool conj, bool kcontig, void *out, const double *scale,
+                 hipStream_t s);
+// scale[0] = the power of two that brings max(|Re|, |Im|) of the operand
+// into [2^14, 2^15) (the half split's range); partials holds
+// split_scale_partials_bytes()
+void launch_split_scale(const double *X, const int64_t *offMN, const int64_t *offK,
+                        int64_t MN, int64_t K, bool kcontig, double *partials,
+                        double *scale, hipStream_t s);
+size_t split_scale_partials_bytes();
+// C (or split-K partials W) from packed operands [Kc][Mp] and [Kc][Np];
+// scaleA/scaleB (both or neither): half parts, the sums are divided by both
 void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
                   const int64_t *rowC, const int64_t *colC, int64_t M, int64_t N,
                   int64_t Mp, int64_t Np, int64_t Kc, int splits, double alpha,
-                  double beta, hipStream_t s);
+                  double beta, const double *scaleA, const double *scaleB,
+                  hipStream_t s);
 
 }  // namespace cc4s_amd

@@ -293,7 +293,10 @@ CoupledCluster::CoupledCluster(const CoupledClusterParams &p, int device)
               "maxResidua must be in [1, 16]");
   CC4S_ASSERT(!p_.mixed || s.field == Field::Complex,
               "mixed precision needs the complex field");
-  mixed_phase_ = p_.mixed;
+  phase_ = p_.mixed ? Phase::Mixed : Phase::Fp64;
+  engine_->mixed_half = p_.mixed_half;
+  engine_->mixed_max_k = p_.mixed_max_k;
+  engine_->apply_env_overrides();
 }
 
 // The DIIS history is the largest allocation; it is placed once the vertex,
@@ -311,6 +314,8 @@ void CoupledCluster::create_mixer() {
   if (p_.mixed) {
     const int64_t rows = (s.No * s.No + 127) / 128 * 128;
     reserve += static_cast<size_t>(s.Nv) * s.Nv * rows * 8;
+    // the defect of the split residuum (one full set)
+    if (p_.mixed_refine) reserve += s.nset() * scalar_bytes(s.field);
   }
   const size_t avail = free_b > reserve ? free_b - reserve : 0;
   const char *env = std::getenv("CC4S_DIIS_PLACEMENT");
@@ -409,6 +414,16 @@ void CoupledCluster::energy(const void *T, cplx *E, cplx *D, cplx *X) {
   *E = *D + *X;
 }
 
+bool CoupledCluster::split_stalled(double norm) {
+  if (best_mixed_norm_ == 0 || norm * p_.mixed_stall_gain < best_mixed_norm_) {
+    best_mixed_norm_ = norm;
+    stalled_passes_ = 0;
+  } else {
+    ++stalled_passes_;
+  }
+  return stalled_passes_ >= p_.mixed_stall_passes;
+}
+
 IterationReport CoupledCluster::iterate() {
   CC4S_ASSERT(inputs_set_, "inputs not set");
   CC4S_HIP(hipSetDevice(device_));
@@ -418,10 +433,29 @@ IterationReport CoupledCluster::iterate() {
   IterationReport rep;
   rep.iteration = iterations_done + 1;
   const void *T = have_amplitudes_ ? T_.get() : nullptr;
-  rep.mixed = mixed_phase_;
-  engine_->set_mixed(mixed_phase_);
-  ccsd_->get_residuum(T, R_.get());                     // :136
-  engine_->set_mixed(false);
+  const int64_t ndoubles = s.nset() * (s.field == Field::Complex ? 2 : 1);
+  if (phase_ == Phase::Anchor && T) {
+    // the split residuum and the FP64 residuum at the same amplitudes:
+    // d = R64(T) - Rsplit(T); the pass itself continues with R64 and a DIIS
+    // history that holds no split residua
+    defect_.reserve(s.nset() * scalar_bytes(s.field));
+    engine_->set_mixed(true);
+    ccsd_->get_residuum(T, defect_.get());
+    engine_->set_mixed(false);
+    ccsd_->get_residuum(T, R_.get());                   // :136
+    axpby(stream_, ndoubles, 1.0, R_.get(), -1.0, defect_.get());
+    mixer_->restart(0);
+    rep.mixed = false;
+    rep.kind = 2;
+  } else {
+    rep.mixed = phase_ == Phase::Mixed || phase_ == Phase::Corrected;
+    rep.kind = phase_ == Phase::Mixed ? 1 : phase_ == Phase::Corrected ? 3 : 0;
+    engine_->set_mixed(rep.mixed);
+    ccsd_->get_residuum(T, R_.get());                   // :136
+    engine_->set_mixed(false);
+    if (phase_ == Phase::Corrected)
+      axpby(stream_, ndoubles, 1.0, defect_.get(), 1.0, R_.get());
+  }
   CC4S_HIP(hipStreamSynchronize(stream_));
   const auto t1 = std::chrono::steady_clock::now();
   // :137-145: T' = -R/(Delta+s) and dT = T' - T, written into the mixer slot
@@ -446,22 +480,38 @@ IterationReport CoupledCluster::iterate() {
                   rep.residuum_norm < p_.amplitudes_convergence;  // :167-173
   // the convergence test only counts FP64 passes; the mixed phase ends once
   // the extrapolated amplitude change falls below the switch threshold, or
-  // stops falling (the rounding floor of the split-bf16 residuum)
+  // stops falling (the rounding floor of the split residuum)
   if (rep.mixed) rep.converged = false;
-  if (mixed_phase_ && !is_second_order) {
-    if (best_mixed_norm_ == 0 || rep.residuum_norm * p_.mixed_stall_gain < best_mixed_norm_) {
+  switch (phase_) {
+    case Phase::Fp64:
+      break;
+    case Phase::Mixed:
+      if (!is_second_order) {
+        const bool stalled = split_stalled(rep.residuum_norm);
+        if (rep.residuum_norm < p_.mixed_switch || stalled) {
+          if (p_.mixed_refine) {
+            phase_ = Phase::Anchor;
+          } else {
+            phase_ = Phase::Fp64;
+            if (p_.mixed_restart == 1 ||
+                (p_.mixed_restart == 2 && (stalled || rep.residuum_norm < 2e-6)))
+              mixer_->restart(p_.mixed_restart_keep);
+          }
+        }
+      }
+      break;
+    case Phase::Anchor:
+      phase_ = Phase::Corrected;
       best_mixed_norm_ = rep.residuum_norm;
       stalled_passes_ = 0;
-    } else {
-      ++stalled_passes_;
-    }
-    const bool stalled = stalled_passes_ >= p_.mixed_stall_passes;
-    if (rep.residuum_norm < p_.mixed_switch || stalled) {
-      mixed_phase_ = false;
-      if (p_.mixed_restart == 1 ||
-          (p_.mixed_restart == 2 && (stalled || rep.residuum_norm < 2e-6)))
-        mixer_->restart(p_.mixed_restart_keep);
-    }
+      break;
+    case Phase::Corrected:
+      if (split_stalled(rep.residuum_norm) ||
+          rep.residuum_norm < 0.1 * p_.amplitudes_convergence) {
+        phase_ = Phase::Fp64;
+        if (p_.mixed_refine == 2) mixer_->restart(0);
+      }
+      break;
   }
   previous_e_ = rep.energy;
   iterations_done += 1;

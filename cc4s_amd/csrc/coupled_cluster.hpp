@@ -35,6 +35,14 @@ struct CoupledClusterParams {
   // dR < 1e-6 it saves them)
   int mixed_restart = 2;
   int mixed_restart_keep = 0;  // most recent pairs kept by a restart
+  // what follows the mixed phase: 0 FP64 passes; 1 or 2 an FP64 anchor pass
+  // that also evaluates the split residuum at the same amplitudes, giving
+  // its defect d = R64(T) - Rsplit(T) (the DIIS history restarts there), then
+  // split passes with Rsplit + d until they stop gaining, then FP64 passes
+  // (2 restarts the DIIS history once more before them, 1 keeps it)
+  int mixed_refine = 0;
+  bool mixed_half = false;        // split parts in power-of-two scaled IEEE half
+  int64_t mixed_max_k = 16384;    // longest FP32 sum of a split contraction
 };
 
 // Where the DIIS history lives: both kinds of slot in HBM; the amplitude
@@ -92,6 +100,9 @@ struct IterationReport {
   double delta_energy = 0, residuum_norm = 0, seconds = 0, flops = 0;
   bool converged = false;
   bool mixed = false;  // pass ran its large contractions in split bf16
+  // 0 FP64, 1 split, 2 FP64 anchor (also evaluates the split residuum),
+  // 3 split corrected by the anchor's defect
+  int kind = 0;
   int diis_placement = 0;  // DiisPlacement of the mixer
   double residuum_seconds = 0, mixer_seconds = 0;
 };
@@ -152,9 +163,14 @@ class CoupledCluster {
   DeviceBuffer adj_gamma_, adj_hp_, adj_ph_, adj_a_, adj_b_, adj_partials_, adj_S_;
   bool have_amplitudes_ = false;
   bool inputs_set_ = false;
-  bool mixed_phase_ = false;  // still iterating in mixed precision
+  enum class Phase { Fp64, Mixed, Anchor, Corrected };
+  Phase phase_ = Phase::Fp64;
+  DeviceBuffer defect_;       // R64(T) - Rsplit(T) at the anchor pass
   double best_mixed_norm_ = 0;
   int stalled_passes_ = 0;
+  // split passes end after `mixed_stall_passes` passes without a
+  // `mixed_stall_gain`-fold drop of dR below their best value
+  bool split_stalled(double norm);
   cplx previous_e_ = 0.0;
 };
 

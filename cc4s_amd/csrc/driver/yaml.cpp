@@ -191,9 +191,21 @@ class Parser {
     return std::string::npos;
   }
 
+  // the value of "key:" on the following, deeper lines: a block sequence, a
+  // block map, or one line holding a flow collection or a scalar (the
+  // reference's operation files write "out:" and then "{}")
   NodePtr block(int indent) {
     if (pos_ >= lines_.size()) return Node::map();
-    return is_item(lines_[pos_].text) ? sequence(indent) : mapping(indent);
+    const Line &l = lines_[pos_];
+    if (is_item(l.text)) return sequence(indent);
+    if (l.text[0] == '{' || l.text[0] == '[' || key_colon(l.text) == std::string::npos) {
+      ++pos_;
+      NodePtr v = flow_or_scalar(l.text, l.number);
+      if (pos_ < lines_.size() && lines_[pos_].indent >= indent)
+        throw err(lines_[pos_].number, "unexpected indentation");
+      return v;
+    }
+    return mapping(indent);
   }
 
   NodePtr sequence(int indent) {

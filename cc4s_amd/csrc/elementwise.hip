@@ -719,4 +719,21 @@ void copy(hipStream_t s, void *y, const void *x, size_t bytes) {
   CC4S_HIP(hipMemcpyAsync(y, x, bytes, hipMemcpyDeviceToDevice, s));
 }
 
+namespace {
+__global__ __launch_bounds__(kBlock) void axpby_kernel(int64_t n, double a,
+                                                       const double *__restrict__ x,
+                                                       double b, double *__restrict__ y) {
+  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * kBlock)
+    y[e] = a * x[e] + b * y[e];
+}
+}  // namespace
+
+void axpby(hipStream_t s, int64_t n, double a, const void *x, double b, void *y) {
+  if (n <= 0) return;
+  axpby_kernel<<<grid_for(n), kBlock, 0, s>>>(n, a, static_cast<const double *>(x), b,
+                                              static_cast<double *>(y));
+  CC4S_HIP(hipGetLastError());
+}
+
 }  // namespace cc4s_amd

@@ -90,5 +90,7 @@ bool is_hermitian(hipStream_t s, const void *gamma, int64_t NG, int64_t Np);
 
 // y = x (device copy on the stream)
 void copy(hipStream_t s, void *y, const void *x, size_t bytes);
+// y = a x + b y over n doubles
+void axpby(hipStream_t s, int64_t n, double a, const void *x, double b, void *y);
 
 }  // namespace cc4s_amd

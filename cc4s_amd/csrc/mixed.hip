@@ -15,6 +15,14 @@
 // lanes 0-31 carry k, lanes 32-63 k+1).  Results are written to C in FP64
 // (alpha * acc + beta * C) or to FP64 split-K partials.  K is processed in
 // chunks, so FP32 sums never run over more than one chunk.
+//
+// FP16 variant: the same split with IEEE half parts (11 significant bits
+// each, about 22 per real part).  Half has a narrow exponent range, so each
+// operand is first multiplied by a power of two that brings its largest
+// magnitude just below 2^15 (a max-|x| reduction over the operand), and the
+// kernel multiplies the FP32 sums by the inverse of both scales; every
+// scaling is exact.  Parts below 2^-24 of the scaled range flush towards
+// zero, which is far below the kept precision of the large elements.
 #include "contract.hpp"
 
 #include <algorithm>
@@ -24,6 +32,7 @@ namespace cc4s_amd {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
@@ -45,7 +54,68 @@ __device__ __forceinline__ unsigned int split_part(double x) {
   return h | (l << 16);
 }
 
+// (hi | lo << 16) of one real part in IEEE half, x already scaled
+__device__ __forceinline__ unsigned int split_part_f16(double x) {
+  const _Float16 h = static_cast<_Float16>(static_cast<float>(x));
+  const double r = x - static_cast<double>(static_cast<float>(h));
+  const _Float16 l = static_cast<_Float16>(static_cast<float>(r));
+  return static_cast<unsigned int>(__builtin_bit_cast(unsigned short, h)) |
+         (static_cast<unsigned int>(__builtin_bit_cast(unsigned short, l)) << 16);
+}
+
 constexpr int kPackTile = 64;
+constexpr int kMaxBlocks = 1024;
+
+// per-block max(|re|, |im|) over the operand elements offMN[mn] + offK[k]:
+// each wave walks whole lines of the contiguous index (k if kcontig, else
+// mn) with its lanes, so the reads are coalesced and need no division
+__global__ __launch_bounds__(256) void maxabs_kernel(const double2 *X, const int64_t *offMN,
+                                                     const int64_t *offK, int64_t MN,
+                                                     int64_t K, bool kcontig,
+                                                     double *partials) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = blockIdx.x * 4ll + (threadIdx.x >> 6);
+  const int64_t waves = gridDim.x * 4ll;
+  const int64_t lines = kcontig ? MN : K, len = kcontig ? K : MN;
+  const int64_t *lineOff = kcontig ? offMN : offK;
+  const int64_t *inOff = kcontig ? offK : offMN;
+  for (int64_t l = wave; l < lines; l += waves) {
+    const int64_t base = lineOff[l];
+    for (int64_t i = lane; i < len; i += 64) {
+      const double2 x = X[base + inOff[i]];
+      m = fmax(m, fmax(fabs(x.x), fabs(x.y)));
+    }
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// scale[0] = 2^(15 - e) for max = f 2^e, f in [0.5, 1): the scaled maximum
+// lies in [2^14, 2^15); an all-zero operand gets scale 1
+__global__ __launch_bounds__(256) void scale_kernel(const double *partials, int nblocks,
+                                                    double *scale) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += 256) m = fmax(m, partials[b]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int e = 0;
+    (void)frexp(sh[0], &e);
+    scale[0] = sh[0] > 0.0 ? ldexp(1.0, 15 - e) : 1.0;
+  }
+}
 
 struct PackArgs {
   const double2 *X;
@@ -58,12 +128,15 @@ struct PackArgs {
   double sign;           // -1 conjugates
   bool kcontig;          // source contiguous along k
   uint2 *out;            // [Kc][MNp]
+  const double *scale;   // FP16: the operand's power-of-two scale
 };
 
 // packs a 64(k) x 64(mn) tile: reads in the source's contiguous order,
 // transposes through LDS, writes mn-fastest
+template <bool FP16>
 __global__ __launch_bounds__(256) void pack_kernel(PackArgs p) {
   __shared__ uint2 tile[kPackTile][kPackTile + 1];
+  const double sc = FP16 ? p.scale[0] : 1.0;
   const int64_t mnb = static_cast<int64_t>(blockIdx.x) * kPackTile;
   const int64_t kb = static_cast<int64_t>(blockIdx.y) * kPackTile;
   const int t = threadIdx.x;
@@ -74,8 +147,13 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs p) {
     uint2 v = make_uint2(0u, 0u);
     if (kb + kk < p.Kc && k < p.K && mn < p.MN) {
       const double2 x = p.X[p.offMN[mn] + p.offK[k]];
-      v.x = split_part(x.x);
-      v.y = split_part(p.sign * x.y);
+      if constexpr (FP16) {
+        v.x = split_part_f16(sc * x.x);
+        v.y = split_part_f16(sc * p.sign * x.y);
+      } else {
+        v.x = split_part(x.x);
+        v.y = split_part(p.sign * x.y);
+      }
     }
     tile[kk][mm] = v;
   }
@@ -96,31 +174,44 @@ struct BgArgs {
   int64_t M, N, Mp, Np, Kc, kChunk;
   double alpha, beta;
   int tilesM, tilesN;
+  const double *scaleA, *scaleB;  // FP16: the operands' scales
 };
 
-__device__ __forceinline__ bf16x8 expand_a(uint2 c) {
+// the eight 16-bit parts of one lane's operand (bf16 or half bit patterns);
+// the sign flip is bit 15 of each part in both formats
+__device__ __forceinline__ u32x4 expand_a(uint2 c) {
   u32x4 w;
   w[0] = (c.x & 0xffffu) * 0x10001u;
   w[1] = (c.x >> 16) * 0x10001u;
   w[2] = (c.y & 0xffffu) * 0x10001u;
   w[3] = (c.y >> 16) * 0x10001u;
-  return __builtin_bit_cast(bf16x8, w);
+  return w;
 }
-__device__ __forceinline__ bf16x8 expand_b_re(uint2 c) {
+__device__ __forceinline__ u32x4 expand_b_re(uint2 c) {
   const unsigned int ni = c.y ^ 0x80008000u;
   u32x4 w = {c.x, c.x, ni, ni};
-  return __builtin_bit_cast(bf16x8, w);
+  return w;
 }
-__device__ __forceinline__ bf16x8 expand_b_im(uint2 c) {
+__device__ __forceinline__ u32x4 expand_b_im(uint2 c) {
   u32x4 w = {c.y, c.y, c.x, c.x};
-  return __builtin_bit_cast(bf16x8, w);
+  return w;
+}
+
+template <bool FP16>
+__device__ __forceinline__ f32x16 mfma_split(u32x4 a, u32x4 b, f32x16 c) {
+  if constexpr (FP16)
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
+                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
 constexpr int kBThreads = 512;
 
 // 8 waves in a 2 x 4 grid over a BM x BN complex tile; each wave owns
 // WM x WN = (WM/32) x (WN/32) blocks of 32x32, re and im accumulators.
-template <int BM, int BN, int BK, int WM, int WN>
+template <int BM, int BN, int BK, int WM, int WN, bool FP16>
 __global__ __launch_bounds__(kBThreads, 1) void bgemm_kernel(BgArgs p) {
   constexpr int MB = WM / 32, NB = WN / 32;
   constexpr int WAVES_N = BN / WN;
@@ -204,20 +295,20 @@ __global__ __launch_bounds__(kBThreads, 1) void bgemm_kernel(BgArgs p) {
       const int kk = 2 * kp + (lane >> 5);
       const int ra0 = wm * WM + (lane & 31), rb0 = wn * WN + (lane & 31);
       // named registers (not arrays), so no fragment is ever placed in LDS
-      const bf16x8 a0 = expand_a(As[buf][kk][ra0]);
-      const bf16x8 a1 = expand_a(As[buf][kk][ra0 + 32]);
+      const u32x4 a0 = expand_a(As[buf][kk][ra0]);
+      const u32x4 a1 = expand_a(As[buf][kk][ra0 + 32]);
       const uint2 c0 = Bs[buf][kk][rb0];
       const uint2 c1 = Bs[buf][kk][rb0 + 32];
-      const bf16x8 br0 = expand_b_re(c0), bi0 = expand_b_im(c0);
-      const bf16x8 br1 = expand_b_re(c1), bi1 = expand_b_im(c1);
-      accR[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, br0, accR[0][0], 0, 0, 0);
-      accI[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bi0, accI[0][0], 0, 0, 0);
-      accR[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, br1, accR[0][1], 0, 0, 0);
-      accI[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bi1, accI[0][1], 0, 0, 0);
-      accR[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, br0, accR[1][0], 0, 0, 0);
-      accI[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bi0, accI[1][0], 0, 0, 0);
-      accR[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, br1, accR[1][1], 0, 0, 0);
-      accI[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bi1, accI[1][1], 0, 0, 0);
+      const u32x4 br0 = expand_b_re(c0), bi0 = expand_b_im(c0);
+      const u32x4 br1 = expand_b_re(c1), bi1 = expand_b_im(c1);
+      accR[0][0] = mfma_split<FP16>(a0, br0, accR[0][0]);
+      accI[0][0] = mfma_split<FP16>(a0, bi0, accI[0][0]);
+      accR[0][1] = mfma_split<FP16>(a0, br1, accR[0][1]);
+      accI[0][1] = mfma_split<FP16>(a0, bi1, accI[0][1]);
+      accR[1][0] = mfma_split<FP16>(a1, br0, accR[1][0]);
+      accI[1][0] = mfma_split<FP16>(a1, bi0, accI[1][0]);
+      accR[1][1] = mfma_split<FP16>(a1, br1, accR[1][1]);
+      accI[1][1] = mfma_split<FP16>(a1, bi1, accI[1][1]);
     }
     if (s + 1 < nstages) {
       store(buf ^ 1);
@@ -227,6 +318,7 @@ __global__ __launch_bounds__(kBThreads, 1) void bgemm_kernel(BgArgs p) {
   }
 
   // lane l, register r of a 32x32 block: row 8*(r/4) + 4*(l/32) + r%4, column l%32
+  const double unscale = FP16 ? 1.0 / (p.scaleA[0] * p.scaleB[0]) : 1.0;
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int64_t n = n0 + wn * WN + j * 32 + (lane & 31);
@@ -238,8 +330,8 @@ __global__ __launch_bounds__(kBThreads, 1) void bgemm_kernel(BgArgs p) {
       for (int r = 0; r < 16; ++r) {
         const int64_t m = m0 + wm * WM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
         if (m >= p.M) continue;
-        const double vr = static_cast<double>(accR[i][j][r]);
-        const double vi = static_cast<double>(accI[i][j][r]);
+        const double vr = unscale * static_cast<double>(accR[i][j][r]);
+        const double vi = unscale * static_cast<double>(accI[i][j][r]);
         if (p.W) {
           const int64_t o = (static_cast<int64_t>(blockIdx.y) * p.M + m) * p.N + n;
           reinterpret_cast<double2 *>(p.W)[o] = make_double2(vr, vi);
@@ -264,9 +356,25 @@ constexpr int kXBM = 128, kXBN = 256, kXBK = 16, kXWM = 64, kXWN = 64;
 
 MixedTile mixed_tile() { return MixedTile{kXBM, kXBN, kXBK}; }
 
+void launch_split_scale(const double *X, const int64_t *offMN, const int64_t *offK,
+                        int64_t MN, int64_t K, bool kcontig, double *partials,
+                        double *scale, hipStream_t s) {
+  const int64_t n = MN * K;
+  const int blocks = static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>(kMaxBlocks, (n + 255) / 256)));
+  maxabs_kernel<<<blocks, 256, 0, s>>>(reinterpret_cast<const double2 *>(X), offMN, offK,
+                                       MN, K, kcontig, partials);
+  CC4S_HIP(hipGetLastError());
+  scale_kernel<<<1, 256, 0, s>>>(partials, blocks, scale);
+  CC4S_HIP(hipGetLastError());
+}
+
+size_t split_scale_partials_bytes() { return kMaxBlocks * sizeof(double); }
+
 void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
                  int64_t MN, int64_t K, int64_t k0, int64_t Kc, int64_t MNp,
-                 bool conj, bool kcontig, void *out, hipStream_t s) {
+                 bool conj, bool kcontig, void *out, const double *scale,
+                 hipStream_t s) {
   PackArgs a;
   a.X = reinterpret_cast<const double2 *>(X);
   a.offMN = offMN;
@@ -279,17 +387,26 @@ void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
   a.sign = conj ? -1.0 : 1.0;
   a.kcontig = kcontig;
   a.out = static_cast<uint2 *>(out);
+  a.scale = scale;
   dim3 grid(static_cast<unsigned>((MNp + kPackTile - 1) / kPackTile),
             static_cast<unsigned>((Kc + kPackTile - 1) / kPackTile));
-  pack_kernel<<<grid, 256, 0, s>>>(a);
+  if (scale)
+    pack_kernel<true><<<grid, 256, 0, s>>>(a);
+  else
+    pack_kernel<false><<<grid, 256, 0, s>>>(a);
   CC4S_HIP(hipGetLastError());
 }
 
 void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
                   const int64_t *rowC, const int64_t *colC, int64_t M, int64_t N,
                   int64_t Mp, int64_t Np, int64_t Kc, int splits, double alpha,
-                  double beta, hipStream_t s) {
+                  double beta, const double *scaleA, const double *scaleB,
+                  hipStream_t s) {
+  CC4S_ASSERT((scaleA == nullptr) == (scaleB == nullptr),
+              "split-half operands need both scales");
   BgArgs a;
+  a.scaleA = scaleA;
+  a.scaleB = scaleB;
   a.A = static_cast<const uint2 *>(Apack);
   a.B = static_cast<const uint2 *>(Bpack);
   a.C = C;
@@ -304,7 +421,10 @@ void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
   a.tilesM = static_cast<int>(Mp / kXBM);
   a.tilesN = static_cast<int>(Np / kXBN);
   dim3 grid(a.tilesM * a.tilesN, splits);
-  bgemm_kernel<kXBM, kXBN, kXBK, kXWM, kXWN><<<grid, kBThreads, 0, s>>>(a);
+  if (scaleA)
+    bgemm_kernel<kXBM, kXBN, kXBK, kXWM, kXWN, true><<<grid, kBThreads, 0, s>>>(a);
+  else
+    bgemm_kernel<kXBM, kXBN, kXBK, kXWM, kXWN, false><<<grid, kBThreads, 0, s>>>(a);
   CC4S_HIP(hipGetLastError());
 }
 

@@ -81,6 +81,16 @@ typedef struct {
   int mixed_restart;              /* MIXED: DIIS history at the switch: 0 keep, 1 restart,
                                      2 restart when dR < 2e-6 or stalled (2) */
   int mixed_restart_keep;         /* MIXED: most recent pairs a restart keeps (0) */
+  int mixed_refine;               /* MIXED: after the split passes: 0 FP64 passes; 1, 2 an
+                                     FP64 anchor pass that also evaluates the split residuum
+                                     at the same amplitudes (defect d = R64 - Rsplit, DIIS
+                                     restarted), split passes with Rsplit + d until they stop
+                                     gaining, then FP64 passes (2: DIIS restarted before
+                                     them, 1: kept) */
+  int mixed_split;                /* MIXED: split parts 0 bf16, 1 IEEE half with a
+                                     power-of-two scale per operand */
+  int64_t mixed_max_k;            /* MIXED: longest FP32 sum of a split contraction; longer
+                                     K is cut into chunks added in FP64 (16384) */
 } cc4s_ccsd_params;
 
 /* One pass of the iteration loop (CoupledCluster.cxx:129-175). */
@@ -98,6 +108,8 @@ typedef struct {
                               host memory, 2 every slot in pinned host memory */
   double residuum_seconds; /* wall time of the residuum */
   double mixer_seconds;    /* wall time of the amplitude update and DIIS step */
+  int pass_kind;           /* 0 FP64, 1 split, 2 FP64 anchor (also evaluates the
+                              split residuum), 3 split corrected by the anchor's defect */
 } cc4s_ccsd_iteration;
 
 /* Result map of CoupledCluster::run (CoupledCluster.cxx:184-191,245-248). */

@@ -152,6 +152,28 @@ def test_tensor_round_trip_text_and_binary(built, tmp_path):
     assert np.array_equal(vb, inp.gamma) and np.array_equal(vt, inp.gamma)
 
 
+def test_flow_value_on_the_next_line(built, tmp_path):
+    """`out:` followed by an indented `{}` (the layout of the reference's own
+    operation files, e.g. test/tests/ueg/rs1.0-7occ-26virt/cc4s.in:69-70)."""
+    inp = syn.generate(2, 5, 3, "complex", 9)
+    syn.write_cc4s_inputs(inp, tmp_path)
+    r = run_driver(tmp_path, """
+- name: Read
+  in:
+    fileName: "CoulombVertex.yaml"
+  out:
+    destination: V
+- name: Write
+  in:
+    source: V
+    fileName: "Vt.yaml"
+  out:
+    {}
+""", "--dry-run")
+    assert r.returncode == 0, r.stderr
+    assert yaml.safe_load((tmp_path / "Vt.yaml").read_text())["scalarType"] == "Complex64"
+
+
 @pytest.mark.parametrize("case,message", [
     ("unknown", "unknown algorithm: FiniteSizeCorrection"),
     ("fermi_high", "Fermi energy above all eigen energies."),

@@ -36,6 +36,25 @@ def test_mixed_contract_accuracy(case):
         assert rel_err(out, ref) < 5e-5, (case, alpha, beta, rel_err(out, ref))
 
 
+@pytest.mark.parametrize("case", CASES, ids=["%s,%s->%s" % c for c in CASES])
+def test_half_split_contract_accuracy(case, monkeypatch):
+    """Half-precision split parts with power-of-two operand scaling: about
+    22 significant bits per part, the FP32 sums bound the error; operands
+    spanning many decades (scaled by 1e-6) keep it."""
+    monkeypatch.setenv("CC4S_MIXED_SPLIT", "half")
+    ia, ib, ic = case
+    rng = np.random.default_rng(7)
+    mk = lambda idx: (rng.standard_normal(tuple(LENS[c] for c in idx))
+                      + 1j * rng.standard_normal(tuple(LENS[c] for c in idx)))
+    A, B, C = mk(ia), mk(ib), mk(ic)
+    A *= 1e-6 * np.exp(rng.uniform(-8, 0, A.shape))
+    for alpha, beta in ((1.0, 0.0), (-0.5, 1.0)):
+        ref = alpha * np.einsum(f"{ia},{ib}->{ic}", A, B) + beta * C
+        out = api.contract(alpha, A, ia, B, ib, beta, C, ic,
+                           precision=api.PRECISION_MIXED)
+        assert rel_err(out, ref) < 2e-6, (case, alpha, beta, rel_err(out, ref))
+
+
 def test_mixed_contract_conjugation_and_large_k():
     rng = np.random.default_rng(3)
     M, N, K = 300, 200, 9000
@@ -53,8 +72,12 @@ def test_mixed_rejects_real_field():
                      np.zeros((3, 2)), "mn", precision=api.PRECISION_MIXED)
 
 
-def test_mixed_residuum_accuracy():
-    No, Nv, NG = 8, 40, 200
+@pytest.mark.parametrize("split,bound", [("bf16", 1e-4), ("half", 1e-5)])
+def test_mixed_residuum_accuracy(split, bound, monkeypatch):
+    """At No=12, Nv=60, NG=300 the ring, Gamma and ladder contractions are
+    above the mixed threshold (1e9 flops) and run split."""
+    monkeypatch.setenv("CC4S_MIXED_SPLIT", split)
+    No, Nv, NG = 12, 60, 300
     inp = syn.generate(No, Nv, NG, "complex", 5)
     T = syn.random_amplitudes(No, Nv, "complex", 9, t1_scale=0.02, t2_scale=0.02)
     out = {}
@@ -62,8 +85,10 @@ def test_mixed_residuum_accuracy():
         with api.CoupledCluster(No, Nv, NG, precision=prec) as cc:
             cc.set_inputs(inp.gamma, inp.eps)
             out[prec] = cc.residuum(T)
-    assert rel_err(out[1][1], out[0][1]) < 1e-4
-    assert rel_err(out[1][0], out[0][0]) < 1e-4
+    e1, e2 = rel_err(out[1][0], out[0][0]), rel_err(out[1][1], out[0][1])
+    print(f"split {split}: residuum rel. error R1 {e1:.2e} R2 {e2:.2e}", flush=True)
+    assert e2 < bound and e1 < bound
+    assert e2 > 0.0  # the mixed path ran
 
 
 @pytest.mark.parametrize("switch", [1e-5, 1e-7, 1e-14],
@@ -121,3 +146,33 @@ def test_restart_keeping_recent_pairs(placement, monkeypatch):
         mixed = cc.run()
     assert mixed["convergenceReached"]
     assert abs(mixed["energy"]["correlation"] - fp64["energy"]["correlation"]) < 1e-8
+
+
+@pytest.mark.parametrize("refine", [1, 2])
+def test_defect_corrected_refinement(refine):
+    """After the split passes, one FP64 anchor pass measures the defect of
+    the split residuum; split passes corrected by it continue, then FP64
+    passes finish.  The run reaches the FP64 energy, the passes come in that
+    order, and only FP64 passes (anchor or plain) declare convergence."""
+    No, Nv, NG = 12, 60, 300
+    print(f"defect-corrected refinement, mode {refine}", flush=True)
+    inp = syn.generate(No, Nv, NG, "complex", 42, scale_rule="mp2_per_occupied")
+    kw = dict(maxIterations=60, energyConvergence=1e-10, amplitudesConvergence=1e-8,
+              maxResidua=8)
+    with api.CoupledCluster(No, Nv, NG, **kw) as cc:
+        cc.set_inputs(inp.gamma, inp.eps)
+        fp64 = cc.run()
+    with api.CoupledCluster(No, Nv, NG, precision=api.PRECISION_MIXED, mixedSwitch=1e-6,
+                            mixedRefine=refine, mixedSplit="half", mixedMaxK=2048,
+                            **kw) as cc:
+        cc.set_inputs(inp.gamma, inp.eps)
+        mixed = cc.run()
+    kinds = "".join("FSAC"[t["pass_kind"]] for t in mixed["trace"])
+    for t in mixed["trace"]:
+        print(f"{t['iteration']:3d} {'FSAC'[t['pass_kind']]} E={t['energy']:.13f} "
+              f"dR={t['residuum_norm']:.3e}", flush=True)
+    assert mixed["convergenceReached"]
+    assert abs(mixed["energy"]["correlation"] - fp64["energy"]["correlation"]) < 1e-8
+    import re
+    assert re.fullmatch(r"S+AC*F*", kinds), kinds
+    assert all(t["pass_kind"] in (0, 2) for t in mixed["trace"] if t["converged"])
