@@ -70,9 +70,9 @@ void cc4s_ccsd_default_params(cc4s_ccsd_params *p) {
   p->mixed_switch = 1e-5;
   p->device = 0;
   p->mixed_restart = 2;
-  p->mixed_refine = 0;
-  p->mixed_split = 0;
-  p->mixed_max_k = 16384;
+  p->mixed_refine = 1;
+  p->mixed_split = 1;
+  p->mixed_max_k = 2048;
 }
 
 int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {

@@ -505,13 +505,18 @@ IterationReport CoupledCluster::iterate() {
       best_mixed_norm_ = rep.residuum_norm;
       stalled_passes_ = 0;
       break;
-    case Phase::Corrected:
-      if (split_stalled(rep.residuum_norm) ||
-          rep.residuum_norm < 0.1 * p_.amplitudes_convergence) {
+    case Phase::Corrected: {
+      // FP64 passes take over once a corrected pass meets the convergence
+      // test (an FP64 pass then has to confirm it) or the passes stall at
+      // the noise of the split residuum
+      const bool meets = std::abs(rep.energy - previous_e_) < p_.energy_convergence &&
+                         rep.residuum_norm < p_.amplitudes_convergence;
+      if (split_stalled(rep.residuum_norm) || meets) {
         phase_ = Phase::Fp64;
         if (p_.mixed_refine == 2) mixer_->restart(0);
       }
       break;
+    }
   }
   previous_e_ = rep.energy;
   iterations_done += 1;

@@ -40,9 +40,9 @@ struct CoupledClusterParams {
   // its defect d = R64(T) - Rsplit(T) (the DIIS history restarts there), then
   // split passes with Rsplit + d until they stop gaining, then FP64 passes
   // (2 restarts the DIIS history once more before them, 1 keeps it)
-  int mixed_refine = 0;
-  bool mixed_half = false;        // split parts in power-of-two scaled IEEE half
-  int64_t mixed_max_k = 16384;    // longest FP32 sum of a split contraction
+  int mixed_refine = 1;
+  bool mixed_half = true;         // split parts in power-of-two scaled IEEE half
+  int64_t mixed_max_k = 2048;     // longest FP32 sum of a split contraction
 };
 
 // Where the DIIS history lives: both kinds of slot in HBM; the amplitude

@@ -66,9 +66,12 @@ __device__ __forceinline__ unsigned int split_part_f16(double x) {
 constexpr int kPackTile = 64;
 constexpr int kMaxBlocks = 1024;
 
-// per-block max(|re|, |im|) over the operand elements offMN[mn] + offK[k]:
-// each wave walks whole lines of the contiguous index (k if kcontig, else
-// mn) with its lanes, so the reads are coalesced and need no division
+// per-block max(|re|, |im|) over the operand elements offMN[mn] + offK[k].
+// Work items are segments of 1024 consecutive elements of one line of the
+// contiguous index (k if kcontig, else mn); a wave takes one item, its lanes
+// read 16 elements each with coalesced loads.
+constexpr int kSegment = 1024;
+
 __global__ __launch_bounds__(256) void maxabs_kernel(const double2 *X, const int64_t *offMN,
                                                      const int64_t *offK, int64_t MN,
                                                      int64_t K, bool kcontig,
@@ -81,9 +84,14 @@ __global__ __launch_bounds__(256) void maxabs_kernel(const double2 *X, const int
   const int64_t lines = kcontig ? MN : K, len = kcontig ? K : MN;
   const int64_t *lineOff = kcontig ? offMN : offK;
   const int64_t *inOff = kcontig ? offK : offMN;
-  for (int64_t l = wave; l < lines; l += waves) {
+  const int64_t nseg = (len + kSegment - 1) / kSegment;
+  for (int64_t item = wave; item < lines * nseg; item += waves) {
+    const int64_t l = item / nseg;
+    const int64_t i0 = (item - l * nseg) * kSegment;
+    const int64_t i1 = min(len, i0 + kSegment);
     const int64_t base = lineOff[l];
-    for (int64_t i = lane; i < len; i += 64) {
+#pragma unroll 4
+    for (int64_t i = i0 + lane; i < i1; i += 64) {
       const double2 x = X[base + inOff[i]];
       m = fmax(m, fmax(fabs(x.x), fabs(x.y)));
     }

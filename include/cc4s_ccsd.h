@@ -45,15 +45,19 @@ typedef struct cc4s_ccsd_context cc4s_ccsd_context;
 enum { CC4S_FIELD_REAL = 0, CC4S_FIELD_COMPLEX = 1 };
 
 /* Arithmetic of the large contractions (ladder, ring, Gamma terms).  FP64
- * reproduces the reference.  MIXED (complex field only) runs them on bf16
- * MFMA with every FP64 operand split into hi + lo parts and FP32
- * accumulation, until the mixer's dR falls below mixed_switch (or stops
- * falling: two passes without halving its best value, the rounding floor of
- * the split residuum), then switches to FP64 for the remaining passes (iterative
- * refinement of the same fixed point); only FP64 passes can declare
- * convergence.  mixed_restart decides whether the DIIS history restarts at
- * the switch (its split-bf16 residua carry rounding at the floor).  Denominators, energy, DIIS and every small contraction stay
- * FP64.  FP32 is accepted by the energy gradient only. */
+ * reproduces the reference.  MIXED (complex field only) runs them on 16-bit
+ * MFMA with every FP64 operand split into hi + lo parts (IEEE half with a
+ * power-of-two scale per operand, or bf16: mixed_split) and FP32 sums of at
+ * most mixed_max_k terms, until the mixer's dR falls below mixed_switch (or
+ * stops falling: two passes without halving its best value).  Then, with
+ * mixed_refine (default), one FP64 anchor pass also evaluates the split
+ * residuum at the same amplitudes; its defect d = R64 - Rsplit corrects the
+ * split passes that follow until they meet the convergence test or stall,
+ * and FP64 passes confirm; without it FP64 passes follow directly and
+ * mixed_restart decides whether the DIIS history restarts at the switch.
+ * Only FP64 passes can declare convergence.  Denominators, energy, DIIS and
+ * every small contraction stay FP64.  FP32 is accepted by the energy
+ * gradient only. */
 enum { CC4S_PRECISION_FP64 = 0, CC4S_PRECISION_MIXED = 1, CC4S_PRECISION_FP32 = 2 };
 
 /* Coupled-cluster method (CoupledCluster.cxx:77-84, registered names "Ccsd"
@@ -86,11 +90,11 @@ typedef struct {
                                      at the same amplitudes (defect d = R64 - Rsplit, DIIS
                                      restarted), split passes with Rsplit + d until they stop
                                      gaining, then FP64 passes (2: DIIS restarted before
-                                     them, 1: kept) */
+                                     them, 1: kept) (1) */
   int mixed_split;                /* MIXED: split parts 0 bf16, 1 IEEE half with a
-                                     power-of-two scale per operand */
+                                     power-of-two scale per operand (1) */
   int64_t mixed_max_k;            /* MIXED: longest FP32 sum of a split contraction; longer
-                                     K is cut into chunks added in FP64 (16384) */
+                                     K is cut into chunks added in FP64 (2048) */
 } cc4s_ccsd_params;
 
 /* One pass of the iteration loop (CoupledCluster.cxx:129-175). */

@@ -45,6 +45,7 @@ def test_default_params_are_the_reference_defaults():
     # mixed-precision settings (no counterpart in the reference)
     assert p.precision == api.PRECISION_FP64
     assert p.mixed_switch == 1e-5
+    assert p.mixed_refine == 1 and p.mixed_split == 1 and p.mixed_max_k == 2048
 
 
 def test_unsupported_precision_is_refused_before_touching_a_device():

@@ -105,8 +105,9 @@ def test_mixed_run_reaches_fp64_energy(shape, switch):
               amplitudesConvergence=1e-8, maxResidua=8)
     res = {}
     for prec in (api.PRECISION_FP64, api.PRECISION_MIXED):
+        # the plain switch: FP64 passes right after the split passes
         with api.CoupledCluster(No, Nv, NG, precision=prec, mixedSwitch=switch,
-                                **kw) as cc:
+                                mixedRefine=0, **kw) as cc:
             cc.set_inputs(inp.gamma, inp.eps)
             res[prec] = cc.run()
     fp64, mixed = res[0], res[1]
@@ -141,7 +142,8 @@ def test_restart_keeping_recent_pairs(placement, monkeypatch):
         cc.set_inputs(inp.gamma, inp.eps)
         fp64 = cc.run()
     with api.CoupledCluster(No, Nv, NG, precision=api.PRECISION_MIXED, mixedSwitch=1e-6,
-                            mixedRestart=True, mixedRestartKeep=2, **kw) as cc:
+                            mixedRestart=True, mixedRestartKeep=2, mixedRefine=0,
+                            **kw) as cc:
         cc.set_inputs(inp.gamma, inp.eps)
         mixed = cc.run()
     assert mixed["convergenceReached"]
