@@ -663,7 +663,7 @@ constexpr int kZBM = 128, kZBN = 128, kZBK = 16, kZWM = 64, kZWN = 32;
 constexpr int kZ3BM = 128, kZ3BN = 64, kZ3BK = 16;
 constexpr int kRBM = 128, kRBN = 128, kRBK = 16, kRWM = 64, kRWN = 64;
 
-int g_complex_generation = 2;
+int g_complex_generation = 3;
 
 void tile_of(Field f, int &BM, int &BN, int &BK) {
   if (f == Field::Real) {
@@ -684,7 +684,7 @@ inline int grid_for(int64_t n) {
 }  // namespace
 
 void set_complex_gett_generation(int g) {
-  g_complex_generation = (g >= 1 && g <= 4) ? g : 2;
+  g_complex_generation = (g >= 1 && g <= 4) ? g : 3;
 }
 
 std::string gett_variant(const GettLaunch &L) {

@@ -132,8 +132,10 @@ void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start = nullptr,
 // name of the kernel variant launch_gett uses for L
 std::string gett_variant(const GettLaunch &L);
 // selects the complex kernel generation (1: 4 waves, 128x64 tile, single
-// LDS buffer; 2: 8 waves, 128x128 tile, double-buffered LDS); plans created
-// afterwards use its tile shape
+// LDS buffer; 2: 8 waves, 128x128 tile, double-buffered LDS, four MFMA
+// products per complex MAC; 3 (default): 8 waves of 32x32 on a 128x64 tile,
+// three products per complex MAC; 4: as 3 with 4 waves of 64x32); plans
+// created afterwards use its tile shape
 void set_complex_gett_generation(int generation);
 
 // split-K reduction shared by both kernel families:

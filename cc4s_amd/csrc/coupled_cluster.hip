@@ -45,6 +45,9 @@ DiisMixer::DiisMixer(Field f, int64_t No, int64_t Nv, int64_t N, DiisPlacement w
     CC4S_ASSERT(v >= static_cast<long long>(es), "CC4S_DIIS_CHUNK_BYTES too small");
     chunk_cap = std::min(chunk_cap, static_cast<size_t>(v));
   }
+  // CC4S_DIIS_PINNED=0 keeps host slots in pageable memory
+  const char *pe = std::getenv("CC4S_DIIS_PINNED");
+  const bool pin = !(pe && std::string(pe) == "0");
   switch (where_) {
     case DiisPlacement::Device:
       A_.resize(N);
@@ -57,15 +60,15 @@ DiisMixer::DiisMixer(Field f, int64_t No, int64_t Nv, int64_t N, DiisPlacement w
     case DiisPlacement::AmplitudesOnHost:
       R_.resize(N);
       for (int64_t i = 0; i < N; ++i) R_[i].allocate(set);
-      for (int64_t i = 0; i < N; ++i) Ahost_.emplace_back(set);
+      for (int64_t i = 0; i < N; ++i) Ahost_.emplace_back(set, pin);
       chunk_ = static_cast<int64_t>(std::min(set, chunk_cap) / es);
       staging_.allocate(chunk_ * es);
       break;
     case DiisPlacement::AllOnHost:
-      for (int64_t i = 0; i < N; ++i) {
-        Ahost_.emplace_back(set);
-        Rhost_.emplace_back(set);
-      }
+      // the change slots first: they are read twice per pass (overlaps and
+      // norm), so they get pinned memory while the system grants it
+      for (int64_t i = 0; i < N; ++i) Rhost_.emplace_back(set, pin);
+      for (int64_t i = 0; i < N; ++i) Ahost_.emplace_back(set, pin);
       current_.allocate(set);
       chunk_ = static_cast<int64_t>(std::min(set, chunk_cap) / es);
       staging_.allocate(static_cast<size_t>(N) * chunk_ * es);
@@ -307,10 +310,10 @@ void CoupledCluster::create_mixer() {
   const auto &s = p_.shape;
   size_t free_b = 0, total_b = 0;
   CC4S_HIP(hipMemGetInfo(&free_b, &total_b));
-  // room kept for buffers allocated on first use (ladder slab, operand
-  // packs, split-K partials, residuum scratch; in mixed mode the packed tau
-  // of the ladder, Nv^2 x No^2 rounded to the kernel tile, 8 bytes each)
-  size_t reserve = (size_t(6) << 30) + s.nset() * scalar_bytes(s.field);
+  // room kept for buffers allocated on first use (ladder slab and block,
+  // operand packs, split-K partials; in mixed mode the packed tau of the
+  // ladder, Nv^2 x No^2 rounded to the kernel tile, 8 bytes each)
+  size_t reserve = size_t(6) << 30;
   if (p_.mixed) {
     const int64_t rows = (s.No * s.No + 127) / 128 * 128;
     reserve += static_cast<size_t>(s.Nv) * s.Nv * rows * 8;

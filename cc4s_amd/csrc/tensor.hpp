@@ -12,6 +12,7 @@
 
 #include <string>
 #include <utility>
+#include <cstdlib>
 #include <vector>
 
 namespace cc4s_amd {
@@ -64,28 +65,36 @@ class DeviceBuffer {
   size_t bytes_ = 0;
 };
 
-// pinned host memory (hipHostMalloc), for data that does not fit in HBM
+// host memory for data that does not fit in HBM: pinned (hipHostMalloc)
+// where the system grants it, else pageable (the runtime stages the copies;
+// slower, but the amount of pinned memory a process may hold is limited)
 class HostBuffer {
  public:
   HostBuffer() = default;
-  explicit HostBuffer(size_t bytes) {
-    if (bytes) CC4S_HIP(hipHostMalloc(&p_, bytes, hipHostMallocDefault));
-    bytes_ = bytes;
+  explicit HostBuffer(size_t bytes, bool try_pinned = true) : bytes_(bytes) {
+    if (!bytes) return;
+    if (try_pinned && hipHostMalloc(&p_, bytes, hipHostMallocDefault) == hipSuccess) {
+      pinned_ = true;
+      return;
+    }
+    (void)hipGetLastError();  // the failed pinning is not an error of later calls
+    p_ = std::aligned_alloc(4096, (bytes + 4095) / 4096 * 4096);
+    if (!p_) throw Error("out of host memory");
+    pinned_ = false;
   }
-  ~HostBuffer() {
-    if (p_) (void)hipHostFree(p_);
-  }
+  ~HostBuffer() { release(); }
   HostBuffer(const HostBuffer &) = delete;
   HostBuffer &operator=(const HostBuffer &) = delete;
-  HostBuffer(HostBuffer &&o) noexcept : p_(o.p_), bytes_(o.bytes_) {
+  HostBuffer(HostBuffer &&o) noexcept : p_(o.p_), bytes_(o.bytes_), pinned_(o.pinned_) {
     o.p_ = nullptr;
     o.bytes_ = 0;
   }
   HostBuffer &operator=(HostBuffer &&o) noexcept {
     if (this != &o) {
-      if (p_) (void)hipHostFree(p_);
+      release();
       p_ = o.p_;
       bytes_ = o.bytes_;
+      pinned_ = o.pinned_;
       o.p_ = nullptr;
       o.bytes_ = 0;
     }
@@ -93,10 +102,21 @@ class HostBuffer {
   }
   void *get() const { return p_; }
   size_t bytes() const { return bytes_; }
+  bool pinned() const { return pinned_; }
 
  private:
+  void release() {
+    if (p_) {
+      if (pinned_)
+        (void)hipHostFree(p_);
+      else
+        std::free(p_);
+    }
+    p_ = nullptr;
+  }
   void *p_ = nullptr;
   size_t bytes_ = 0;
+  bool pinned_ = false;
 };
 
 struct View {

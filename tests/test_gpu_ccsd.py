@@ -78,17 +78,21 @@ def test_residuum_parity_non_hermitian_vertex():
     cc.close()
 
 
-@pytest.mark.parametrize("placement,chunk", [("host", None), ("host", 4096),
-                                             ("host_all", None), ("host_all", 4096)])
+@pytest.mark.parametrize("placement,chunk,pinned", [
+    ("host", None, True), ("host", 4096, True), ("host_all", None, True),
+    ("host_all", 4096, True), ("host_all", 4096, False)])
 @pytest.mark.parametrize("field", ["complex", "real"])
-def test_diis_slots_on_host(field, placement, chunk, monkeypatch):
-    """DIIS slots in pinned host memory (the amplitude slots, or every slot,
+def test_diis_slots_on_host(field, placement, chunk, pinned, monkeypatch):
+    """DIIS slots in host memory (the amplitude slots, or every slot,
     streamed through HBM in chunks; 4096-byte chunks split the sets many
-    times, across the T1/T2 boundary) give the oracle's run."""
+    times, across the T1/T2 boundary; pinned, or pageable as when the
+    system's pinned memory is exhausted) give the oracle's run."""
     monkeypatch.setenv("CC4S_DIIS_PLACEMENT", placement)
     if chunk:
         monkeypatch.setenv("CC4S_DIIS_CHUNK_BYTES", str(chunk))
-    print(f"placement {placement} chunk {chunk} field {field}", flush=True)
+    if not pinned:
+        monkeypatch.setenv("CC4S_DIIS_PINNED", "0")
+    print(f"placement {placement} chunk {chunk} pinned {pinned} field {field}", flush=True)
     kw = dict(maxIterations=30, energyConvergence=1e-10,
               amplitudesConvergence=1e-8, maxResidua=5)
     gpu, T_gpu, ref = _run_both(4, 14, 25, field, 17, **kw)
