@@ -174,8 +174,8 @@ def device_memory():
     return round(free.value / 1e9, 2), round(total.value / 1e9, 2)
 
 
-PLACEMENTS = {0: "hbm", 1: "amplitude slots in pinned host memory",
-              2: "every slot in pinned host memory"}
+PLACEMENTS = {0: "hbm", 1: "amplitude slots in host memory",
+              2: "every slot in host memory"}
 
 
 def phase_report(passes):

@@ -317,8 +317,9 @@ void CoupledCluster::create_mixer() {
   if (p_.mixed) {
     const int64_t rows = (s.No * s.No + 127) / 128 * 128;
     reserve += static_cast<size_t>(s.Nv) * s.Nv * rows * 8;
-    // the defect of the split residuum (one full set)
-    if (p_.mixed_refine) reserve += s.nset() * scalar_bytes(s.field);
+    // the defect of the split residuum (one pair-packed set)
+    if (p_.mixed_refine)
+      reserve += static_cast<size_t>(packed_set_size(s.No, s.Nv)) * scalar_bytes(s.field);
   }
   const size_t avail = free_b > reserve ? free_b - reserve : 0;
   const char *env = std::getenv("CC4S_DIIS_PLACEMENT");
@@ -436,17 +437,19 @@ IterationReport CoupledCluster::iterate() {
   IterationReport rep;
   rep.iteration = iterations_done + 1;
   const void *T = have_amplitudes_ ? T_.get() : nullptr;
-  const int64_t ndoubles = s.nset() * (s.field == Field::Complex ? 2 : 1);
   if (phase_ == Phase::Anchor && T) {
     // the split residuum and the FP64 residuum at the same amplitudes:
-    // d = R64(T) - Rsplit(T); the pass itself continues with R64 and a DIIS
-    // history that holds no split residua
-    defect_.reserve(s.nset() * scalar_bytes(s.field));
+    // d = R64(T) - Rsplit(T), kept pair-packed (both residua satisfy
+    // R_abij = R_baji, and the amplitude update reads one element of each
+    // pair); the pass itself continues with R64 and a DIIS history that
+    // holds no split residua
+    defect_.reserve(packed_set_size(s.No, s.Nv) * scalar_bytes(s.field));
     engine_->set_mixed(true);
-    ccsd_->get_residuum(T, defect_.get());
+    ccsd_->get_residuum(T, R_.get());
     engine_->set_mixed(false);
+    pack_axpby(stream_, s.field, R_.get(), -1.0, defect_.get(), 0.0, s.No, s.Nv);
     ccsd_->get_residuum(T, R_.get());                   // :136
-    axpby(stream_, ndoubles, 1.0, R_.get(), -1.0, defect_.get());
+    pack_axpby(stream_, s.field, R_.get(), 1.0, defect_.get(), 1.0, s.No, s.Nv);
     mixer_->restart(0);
     rep.mixed = false;
     rep.kind = 2;
@@ -457,7 +460,8 @@ IterationReport CoupledCluster::iterate() {
     ccsd_->get_residuum(T, R_.get());                   // :136
     engine_->set_mixed(false);
     if (phase_ == Phase::Corrected)
-      axpby(stream_, ndoubles, 1.0, defect_.get(), 1.0, R_.get());
+      unpack_accumulate(stream_, s.field, defect_.get(), 0, packed_set_size(s.No, s.Nv),
+                        1.0, false, R_.get(), s.No, s.Nv);
   }
   CC4S_HIP(hipStreamSynchronize(stream_));
   const auto t1 = std::chrono::steady_clock::now();

@@ -545,6 +545,45 @@ __global__ __launch_bounds__(kBlock) void norm_packed_kernel(DotArgs p, int coun
 
 }  // namespace
 
+namespace {
+// y[t] = alpha * x[e(t)] + beta * y[t] for the packed elements t of a set,
+// e(t) its full-layout position (the element update_packed_kernel reads)
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void pack_axpby_kernel(
+    const typename Sc<CPLX>::T *x, double alpha, typename Sc<CPLX>::T *y, double beta,
+    int64_t No, int64_t Nv) {
+  using S = Sc<CPLX>;
+  const int64_t n1 = Nv * No;
+  const int64_t np = n1 * (n1 + 1) / 2;
+  for (int64_t t = blockIdx.x * int64_t(kBlock) + threadIdx.x; t < n1 + np;
+       t += int64_t(gridDim.x) * kBlock) {
+    int64_t e = t;
+    if (t >= n1) {
+      int64_t P, Q;
+      pair_of(t - n1, P, Q);
+      const int64_t a = P % Nv, i = P / Nv, b = Q % Nv, j = Q / Nv;
+      e = n1 + a + Nv * (b + Nv * (i + No * j));
+    }
+    const typename S::T v = S::scale(x[e], alpha);
+    y[t] = beta == 0.0 ? v : S::add(v, S::scale(y[t], beta));
+  }
+}
+}  // namespace
+
+void pack_axpby(hipStream_t s, Field f, const void *full, double alpha, void *packed,
+                double beta, int64_t No, int64_t Nv) {
+  const int64_t n = packed_set_size(No, Nv);
+  if (f == Field::Complex)
+    pack_axpby_kernel<true><<<grid_for(n), kBlock, 0, s>>>(
+        static_cast<const double2 *>(full), alpha, static_cast<double2 *>(packed), beta,
+        No, Nv);
+  else
+    pack_axpby_kernel<false><<<grid_for(n), kBlock, 0, s>>>(
+        static_cast<const double *>(full), alpha, static_cast<double *>(packed), beta,
+        No, Nv);
+  CC4S_HIP(hipGetLastError());
+}
+
 void unpack_accumulate(hipStream_t s, Field f, const void *chunk, int64_t t0,
                        int64_t len, double c, bool first, void *out, int64_t No,
                        int64_t Nv) {

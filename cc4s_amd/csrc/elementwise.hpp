@@ -64,6 +64,9 @@ double combine_packed(hipStream_t s, Field f, const void *const *A,
                       const void *const *R, const std::vector<double> &c,
                       int64_t No, int64_t Nv, void *out, void *partials);
 
+// packed = alpha * pack(full) + beta * packed (beta 0: packed is not read)
+void pack_axpby(hipStream_t s, Field f, const void *full, double alpha, void *packed,
+                double beta, int64_t No, int64_t Nv);
 // out (full layout) (first ? = : +=) c * x for the packed elements
 // [t0, t0 + len) of one set, x = chunk[0 .. len); writes both mirror
 // positions of a pair

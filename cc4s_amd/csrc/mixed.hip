@@ -144,17 +144,27 @@ struct PackArgs {
 template <bool FP16>
 __global__ __launch_bounds__(256) void pack_kernel(PackArgs p) {
   __shared__ uint2 tile[kPackTile][kPackTile + 1];
+  // the tile's offsets, read once (-1: outside the operand)
+  __shared__ int64_t oK[kPackTile], oMN[kPackTile];
   const double sc = FP16 ? p.scale[0] : 1.0;
   const int64_t mnb = static_cast<int64_t>(blockIdx.x) * kPackTile;
   const int64_t kb = static_cast<int64_t>(blockIdx.y) * kPackTile;
   const int t = threadIdx.x;
+  if (t < kPackTile) {
+    const int64_t k = p.k0 + kb + t;
+    oK[t] = (kb + t < p.Kc && k < p.K) ? p.offK[k] : -1;
+  } else if (t < 2 * kPackTile) {
+    const int64_t mn = mnb + (t - kPackTile);
+    oMN[t - kPackTile] = mn < p.MN ? p.offMN[mn] : -1;
+  }
+  __syncthreads();
   for (int e = t; e < kPackTile * kPackTile; e += 256) {
     const int a = e % kPackTile, b = e / kPackTile;
     const int kk = p.kcontig ? a : b, mm = p.kcontig ? b : a;
-    const int64_t k = p.k0 + kb + kk, mn = mnb + mm;
     uint2 v = make_uint2(0u, 0u);
-    if (kb + kk < p.Kc && k < p.K && mn < p.MN) {
-      const double2 x = p.X[p.offMN[mn] + p.offK[k]];
+    const int64_t ok = oK[kk], om = oMN[mm];
+    if (ok >= 0 && om >= 0) {
+      const double2 x = p.X[om + ok];
       if constexpr (FP16) {
         v.x = split_part_f16(sc * x.x);
         v.y = split_part_f16(sc * p.sign * x.y);
