@@ -6,6 +6,7 @@
 #include <cstring>
 #include <numeric>
 #include <sstream>
+#include <type_traits>
 
 namespace cc4s_amd {
 
@@ -301,7 +302,10 @@ __global__ __launch_bounds__(kThreads, 2) void gett_kernel(KArgs p) {
 // the four-product one with |ar|+|ai| and |br|+|bi| in place of |a|, |b|.
 constexpr int kZThreads = 512;
 
-template <int NT, int BM, int BN, int BK, int WM, int WN, bool AKC, bool BKC, bool M3>
+// DEEP keeps two stages of global loads in flight (two register sets): the
+// loads for stage s+3 are issued while stage s is multiplied.
+template <int NT, int BM, int BN, int BK, int WM, int WN, bool AKC, bool BKC, bool M3,
+          bool DEEP>
 __global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
   constexpr int MB = WM / 16, NB = WN / 16;
   constexpr int WAVES_N = BN / WN;
@@ -387,31 +391,37 @@ __global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
     for (int i = 0; i < NKB; ++i) { bK[i] = bKn[i]; bKok[i] = bKokn[i]; }
   };
 
-  double2 ra[EA], rb[EB];
-  auto load_data = [&]() {
+  // staging registers; DEEP uses two sets, always indexed by constants
+  constexpr int NSET = DEEP ? 2 : 1;
+  double2 ra[NSET][EA], rb[NSET][EB];
+  auto load_data = [&](auto set_c) {
+    constexpr int q = decltype(set_c)::value;
     const double2 *A = reinterpret_cast<const double2 *>(p.A);
     const double2 *B = reinterpret_cast<const double2 *>(p.B);
 #pragma unroll
     for (int i = 0; i < EA; ++i) {
       const int ir = AKC ? i : 0, ik = AKC ? 0 : i;
       const bool ok = aRowOk[ir] && aKok[ik];
-      ra[i] = ok ? A[aRow[ir] + aK[ik]] : make_double2(0.0, 0.0);
-      ra[i].y *= p.signA;
+      ra[q][i] = ok ? A[aRow[ir] + aK[ik]] : make_double2(0.0, 0.0);
+      ra[q][i].y *= p.signA;
     }
 #pragma unroll
     for (int i = 0; i < EB; ++i) {
       const int ir = BKC ? i : 0, ik = BKC ? 0 : i;
       const bool ok = bColOk[ir] && bKok[ik];
-      rb[i] = ok ? B[bCol[ir] + bK[ik]] : make_double2(0.0, 0.0);
-      rb[i].y *= p.signB;
+      rb[q][i] = ok ? B[bCol[ir] + bK[ik]] : make_double2(0.0, 0.0);
+      rb[q][i].y *= p.signB;
     }
   };
-  auto store_lds = [&](int buf) {
+  auto store_lds = [&](int buf, auto set_c) {
+    constexpr int q = decltype(set_c)::value;
 #pragma unroll
-    for (int i = 0; i < EA; ++i) As[buf][a_k(i)][a_m(i)] = ra[i];
+    for (int i = 0; i < EA; ++i) As[buf][a_k(i)][a_m(i)] = ra[q][i];
 #pragma unroll
-    for (int i = 0; i < EB; ++i) Bs[buf][b_k(i)][b_n(i)] = rb[i];
+    for (int i = 0; i < EB; ++i) Bs[buf][b_k(i)][b_n(i)] = rb[q][i];
   };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, DEEP ? 1 : 0>;
 
   const int lane = t & 63;
   const int w = t >> 6;
@@ -427,20 +437,28 @@ __global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
       if constexpr (M3) accS[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
     }
 
-  // prologue: stage 0 in LDS buffer 0, stage 1 in registers
+  // prologue: stage 0 in LDS buffer 0; stage 1 in registers (DEEP: stage 1
+  // in set 1, stage 2 in set 0)
   if (nstages > 0) {
     load_koff(0, aK, aKok, bK, bKok);
     if (nstages > 1) load_koff(1, aKn, aKokn, bKn, bKokn);
-    load_data();
-    store_lds(0);
+    load_data(S0{});
+    store_lds(0, S0{});
     if (nstages > 1) {
       advance_koff();
       if (nstages > 2) load_koff(2, aKn, aKokn, bKn, bKokn);
-      load_data();
+      load_data(S1{});
+      if (DEEP && nstages > 2) {
+        advance_koff();
+        if (nstages > 3) load_koff(3, aKn, aKokn, bKn, bKokn);
+        load_data(S0{});
+      }
     }
     __syncthreads();
   }
-  for (int s = 0; s < nstages; ++s) {
+  // one K-stage: multiply LDS buffer s&1, then move stage s+1 from its
+  // registers to the other buffer and load the next stage into them
+  auto stage = [&](int s, auto set_c) {
     const int buf = s & 1;
 #pragma unroll
     for (int kq = 0; kq < BK / 4; ++kq) {
@@ -496,15 +514,21 @@ __global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
             accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].x, accI[i][j], 0, 0, 0);
       }
     }
+    constexpr int AHEAD = DEEP ? 3 : 2;  // stage loaded into the freed registers
     if (s + 1 < nstages) {
-      store_lds(buf ^ 1);  // stage s+1, loaded during stage s-1
-      if (s + 2 < nstages) {
+      store_lds(buf ^ 1, set_c);  // stage s+1
+      if (s + AHEAD < nstages) {
         advance_koff();
-        if (s + 3 < nstages) load_koff(s + 3, aKn, aKokn, bKn, bKokn);
-        load_data();
+        if (s + AHEAD + 1 < nstages) load_koff(s + AHEAD + 1, aKn, aKokn, bKn, bKokn);
+        load_data(set_c);
       }
     }
     __syncthreads();
+  };
+  // stage s+1 sits in set (s+1)&1 (DEEP) or in the single set
+  for (int s = 0; s < nstages; s += 2) {
+    stage(s, S1{});
+    if (s + 1 < nstages) stage(s + 1, S0{});
   }
 
   int64_t cCol[NB];
@@ -643,16 +667,16 @@ void dispatch_gett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_
     gett_kernel<CPLX, BM, BN, BK, WM, WN, false, false><<<grid, kThreads, 0, s>>>(args);
 }
 
-template <int NT, int BM, int BN, int BK, int WM, int WN, bool M3>
+template <int NT, int BM, int BN, int BK, int WM, int WN, bool M3, bool DEEP = false>
 void dispatch_zgett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_t s) {
   if (L.a_kcontig && L.b_kcontig)
-    zgett_kernel<NT, BM, BN, BK, WM, WN, true, true, M3><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, true, true, M3, DEEP><<<grid, NT, 0, s>>>(args);
   else if (L.a_kcontig)
-    zgett_kernel<NT, BM, BN, BK, WM, WN, true, false, M3><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, true, false, M3, DEEP><<<grid, NT, 0, s>>>(args);
   else if (L.b_kcontig)
-    zgett_kernel<NT, BM, BN, BK, WM, WN, false, true, M3><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, false, true, M3, DEEP><<<grid, NT, 0, s>>>(args);
   else
-    zgett_kernel<NT, BM, BN, BK, WM, WN, false, false, M3><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, false, false, M3, DEEP><<<grid, NT, 0, s>>>(args);
 }
 
 // tile shapes (complex elements): first generation 128x64; second 128x128;
@@ -684,7 +708,7 @@ inline int grid_for(int64_t n) {
 }  // namespace
 
 void set_complex_gett_generation(int g) {
-  g_complex_generation = (g >= 1 && g <= 4) ? g : 3;
+  g_complex_generation = (g >= 1 && g <= 5) ? g : 3;
 }
 
 std::string gett_variant(const GettLaunch &L) {
@@ -724,6 +748,8 @@ void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start,
     dispatch_zgett<512, kZ3BM, kZ3BN, kZ3BK, 32, 32, true>(L, a, grid, s);
   else if (cplx && g_complex_generation == 4)
     dispatch_zgett<256, kZ3BM, kZ3BN, kZ3BK, 64, 32, true>(L, a, grid, s);
+  else if (cplx && g_complex_generation == 5)
+    dispatch_zgett<512, kZ3BM, kZ3BN, kZ3BK, 32, 32, true, true>(L, a, grid, s);
   else if (cplx)
     dispatch_gett<true, kCBM, kCBN, kCBK, kCWM, kCWN>(L, a, grid, s);
   else

@@ -2,8 +2,8 @@
 //
 //   gett_bench [generation=2] [repetitions=5] [No Nv NG slice]
 //   (generation 1, 2: FP64 four-product kernels; 3: split-bf16 mixed
-//    precision; 4, 5: FP64 three-product kernels (contraction-engine
-//    generations 3 and 4), each checked against generation 2 on the same
+//    precision; 4, 5, 6: FP64 three-product kernels (contraction-engine
+//    generations 3, 4 and 5), each checked against generation 2 on the same
 //    operands; 0: a plain 1 GiB device copy kernel, the reference for HBM
 //    counters)
 //
@@ -81,8 +81,8 @@ int main(int argc, char **argv) {
   const int64_t G = argc > 5 ? std::atoll(argv[5]) : 1500;
   const int64_t s = argc > 6 ? std::atoll(argv[6]) : o;
   // generation 3: the split-bf16 mixed-precision path (packing included)
-  const int engine_gen = gen == 1 ? 1 : gen == 4 ? 3 : gen == 5 ? 4 : 2;
-  const bool check = gen == 4 || gen == 5;
+  const int engine_gen = gen == 1 ? 1 : gen == 4 ? 3 : gen == 5 ? 4 : gen == 6 ? 5 : 2;
+  const bool check = gen >= 4;
   const std::vector<Case> cases = {
       {"ladder-form  Gxc,Gyd->cxdy", "Gxc", "Gyd", "cxdy", {G, s, v}, {G, s, v}, {v, s, v, s}},
       {"ladder-ctr   cxdy,cdij->xyij", "cxdy", "cdij", "xyij", {v, s, v, s}, {v, v, o, o}, {s, s, o, o}},
