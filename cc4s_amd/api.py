@@ -31,6 +31,7 @@ EXPORTS = (
     "cc4s_ccsd_residuum", "cc4s_ccsd_energy",
     "cc4s_ccsd_energy_grad_forward", "cc4s_ccsd_energy_grad_backward",
     "cc4s_ccsd_triples",
+    "cc4s_ccsd_transition_structure_factor", "cc4s_finite_size_correction",
     "cc4s_ccsd_set_profiling",
     "cc4s_ccsd_ladder_stats", "cc4s_ccsd_kernel_stats",
     "cc4s_ccsd_last_error", "cc4s_ccsd_destroy",
@@ -122,6 +123,9 @@ def load_library(path: pathlib.Path | str | None = None):
     lib.cc4s_ccsd_energy.argtypes = [ctxp, vp, vp, dp]
     lib.cc4s_ccsd_energy_grad_forward.argtypes = [ctxp, vp, C.c_int, dp]
     lib.cc4s_ccsd_triples.argtypes = [ctxp, dp]
+    lib.cc4s_ccsd_transition_structure_factor.argtypes = [ctxp, vp, C.c_int64, vp, vp]
+    lib.cc4s_finite_size_correction.argtypes = [vp, C.c_int64, vp, vp, C.c_int64,
+                                                C.c_double, C.c_double, C.c_double, dp]
     lib.cc4s_ccsd_energy_grad_backward.argtypes = [ctxp, vp, C.c_int, vp, C.c_int,
                                                    vp, C.c_int]
     lib.cc4s_ccsd_set_profiling.argtypes = [ctxp, C.c_int]
@@ -163,6 +167,26 @@ def _fortran(a, dtype):
 
 
 MIXED_SPLITS = {"bf16": 0, "half": 1}
+
+
+def finite_size_correction(grid, B, S, N=20, vertex_unit=1.0, grid_unit=1.0,
+                           potential_unit=1.0):
+    """(corrected, uncorrected) of FiniteSizeCorrection's interpolation:
+    grid [NG, 3] Cartesian momenta, B [3, 3] reciprocal lattice vectors (rows
+    Gi, Gj, Gk), S [NG] the structure factor.  Host computation."""
+    lib = load_library()
+    g = np.ascontiguousarray(grid, dtype=np.float64)
+    b = np.ascontiguousarray(B, dtype=np.float64)
+    s = np.ascontiguousarray(S, dtype=np.float64)
+    if g.ndim != 2 or g.shape[1] != 3 or b.shape != (3, 3) or s.shape != (g.shape[0],):
+        raise ValueError("expecting grid [NG, 3], B [3, 3], S [NG]")
+    out = (C.c_double * 2)()
+    rc = lib.cc4s_finite_size_correction(_ptr(g), g.shape[0], _ptr(b), _ptr(s), int(N),
+                                         float(vertex_unit), float(grid_unit),
+                                         float(potential_unit), out)
+    if rc != 0:
+        raise CcsdError(lib.cc4s_ccsd_last_error(None).decode())
+    return out[0], out[1]
 
 
 class CoupledCluster:
@@ -317,6 +341,20 @@ class CoupledCluster:
         e = C.c_double()
         self._check(self.lib.cc4s_ccsd_triples(self._ctx, C.byref(e)))
         return e.value
+
+    def transition_structure_factor(self, V, U=None):
+        """S[G] of FiniteSizeCorrection at the current amplitudes: V [NGg]
+        the Coulomb potential, U [NGg, NG] the singular vectors (None: the
+        vertex is on the grid)."""
+        V = _fortran(V, np.float64)
+        NGg = V.shape[0]
+        u = None if U is None else _fortran(U, np.complex128)
+        if u is not None and u.shape != (NGg, self.NG):
+            raise ValueError(f"singular vectors shape {u.shape} != {(NGg, self.NG)}")
+        S = np.zeros(NGg)
+        self._check(self.lib.cc4s_ccsd_transition_structure_factor(
+            self._ctx, None if u is None else _ptr(u), NGg, _ptr(V), _ptr(S)))
+        return S
 
     def energy_forward(self, gamma=None):
         """Correlation energy at the current amplitudes with the vertex

@@ -2,6 +2,7 @@
 #include "../../include/cc4s_ccsd.h"
 
 #include "coupled_cluster.hpp"
+#include "finite_size.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -216,6 +217,34 @@ int cc4s_ccsd_triples(cc4s_ccsd_context *ctx, double *energy) {
   return guarded(ctx, [&] {
     CC4S_ASSERT(energy, "null output");
     *energy = ctx->cc->triples();
+  });
+}
+
+int cc4s_ccsd_transition_structure_factor(cc4s_ccsd_context *ctx,
+                                          const double *singular_vectors,
+                                          int64_t NG_grid,
+                                          const double *coulomb_potential,
+                                          double *structure_factor) {
+  if (!ctx) return 1;
+  return guarded(ctx, [&] {
+    ctx->cc->transition_structure_factor(singular_vectors, NG_grid, coulomb_potential,
+                                         structure_factor);
+  });
+}
+
+int cc4s_finite_size_correction(const double *grid_vectors, int64_t NG,
+                                const double *reciprocal_lattice,
+                                const double *structure_factor, int64_t N,
+                                double vertex_unit, double grid_unit,
+                                double potential_unit, double *out) {
+  return guarded(nullptr, [&] {
+    CC4S_ASSERT(grid_vectors && reciprocal_lattice && structure_factor && out,
+                "null argument");
+    const FiniteSizeEnergy e =
+        finite_size_correction(grid_vectors, NG, reciprocal_lattice, structure_factor, N,
+                               vertex_unit, grid_unit, potential_unit);
+    out[0] = e.corrected;
+    out[1] = e.uncorrected;
   });
 }
 

@@ -618,6 +618,58 @@ View CoupledCluster::vertex_block(const void *gamma, bool hp,
   return v.as(names);
 }
 
+// S[G] = Re sum_ai conj(Gamma^G_ai) Y^G_ai / V[G] with Y^G_ai = sum_bj
+// Gamma^G_jb Z_abij, Z = 2 tau_abij - tau_baij: the reference's two terms
+// 2 cTC_Gia C_Gjb tau_abij - cTC_Gja C_Gib tau_abij with i <-> j renamed in
+// the second (tau_abji = tau_baij), C = Gamma / sqrt(V).
+void CoupledCluster::transition_structure_factor(const double *U, int64_t NGg,
+                                                 const double *V, double *S) {
+  CC4S_ASSERT(inputs_set_, "inputs not set");
+  CC4S_ASSERT(have_amplitudes_, "no amplitudes yet");
+  CC4S_ASSERT(p_.shape.field == Field::Complex,
+              "the transition structure factor needs the complex field");
+  CC4S_ASSERT(V && S, "null argument");
+  CC4S_HIP(hipSetDevice(device_));
+  const auto &s = p_.shape;
+  const int64_t No = s.No, Nv = s.Nv, NG = s.NG;
+  CC4S_ASSERT(U ? NGg > 0 : NGg == NG,
+              "without singular vectors the grid has the vertex's NG points");
+  const size_t block = static_cast<size_t>(NGg) * No * Nv * 16;
+  energy_weights(stream_, s.field, T_.get(), No, Nv, R_.get());
+  const View Z = dense_view(R_.get(), s.field, "abij", {Nv, Nv, No, No});
+  View ph, hp;
+  if (U) {
+    fsc_u_.reserve(static_cast<size_t>(NGg) * NG * 16);
+    CC4S_HIP(hipMemcpyAsync(fsc_u_.get(), U, static_cast<size_t>(NGg) * NG * 16,
+                            hipMemcpyHostToDevice, stream_));
+    fsc_ph_.reserve(block);
+    fsc_hp_.reserve(block);
+    const View Uv = dense_view(fsc_u_.get(), Field::Complex, "GF", {NGg, NG});
+    ph = dense_view(fsc_ph_.get(), Field::Complex, "Gai", {NGg, Nv, No});
+    hp = dense_view(fsc_hp_.get(), Field::Complex, "Gjb", {NGg, No, Nv});
+    engine_->contract(1.0, Uv, vertex_block(gamma_.get(), false, "Fai"), 0.0, ph);  // :71-80
+    engine_->contract(1.0, Uv, vertex_block(gamma_.get(), true, "Fjb"), 0.0, hp);
+  } else {
+    ph = vertex_block(gamma_.get(), false, "Gai");
+    hp = vertex_block(gamma_.get(), true, "Gjb");
+  }
+  fsc_y_.reserve(block);
+  const View Y = dense_view(fsc_y_.get(), Field::Complex, "Gai", {NGg, Nv, No});
+  engine_->contract(1.0, hp, Z, 0.0, Y);
+  adj_partials_.reserve(block_dots_partials_size(NGg));
+  adj_S_.reserve(NGg * sizeof(double));
+  if (U)  // dense [NGg, (a,i)] blocks
+    block_dots(stream_, fsc_ph_.get(), NGg, Nv * No, 0, Nv * No, 0, 1, fsc_y_.get(), 1.0,
+               false, adj_partials_.get(), adj_S_.as<double>());
+  else
+    block_dots(stream_, gamma_.get(), NG, s.Np(), No, Nv, 0, No, fsc_y_.get(), 1.0, false,
+               adj_partials_.get(), adj_S_.as<double>());
+  CC4S_HIP(hipMemcpyAsync(S, adj_S_.get(), NGg * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_));
+  CC4S_HIP(hipStreamSynchronize(stream_));
+  for (int64_t G = 0; G < NGg; ++G) S[G] /= V[G];  // (1 / sqrt V)^2, :87-102
+}
+
 // E = sum_G Re sum_ia conj(Gamma^G_ia) g^G_ia with g^G_ia = sum_bj conj(Z_abij)
 // Gamma^G_bj (complex), or 1/2 sum_G sum_ai Re(conj Gamma^G_ai g^G_ai) with
 // g^G_ai = sum_bj Zs_abij Gamma^G_bj (real): the pphh integrals are never

@@ -136,6 +136,15 @@ class CoupledCluster {
   void energy_backward(const double *gamma, bool gamma_on_device, double *grad,
                        bool grad_on_device, double *S, bool fp32);
 
+  // Transition structure factor of FiniteSizeCorrection at the current
+  // amplitudes (FiniteSizeCorrection.cxx:49-133), complex field: the vertex
+  // on the momentum grid, Gamma^G = sum_F Gamma^F U[G,F] (U [NGg, NG] complex,
+  // host memory; null: the vertex is on the grid already, NGg = NG), and
+  // S[G] = Re sum_abij conj(Gamma^G_ai) Gamma^G_jb (2 tau_abij - tau_baij) / V[G]
+  // (V [NGg] host, S [NGg] host).
+  void transition_structure_factor(const double *U, int64_t NGg, const double *V,
+                                   double *S);
+
   const CoupledClusterParams &params() const { return p_; }
   double second_order = std::numeric_limits<double>::quiet_NaN();
   cplx final_energy, final_direct, final_exchange;
@@ -161,6 +170,8 @@ class CoupledCluster {
   DeviceBuffer T_, R_, scratch_, energy_partials_;
   // energy adjoint: uploaded vertex, gradient blocks, FP32 operands, dots
   DeviceBuffer adj_gamma_, adj_hp_, adj_ph_, adj_a_, adj_b_, adj_partials_, adj_S_;
+  // transition structure factor: singular vectors, grid vertex blocks, Y
+  DeviceBuffer fsc_u_, fsc_ph_, fsc_hp_, fsc_y_;
   bool have_amplitudes_ = false;
   bool inputs_set_ = false;
   enum class Phase { Fp64, Mixed, Anchor, Corrected };

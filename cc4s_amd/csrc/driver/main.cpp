@@ -13,6 +13,7 @@
 //                           through the C ABI of include/cc4s_ccsd.h
 //   Write                   Write.cxx:27-42 (Writer.hpp)
 //   PerturbativeTriples(Reference)  PerturbativeTriplesReference.cxx:55-180
+//   FiniteSizeCorrection    FiniteSizeCorrection.cxx:27-311 (complex field)
 // Any other algorithm stops the run with "unknown algorithm: <name>", as the
 // reference does for names it does not know.  Step arguments that are unquoted
 // names of stored objects are replaced by the objects (Cc4s.cxx:177-195), the
@@ -170,6 +171,8 @@ class Runner {
       write(args, out_report);
     } else if (name == "PerturbativeTriplesReference" || name == "PerturbativeTriples") {
       triples(args, results);
+    } else if (name == "FiniteSizeCorrection") {
+      finite_size_correction(args, results);
     } else {
       throw std::runtime_error("unknown algorithm: " + name);
     }
@@ -382,6 +385,104 @@ class Runner {
     auto o = make_object(Object::Kind::Data);
     o->data = energy;
     results["energy"] = o;
+  }
+
+  // FiniteSizeCorrection: the transition structure factor on the momentum
+  // grid at the amplitudes (FiniteSizeCorrection.cxx:49-133, on the GPU)
+  // and its interpolation to the finite-size correction (:136-311, host)
+  void finite_size_correction(const NodePtr &args, std::map<std::string, ObjectPtr> &results) {
+    auto vtx = symbol(args, "slicedCoulombVertex");
+    auto amps = symbol(args, "amplitudes");
+    auto sv = symbol(args, "coulombVertexSingularVectors", false);
+    auto pot = symbol(args, "coulombPotential");
+    auto grid = symbol(args, "gridVectors");
+    if (vtx->kind != Object::Kind::SlicedVertex) throw std::runtime_error("expecting a sliced Coulomb vertex");
+    if (amps->kind != Object::Kind::TensorSet || !amps->set->count("ph") || !amps->set->count("pphh"))
+      throw std::runtime_error("unsupported tensor type as 'amplitudes'");
+    if (pot->kind != Object::Kind::Tensor || pot->tensor->complex || pot->tensor->lens.size() != 1)
+      throw std::runtime_error("expecting a Real64 tensor [G] as 'coulombPotential'");
+    if (grid->kind != Object::Kind::Tensor || grid->tensor->complex ||
+        grid->tensor->lens.size() != 2 || grid->tensor->lens[0] != 3)
+      throw std::runtime_error("expecting a Real64 tensor [3, G] as 'gridVectors'");
+    const int64_t No = vtx->No, Nv = vtx->Nv, NF = vtx->vertex->lens[0];
+    const int64_t NGg = grid->tensor->lens[1];
+    if (pot->tensor->lens[0] != NGg) throw std::runtime_error("coulombPotential and gridVectors differ in G");
+    const HostTensor &t1 = amps->set->at("ph"), &t2 = amps->set->at("pphh");
+    if (!t1.complex || !t2.complex || t1.elements() != Nv * No || t2.elements() != Nv * Nv * No * No)
+      throw std::runtime_error("the finite-size correction needs complex amplitudes of the vertex's sizes");
+    const double *U = nullptr;
+    if (sv) {
+      const HostTensor &u = *sv->tensor;
+      if (sv->kind != Object::Kind::Tensor || !u.complex || u.lens.size() != 2 || u.lens[0] != NGg ||
+          u.lens[1] != NF)
+        throw std::runtime_error("expecting Complex64 singular vectors [G, F] matching the grid and the vertex");
+      U = u.data.data();
+    } else if (NGg != NF) {
+      throw std::runtime_error("without coulombVertexSingularVectors the grid must have the vertex's size");
+    }
+    // reciprocal lattice vectors from the grid's metaData Gi, Gj, Gk (:165-187)
+    double B[9];
+    const char *keys[3] = {"Gi", "Gj", "Gk"};
+    for (int r = 0; r < 3; ++r) {
+      auto v = grid->tensor->meta_data ? grid->tensor->meta_data->get(keys[r]) : nullptr;
+      if (!v || v->size() != 3) throw std::runtime_error(std::string("gridVectors without metaData.") + keys[r]);
+      for (int c = 0; c < 3; ++c) B[3 * r + c] = v->get(c)->real();
+    }
+    const int64_t N = yaml::get_integer(args, "interpolationGridSize", 20);
+    std::vector<double> S(static_cast<size_t>(NGg), 0.0);
+    double out[2] = {0.0, 0.0};
+    if (!opt_.dry_run) {
+      std::vector<double> compact, e;
+      HostTensorSet eps;  // the structure factor does not use orbital energies
+      eps["h"].data.assign(No, 0.0);
+      eps["h"].lens = {No};
+      eps["p"].data.assign(Nv, 0.0);
+      eps["p"].lens = {Nv};
+      auto eobj = make_object(Object::Kind::TensorSet);
+      eobj->set = std::make_shared<HostTensorSet>(std::move(eps));
+      const double *gamma = step_inputs(vtx, eobj, compact, e);
+      cc4s_ccsd_params p;
+      cc4s_ccsd_default_params(&p);
+      p.field = CC4S_FIELD_COMPLEX;
+      p.No = No;
+      p.Nv = Nv;
+      p.NG = NF;
+      p.max_residua = 1;
+      p.device = static_cast<int>(yaml::get_integer(args, "device", 0));
+      cc4s_ccsd_context *ctx = nullptr;
+      auto check = [&](int rc) {
+        if (rc != 0) {
+          std::string msg = cc4s_ccsd_last_error(ctx);
+          if (ctx) cc4s_ccsd_destroy(ctx);
+          throw std::runtime_error(msg);
+        }
+      };
+      check(cc4s_ccsd_create(&p, &ctx));
+      check(cc4s_ccsd_set_inputs(ctx, gamma, e.data()));
+      check(cc4s_ccsd_set_amplitudes(ctx, t1.data.data(), t2.data.data()));
+      check(cc4s_ccsd_transition_structure_factor(ctx, U, NGg, pot->tensor->data.data(), S.data()));
+      cc4s_ccsd_destroy(ctx);
+      if (cc4s_finite_size_correction(grid->tensor->data.data(), NGg, B, S.data(), N,
+                                      vtx->vertex->unit, grid->tensor->unit, pot->tensor->unit,
+                                      out) != 0)
+        throw std::runtime_error(cc4s_ccsd_last_error(nullptr));
+      std::printf("Finite-size energy correction:    %.10f\n", out[0] - out[1]);
+    }
+    auto sf = make_object(Object::Kind::Tensor);
+    sf->tensor = std::make_shared<HostTensor>();
+    sf->tensor->lens = {NGg};
+    sf->tensor->dimension_types = {grid->tensor->dimension_types.size() > 1
+                                       ? grid->tensor->dimension_types[1] : std::string("Momentum")};
+    sf->tensor->data = std::move(S);
+    results["transitionStructureFactor"] = sf;
+    auto energy = Node::map();
+    energy->set("correction", out[0] - out[1]);
+    energy->set("uncorrected", out[1]);
+    energy->set("corrected", out[0]);
+    energy->set("unit", pot->tensor->unit * std::pow(grid->tensor->unit, 3.0));
+    auto eo = make_object(Object::Kind::Data);
+    eo->data = energy;
+    results["energy"] = eo;
   }
 
   double coupled_cluster(const NodePtr &args, NodePtr &in_report,

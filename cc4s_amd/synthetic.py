@@ -336,3 +336,26 @@ def write_cc4s_inputs(inp: CcsdInputs, directory, binary: bool = True) -> None:
                 "".join(f"({v.real:.17g},{v.imag:.17g})\n" for v in flat))
         else:
             (d / f"{name}.elements").write_text("".join(f"{v:.17g}\n" for v in flat))
+
+
+def write_tensor(directory, name, values, dims, meta=None, unit=1.0, binary=True):
+    """Writes one cc4s tensor `name`.yaml / .elements (TensorIo.cxx:114-158):
+    values in the cc4s layout (first index fastest, i.e. a Fortran-ordered
+    array), dims = [(length, type)], Complex64 when values are complex."""
+    import pathlib
+    import yaml as _yaml
+    d = pathlib.Path(directory)
+    d.mkdir(parents=True, exist_ok=True)
+    values = np.asfortranarray(values)
+    scalar = "Complex64" if np.iscomplexobj(values) else "Real64"
+    header = _tensor_header(scalar, dims, binary, unit=unit, meta=meta)
+    (d / f"{name}.yaml").write_text(_yaml.safe_dump(header, sort_keys=False))
+    flat = values.reshape(-1, order="F")
+    if binary:
+        flat.astype(np.complex128 if scalar == "Complex64" else np.float64).tofile(
+            d / f"{name}.elements")
+    elif scalar == "Complex64":
+        (d / f"{name}.elements").write_text(
+            "".join(f"({v.real:.17g},{v.imag:.17g})\n" for v in flat))
+    else:
+        (d / f"{name}.elements").write_text("".join(f"{v:.17g}\n" for v in flat))

@@ -169,6 +169,33 @@ int cc4s_ccsd_residuum(cc4s_ccsd_context *ctx, const double *T1,
 int cc4s_ccsd_energy(cc4s_ccsd_context *ctx, const double *T1,
                      const double *T2, double *out);
 
+/* FiniteSizeCorrection::calculateTransitionStructureFactor
+ * (FiniteSizeCorrection.cxx:49-133) at the context's current amplitudes,
+ * complex field: the vertex on the momentum grid Gamma^G = sum_F Gamma^F
+ * U[G,F] (singular_vectors: U [NG_grid, NG] complex interleaved, first index
+ * fastest, host memory; NULL when the vertex is on the grid, NG_grid = NG),
+ * codensities C = Gamma^G / sqrt(V[G]) (coulomb_potential: V [NG_grid], host)
+ * and structure_factor[G] = Re(2 conj(C_Gai) C_Gjb tau_abij -
+ * conj(C_Gaj) C_Gib tau_abij), tau = T2 + T1 T1 (host, NG_grid values). */
+int cc4s_ccsd_transition_structure_factor(cc4s_ccsd_context *ctx,
+                                          const double *singular_vectors,
+                                          int64_t NG_grid,
+                                          const double *coulomb_potential,
+                                          double *structure_factor);
+
+/* FiniteSizeCorrection::interpolation (FiniteSizeCorrection.cxx:136-311),
+ * host computation, no context: grid_vectors [NG][3] Cartesian momenta,
+ * reciprocal_lattice [3][3] (rows: metaData Gi, Gj, Gk of the grid),
+ * structure_factor [NG], N the fine-grid resolution
+ * (interpolationGridSize, 20), and the units of the vertex, the grid and the
+ * potential.  out[0] = corrected, out[1] = uncorrected; the correction is
+ * out[0] - out[1]. */
+int cc4s_finite_size_correction(const double *grid_vectors, int64_t NG,
+                                const double *reciprocal_lattice,
+                                const double *structure_factor, int64_t N,
+                                double vertex_unit, double grid_unit,
+                                double potential_unit, double *out);
+
 /* (T) correlation energy of the context's current amplitudes (after run /
  * iterate, or set_amplitudes): PerturbativeTriplesReference::run
  * (PerturbativeTriplesReference.cxx:55-180, cT off), with the V_ppph slice

@@ -106,6 +106,31 @@ def generate(No: int, Nv: int, rs: float, hartree_fock: bool = True):
     return gamma, eps, fermi
 
 
+def momentum_grid(No: int, Nv: int, rs: float):
+    """(grid [NG, 3], B [3, 3], V [NG]) of the vertex that `generate` builds:
+    the momentum transfer b t of each G index, the reciprocal lattice vectors
+    b e_x, b e_y, b e_z, and the Coulomb potential 4 pi / (v |b t|^2) (the
+    madelung term at t = 0), i.e. Gamma^G_pq^2 for the pairs with momentum
+    transfer t (UegVertexGenerator.cxx:213-296)."""
+    Np = No + Nv
+    maxG = int((5.0 * Np) ** (1.0 / 3.0))
+    grid = [g for g in itertools.product(range(-maxG, maxG + 1), repeat=3)]
+    grid.sort(key=lambda g: g[0] * g[0] + g[1] * g[1] + g[2] * g[2])
+    grid = np.array(grid[:Np], dtype=np.int64)
+    v = rs ** 3 / 3.0 * 4.0 * math.pi * No * 2
+    b = 2.0 * math.pi / v ** (1.0 / 3.0)
+    diffs = grid[:, None, :] - grid[None, :, :]
+    d2 = np.sum(diffs * diffs, axis=2)
+    maxR = int(d2.max())
+    maxT = int(np.max(np.abs(diffs[d2 == maxR])))
+    ts = [t for t in itertools.product(range(-maxT, maxT + 1), repeat=3)
+          if t[0] * t[0] + t[1] * t[1] + t[2] * t[2] <= maxR]
+    fac = 4.0 * math.pi / v
+    pot = np.array([fac / (b * b * (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]))
+                    if any(t) else madelung(v) for t in ts])
+    return b * np.array(ts, dtype=np.float64), b * np.eye(3), pot
+
+
 # recorded by the reference for rs = 1, No = 7, Nv = 26
 # (test/tests/ueg/rs1.0-7occ-26virt/correct.out.yaml:120-170)
 UEG_RS1_NO7_NV26 = {

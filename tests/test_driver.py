@@ -175,7 +175,7 @@ def test_flow_value_on_the_next_line(built, tmp_path):
 
 
 @pytest.mark.parametrize("case,message", [
-    ("unknown", "unknown algorithm: FiniteSizeCorrection"),
+    ("unknown", "unknown algorithm: BasisSetCorrection"),
     ("fermi_high", "Fermi energy above all eigen energies."),
     ("fermi_low", "Fermi energy below all eigen energies."),
     ("version", "serialization versions mismatch"),
@@ -187,7 +187,7 @@ def test_reference_errors(built, tmp_path, case, message):
     syn.write_cc4s_inputs(inp, tmp_path)
     steps = STEPS + CCSD
     if case == "unknown":
-        steps += "\n- name: FiniteSizeCorrection\n  in: {}\n  out: {}\n"
+        steps += "\n- name: BasisSetCorrection\n  in: {}\n  out: {}\n"
     if case in ("fermi_high", "fermi_low"):
         h = yaml.safe_load((tmp_path / "EigenEnergies.yaml").read_text())
         h["metaData"]["fermiEnergy"] = 10.0 if case == "fermi_high" else -10.0
