@@ -20,8 +20,10 @@ Two parts:
 Parity pins (tests/test_finite_size.py): the spline reproduces the grid
 values and quadratic functions exactly; with identity singular vectors,
 sum_G V(G) S(G) equals the CCSD correlation energy of the same amplitudes
-(the energy decomposed over G); the host restatement in the library is
-compared to this one on synthetic grids.
+(the energy decomposed over G); for a grid filling its bounding box and a
+constant S the corrected energy is the plain lattice sum over the fine grid,
+the vertex unit cancels and the result is linear in S; the host restatement
+in the library is compared to this one on synthetic grids.
 """
 
 from __future__ import annotations

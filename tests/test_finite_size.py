@@ -91,3 +91,35 @@ def test_library_interpolation_refuses_a_flat_lattice():
     B = np.array([[1.0, 0, 0], [0, 1.0, 0], [1.0, 1.0, 0]])
     with pytest.raises(api.CcsdError, match="linearly dependent"):
         api.finite_size_correction(np.zeros((1, 3)), B, np.ones(1), N=2)
+
+
+def test_finite_size_integration_properties():
+    """Pins of the integration that do not go through the spline: for a
+    constant S the interpolant is that constant (Catmull-Rom weights sum to
+    one) when the grid fills its bounding box (a parallelepiped of lattice
+    points), so `corrected` is the plain lattice sum of c / |G + s|^2 over
+    every fine-grid point; the vertex unit cancels; the result is linear in
+    S."""
+    B = _lattice("bcc")
+    r2 = range(-2, 3)
+    grid = np.array([i * B[0] + j * B[1] + k * B[2] for i in r2 for j in r2 for k in r2])
+    N = 2
+    omega = abs(np.dot(np.cross(B[0], B[1]), B[2]))
+    c = 0.37
+    corrected, uncorrected = F.finite_size_correction(grid, B, np.full(len(grid), c), N=N)
+    r = np.arange(-N, N)
+    shifts = np.array([(a * B[0] + b * B[1] + cc * B[2]) / (2 * N) for a in r for b in r for cc in r])
+    pts = (grid[:, None, :] + shifts[None, :, :]).reshape(-1, 3)
+    q2 = np.sum(pts * pts, axis=1)
+    lattice_sum = np.sum(1.0 / q2[q2 > 1e-16])
+    pref = omega / (2.0 * np.pi ** 2)
+    assert abs(corrected - pref * c * lattice_sum / (8 * N ** 3)) < 1e-12 * abs(corrected)
+    g2 = np.sum(grid * grid, axis=1)
+    assert abs(uncorrected - pref * c * np.sum(1.0 / g2[g2 > 1e-16])) < 1e-12 * abs(uncorrected)
+    S = np.cos(grid[:, 0]) + 0.2 * grid[:, 1] ** 2
+    a = F.finite_size_correction(grid, B, S, N=N, vertex_unit=0.3, grid_unit=1.1)
+    b = F.finite_size_correction(grid, B, S, N=N, vertex_unit=2.0, grid_unit=1.1)
+    assert np.allclose(a, b, rtol=1e-13)
+    two = F.finite_size_correction(grid, B, 2.0 * S, N=N, grid_unit=1.1)
+    one = F.finite_size_correction(grid, B, S, N=N, grid_unit=1.1)
+    assert np.allclose(two, 2.0 * np.array(one), rtol=1e-13)
