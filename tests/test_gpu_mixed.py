@@ -178,3 +178,23 @@ def test_defect_corrected_refinement(refine):
     import re
     assert re.fullmatch(r"S+AC*F*", kinds), kinds
     assert all(t["pass_kind"] in (0, 2) for t in mixed["trace"] if t["converged"])
+
+
+@pytest.mark.parametrize("opts", [dict(method="Drccd"), dict(levelShift=0.3)],
+                         ids=["drccd", "levelShift"])
+def test_refinement_with_other_methods_and_shifts(opts):
+    """The anchored refinement (default) with method Drccd and with a level
+    shift: the mixed run reaches the FP64 run's energy."""
+    No, Nv, NG = 12, 60, 300
+    print(f"mixed refinement {opts}", flush=True)
+    inp = syn.generate(No, Nv, NG, "complex", 42, scale_rule="mp2_per_occupied")
+    kw = dict(maxIterations=80, energyConvergence=1e-10, amplitudesConvergence=1e-8,
+              maxResidua=8, **opts)
+    res = {}
+    for prec in (api.PRECISION_FP64, api.PRECISION_MIXED):
+        with api.CoupledCluster(No, Nv, NG, precision=prec, **kw) as cc:
+            cc.set_inputs(inp.gamma, inp.eps)
+            res[prec] = cc.run()
+    assert res[0]["convergenceReached"] and res[1]["convergenceReached"]
+    assert abs(res[1]["energy"]["correlation"] - res[0]["energy"]["correlation"]) < 1e-8
+    assert all(t["pass_kind"] in (0, 2) for t in res[1]["trace"] if t["converged"])
