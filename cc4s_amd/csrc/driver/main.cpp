@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -173,6 +174,8 @@ class Runner {
       triples(args, results);
     } else if (name == "FiniteSizeCorrection") {
       finite_size_correction(args, results);
+    } else if (name == "SecondOrderPerturbationTheory") {
+      flops = second_order(args, results);
     } else {
       throw std::runtime_error("unknown algorithm: " + name);
     }
@@ -281,6 +284,11 @@ class Runner {
     sliced_counts(e, No, Nv);
     if (op->kind != Object::Kind::Tensor) throw std::runtime_error("expecting a tensor as 'operator'");
     const auto &t = *op->tensor;
+    if (t.lens.size() == 2 && t.dimension_types.size() == 2 && t.dimension_types[0] == "State" &&
+        t.dimension_types[1] == "State") {
+      results["slicedOperator"] = slice_state_operator(t, No, Nv);
+      return;
+    }
     // the CCSD path slices the vertex [G, State, State]
     if (t.lens.size() != 3 || t.dimension_types.size() != 3 || t.dimension_types[1] != "State" ||
         t.dimension_types[2] != "State" || t.dimension_types[0] == "State")
@@ -294,6 +302,40 @@ class Runner {
     o->No = No;
     o->Nv = Nv;
     results["slicedOperator"] = o;
+  }
+
+  // a one-body operator [State, State] (the Fock operator) as the tensor set
+  // {hh, hp, ph, pp}: holes [0, No), particles [Np - Nv, Np) along both
+  // dimensions (SliceOperator.cxx:95-131)
+  static ObjectPtr slice_state_operator(const HostTensor &t, int64_t No, int64_t Nv) {
+    const int64_t Np = t.lens[0];
+    if (t.lens[1] != Np || Np < No + Nv) throw std::runtime_error("operator has fewer states than holes + particles");
+    std::printf("Slicing operator [State, State] into holes and particles.\n");
+    const size_t es = t.complex ? 2 : 1;
+    auto o = make_object(Object::Kind::TensorSet);
+    o->set = std::make_shared<HostTensorSet>();
+    const char *parts[4] = {"hh", "hp", "ph", "pp"};
+    for (const char *part : parts) {
+      int64_t begin[2], len[2];
+      for (int d = 0; d < 2; ++d) {
+        begin[d] = part[d] == 'h' ? 0 : Np - Nv;
+        len[d] = part[d] == 'h' ? No : Nv;
+      }
+      HostTensor s;
+      s.complex = t.complex;
+      s.lens = {len[0], len[1]};
+      s.dimension_types = t.dimension_types;
+      s.unit = t.unit;
+      s.meta_data = t.meta_data;
+      s.data.resize(static_cast<size_t>(len[0] * len[1]) * es);
+      for (int64_t y = 0; y < len[1]; ++y)
+        for (int64_t x = 0; x < len[0]; ++x)
+          for (size_t c = 0; c < es; ++c)
+            s.data[es * (x + len[0] * y) + c] =
+                t.data[es * ((begin[0] + x) + Np * (begin[1] + y)) + c];
+      (*o->set)[part] = std::move(s);
+    }
+    return o;
   }
 
   void vertex_coulomb_integrals(const NodePtr &args, std::map<std::string, ObjectPtr> &results) {
@@ -483,6 +525,112 @@ class Runner {
     auto eo = make_object(Object::Kind::Data);
     eo->data = energy;
     results["energy"] = eo;
+  }
+
+  // spin degeneracy of SecondOrderPerturbationTheory.cxx:67-85: the Spin
+  // property of the State dimension (propertyIndices.Spin of State.yaml in
+  // the working directory, index -> value, TensorIo.cxx:30-63) marks an open
+  // shell when fewer states carry the value 0 than there are states; closed
+  // shell (2) otherwise and without the property
+  static double spin_degeneracy() {
+    if (!std::ifstream("State.yaml").good()) return 2.0;
+    auto dim = yaml::parse_file("State.yaml");
+    auto props = dim ? dim->get("propertyIndices") : nullptr;
+    auto spin = props ? props->get("Spin") : nullptr;
+    if (!spin || !spin->is_map()) return 2.0;
+    size_t first = 0;
+    for (const auto &e : spin->entries)
+      if (e.second && e.second->integer() == 0) ++first;
+    return first < spin->size() ? 1.0 : 2.0;
+  }
+
+  // SecondOrderPerturbationTheory (SecondOrderPerturbationTheory.cxx:54-149):
+  // the second-order energy of the first-order amplitudes
+  // T_ai = conj(f_ai) / (e_a - e_i), T_abij = conj(V_abij) / (e_a + e_b - e_i - e_j).
+  // The doubles are the first pass of the iteration on the GPU (R2 = V_abij,
+  // T2 = -R2 / D, the energy kernel): its direct term is -2 sum |V_abij|^2 / D
+  // and its exchange term sum V_abij conj(V_baij) / D, the reference's
+  // exchange sum V_abji conj(V_abij) / D by the pair symmetry V_abij = V_baji
+  // of Coulomb integrals; the degeneracy g scales them by g^2 / 4 and g / 2.
+  // The singles -g sum |f_ai|^2 / (e_a - e_i) are No * Nv host terms.
+  double second_order(const NodePtr &args, std::map<std::string, ObjectPtr> &results) {
+    auto ints = symbol(args, "coulombIntegrals");
+    auto eps = symbol(args, "slicedEigenEnergies");
+    auto fock = symbol(args, "slicedFockOperator", false);
+    if (args->has("slicedFockOperator") && !fock)
+      throw std::runtime_error("'slicedFockOperator' does not name a stored object");
+    if (ints->kind != Object::Kind::Integrals)
+      throw std::runtime_error("unsupported tensor type as 'coulombIntegrals'");
+    int64_t No, Nv;
+    sliced_counts(eps, No, Nv);
+    auto vtx = ints->sliced_vertex;
+    if (No != vtx->No || Nv != vtx->Nv) throw std::runtime_error("slices of different hole/particle counts");
+    const bool cplx = !ints->real_field;
+    const double g = spin_degeneracy();
+    const auto &eh = eps->set->at("h").data;
+    const auto &ep = eps->set->at("p").data;
+    double singles = 0.0;
+    if (fock) {
+      if (fock->kind != Object::Kind::TensorSet || !fock->set->count("ph"))
+        throw std::runtime_error("expecting a TensorSet {hh, hp, ph, pp} as 'slicedFockOperator'");
+      const HostTensor &f = fock->set->at("ph");
+      if (f.complex != cplx || f.lens.size() != 2 || f.lens[0] != Nv || f.lens[1] != No)
+        throw std::runtime_error("slicedFockOperator.ph does not match the field or [particles, holes]");
+      const size_t es = cplx ? 2 : 1;
+      for (int64_t i = 0; i < No; ++i)
+        for (int64_t a = 0; a < Nv; ++a) {
+          const double *v = &f.data[es * (a + Nv * i)];
+          const double m2 = v[0] * v[0] + (cplx ? v[1] * v[1] : 0.0);
+          singles -= g * m2 / (ep[a] - eh[i]);
+        }
+    }
+    std::printf("Contracting second order energy...\n");
+    double direct = 0.0, exchange = 0.0, flops = 0.0;
+    if (!opt_.dry_run) {
+      std::vector<double> compact, e;
+      const double *gamma = step_inputs(vtx, eps, compact, e);
+      cc4s_ccsd_params p;
+      cc4s_ccsd_default_params(&p);
+      p.field = cplx ? CC4S_FIELD_COMPLEX : CC4S_FIELD_REAL;
+      p.No = No;
+      p.Nv = Nv;
+      p.NG = vtx->vertex->lens[0];
+      p.max_residua = 1;
+      p.device = static_cast<int>(yaml::get_integer(args, "device", 0));
+      cc4s_ccsd_context *ctx = nullptr;
+      auto check = [&](int rc) {
+        if (rc != 0) {
+          std::string msg = cc4s_ccsd_last_error(ctx);
+          if (ctx) cc4s_ccsd_destroy(ctx);
+          throw std::runtime_error(msg);
+        }
+      };
+      check(cc4s_ccsd_create(&p, &ctx));
+      check(cc4s_ccsd_set_inputs(ctx, gamma, e.data()));
+      cc4s_ccsd_iteration it;
+      check(cc4s_ccsd_iterate(ctx, &it));
+      cc4s_ccsd_destroy(ctx);
+      direct = it.direct * g * g / 4.0;
+      exchange = it.exchange * g / 2.0;
+      flops = it.flops;
+    }
+    const double total = singles + direct + exchange;
+    std::printf("correlation energy: %.12f\n  singles:  %.12f\n  direct:   %.12f\n  exchange: %.12f\n",
+                total, singles, direct, exchange);
+    auto energy = Node::map();
+    energy->set("singles", singles);
+    energy->set("direct", direct);
+    energy->set("exchange", exchange);
+    energy->set("correlation", total);
+    energy->set("secondOrderSingles", singles);
+    energy->set("secondOrderDirect", direct);
+    energy->set("secondOrderExchange", exchange);
+    energy->set("secondOrder", total);
+    energy->set("unit", eps->set->at("h").unit);
+    auto o = make_object(Object::Kind::Data);
+    o->data = energy;
+    results["energy"] = o;
+    return flops;
   }
 
   double coupled_cluster(const NodePtr &args, NodePtr &in_report,

@@ -32,7 +32,12 @@ binary, which needs CTF and yaml-cpp that are not available offline):
   formulation from explicit V slices (including V_pppp); tests require the two
   residua to agree to 1e-12 for real and complex inputs;
 * the first iteration reproduces the MP2 closed form
-  (SecondOrderPerturbationTheory.cxx:108-127);
+  (SecondOrderPerturbationTheory.cxx:108-127); that second-order energy's
+  singles term equals the second derivative of the exact ground-state energy
+  of the one-body Hamiltonian diag(e) + lambda f (finite differences), and
+  its degeneracy factors are pinned by the spin-orbital restatement of a
+  closed-shell system (degeneracy 1 on the doubled orbitals gives the
+  closed-shell energy);
 * DIIS with maxResidua=1 is plain Jacobi; the converged fixed point is
   independent of levelShift and integralsSliceSize;
 * V slices are checked against brute-force sums over G;
@@ -532,17 +537,36 @@ def coupled_cluster(P, max_iterations=16, energy_convergence=1e-7,
     return result
 
 
-def mp2_energy(P):
-    """Second-order energy as SecondOrderPerturbationTheory.cxx:108-127 forms
-    it: T_abij = conj(V_abij) / ((e_a - e_i) + (e_b - e_j)),
-    D = -2 sum V_abij T_abij, X = + sum V_abji T_abij (no singles: f_ph = 0)."""
+def second_order(P, fph=None, degeneracy=2.0):
+    """SecondOrderPerturbationTheory::run (SecondOrderPerturbationTheory.cxx:
+    54-149): first-order amplitudes T_ai = conj(f_ai) / (e_a - e_i) and
+    T_abij = conj(V_abij) / ((e_a - e_i) + (e_b - e_j)) (:98-120); singles
+    S = -g sum f_ai T_ai, direct D = -g^2/2 sum V_abij T_abij, exchange
+    X = +g/2 sum V_abji T_abij (:121-127) with the spin degeneracy g (2 for
+    closed shells, 1 when fewer states carry the first spin value than there
+    are states, :67-85).  fph [a, i]: the particle-hole block of the Fock
+    operator (None: canonical orbitals, f_ph = 0, :155-185).  Returns the
+    real parts {singles, direct, exchange, correlation}."""
+    g = float(degeneracy)
     V = P.V["pphh"]
     dph = P.eps_p[:, None] - P.eps_h[None, :]
     Dpphh = dph[:, None, :, None] + dph[None, :, None, :]
     T = np.conj(V) * (1.0 / Dpphh)
-    D = -2.0 * np.sum(V * T)
-    X = np.sum(V.transpose(0, 1, 3, 2) * T)
-    return float(np.real(D + X))
+    S = 0.0
+    if fph is not None:
+        S = -g * np.sum(fph * (np.conj(fph) * (1.0 / dph)))
+    D = -0.5 * g * g * np.sum(V * T)
+    X = 0.5 * g * np.sum(V.transpose(0, 1, 3, 2) * T)
+    out = {"singles": float(np.real(S)), "direct": float(np.real(D)),
+           "exchange": float(np.real(X))}
+    out["correlation"] = float(np.real(S + D + X))
+    return out
+
+
+def mp2_energy(P):
+    """Second-order correlation energy of canonical closed-shell orbitals
+    (f_ph = 0, degeneracy 2), SecondOrderPerturbationTheory.cxx:108-127."""
+    return second_order(P)["correlation"]
 
 
 # ---------------------------------------------------------------------------

@@ -16,6 +16,7 @@ import yaml
 
 from cc4s_amd import build as B
 from cc4s_amd import synthetic as syn
+from oracle import ccsd_oracle as O
 
 DRIVER = B.DRIVER
 
@@ -203,3 +204,75 @@ def test_reference_errors(built, tmp_path, case, message):
     r = run_driver(tmp_path, steps, "--dry-run")
     assert r.returncode == 1
     assert message in r.stderr
+
+
+MP2 = """
+- name: Read
+  in:
+    fileName: "FockOperator.yaml"
+  out:
+    destination: FockOperator
+- name: SliceOperator
+  in:
+    slicedEigenEnergies: EigenEnergies
+    operator: FockOperator
+  out:
+    slicedOperator: FockOperator
+- name: SecondOrderPerturbationTheory
+  in:
+    coulombIntegrals: CoulombIntegrals
+    slicedEigenEnergies: EigenEnergies
+    slicedFockOperator: FockOperator
+  out:
+    energy: Mp2Energy
+- name: Write
+  in:
+    source: FockOperator
+    fileName: "SlicedFock.yaml"
+  out: {}
+"""
+
+
+def fock_operator(inp, seed, scale=0.05):
+    """A Hermitian Fock operator [State, State]: the eigen energies on the
+    diagonal, random hole-particle couplings (cc4s layout F[p, q])."""
+    No, Np = inp.No, inp.Np
+    rng = np.random.default_rng(seed)
+    cplx = inp.field == "complex"
+    F = np.diag(np.asarray(inp.eps, dtype=complex if cplx else float))
+    f = scale * rng.standard_normal((Np - No, No))
+    if cplx:
+        f = f + 1j * scale * rng.standard_normal((Np - No, No))
+    F[No:, :No] = f
+    F[:No, No:] = np.conj(f).T
+    return F, f
+
+
+@pytest.mark.parametrize("spin", ["closed", "open"])
+@pytest.mark.parametrize("field", ["real", "complex"])
+def test_second_order_singles_and_fock_slices(built, tmp_path, field, spin):
+    """SliceOperator of a [State, State] operator gives {hh, hp, ph, pp};
+    SecondOrderPerturbationTheory's singles term (host, also in a dry run)
+    equals the oracle's, halved when State.yaml's Spin property marks an
+    open shell (SecondOrderPerturbationTheory.cxx:67-85)."""
+    No, Nv, NG = 3, 7, 5
+    inp = syn.generate(No, Nv, NG, field, 6)
+    syn.write_cc4s_inputs(inp, tmp_path)
+    F, f = fock_operator(inp, 2)
+    syn.write_tensor(tmp_path, "FockOperator", F, [(No + Nv, "State"), (No + Nv, "State")])
+    if spin == "open":
+        spins = {i: (0 if i < (No + Nv) // 2 else 1) for i in range(No + Nv)}
+        (tmp_path / "State.yaml").write_text(yaml.safe_dump({"propertyIndices": {"Spin": spins}}))
+    r = run_driver(tmp_path, STEPS + MP2, "--dry-run")
+    assert r.returncode == 0, r.stderr
+    rep = yaml.safe_load((tmp_path / "dry-cc4s.out.yaml").read_text())
+    e = rep["steps"][7]["out"]["energy"]
+    P = O.CcsdProblem(inp.gamma, inp.eps, No, field)
+    want = O.second_order(P, fph=f, degeneracy=1.0 if spin == "open" else 2.0)["singles"]
+    assert abs(e["singles"] - want) < 1e-14 * abs(want)
+    assert e["secondOrderSingles"] == e["singles"]
+    out = yaml.safe_load((tmp_path / "SlicedFock.yaml").read_text())
+    rng = {"h": slice(0, No), "p": slice(No, No + Nv)}
+    for part in ("hh", "hp", "ph", "pp"):
+        got = read_tensor(tmp_path, f"SlicedFock.components.{part}", out["components"][part])
+        assert np.array_equal(got, F[rng[part[0]], rng[part[1]]]), part

@@ -205,3 +205,89 @@ def test_reference_ueg_operation_file(tmp_path):
     assert triples["name"] == "PerturbativeTriplesReference"
     assert abs(triples["out"]["energy"]["correlation"]
                - ueg.UEG_RS1_NO7_NV26["Triples"]["correlation"]) < 1e-10
+
+
+MP2_STEP = """
+- name: SecondOrderPerturbationTheory
+  in:
+    coulombIntegrals: CoulombIntegrals
+    slicedEigenEnergies: EigenEnergies
+{fock}  out:
+    energy: Mp2Energy
+"""
+
+
+def mp2_out(report):
+    for k in sorted(report["steps"], key=int):
+        if report["steps"][k]["name"] == "SecondOrderPerturbationTheory":
+            return report["steps"][k]["out"]["energy"]
+
+
+def test_second_order_ueg(tmp_path):
+    """SecondOrderPerturbationTheory on the rs = 1, No = 7, Nv = 26 electron
+    gas: the reference's recorded second-order energy (correct.out.yaml,
+    CoupledCluster's secondOrder) and the oracle's direct and exchange terms."""
+    from oracle import ueg
+    gamma, eps, fermi = ueg.generate(7, 26, 1.0)
+    print("second order, UEG rs=1 No=7 Nv=26", flush=True)
+    inp = syn.CcsdInputs(gamma=gamma, eps=eps, No=7, field="complex", scale=1.0,
+                         fermi_energy=fermi)
+    syn.write_cc4s_inputs(inp, tmp_path)
+    e = mp2_out(run(tmp_path, STEPS + MP2_STEP.format(fock="")))
+    assert abs(e["secondOrder"] - ueg.UEG_RS1_NO7_NV26["Ccsd"]["secondOrder"]) < 1e-12
+    want = O.second_order(O.CcsdProblem(gamma, eps, 7, "complex"))
+    for key in ("direct", "exchange", "correlation"):
+        assert abs(e[key] - want[key]) < 1e-12, key
+    assert e["singles"] == 0.0
+
+
+@pytest.mark.parametrize("field", ["real", "complex"])
+def test_second_order_with_fock_singles(tmp_path, field):
+    """A non-canonical Fock operator (ph couplings) read, sliced and passed as
+    slicedFockOperator: every energy term matches the oracle."""
+    from tests.test_driver import fock_operator
+    No, Nv, NG = 5, 19, 40
+    print(f"second order with Fock singles, {field}", flush=True)
+    inp = syn.generate(No, Nv, NG, field, 13)
+    syn.write_cc4s_inputs(inp, tmp_path)
+    F, f = fock_operator(inp, 3)
+    syn.write_tensor(tmp_path, "FockOperator", F, [(No + Nv, "State"), (No + Nv, "State")])
+    fock = "    slicedFockOperator: FockOperator\n"
+    steps = STEPS + """
+- name: Read
+  in:
+    fileName: "FockOperator.yaml"
+  out:
+    destination: FockOperator
+- name: SliceOperator
+  in:
+    slicedEigenEnergies: EigenEnergies
+    operator: FockOperator
+  out:
+    slicedOperator: FockOperator
+""" + MP2_STEP.format(fock=fock)
+    e = mp2_out(run(tmp_path, steps))
+    want = O.second_order(O.CcsdProblem(inp.gamma, inp.eps, No, field), fph=f)
+    for key in ("singles", "direct", "exchange", "correlation"):
+        assert abs(e[key] - want[key]) <= 1e-12 * abs(want["correlation"]), key
+    assert e["secondOrder"] == e["correlation"]
+
+
+def test_second_order_open_shell_spin_orbitals(tmp_path):
+    """The spin-orbital restatement of a closed-shell system, marked open
+    shell by State.yaml's Spin property (degeneracy 1), gives the closed-shell
+    second-order energy."""
+    from tests.test_oracle import _spin_orbitals
+    No, Nv, NG = 3, 9, 20
+    print("second order, open shell (spin orbitals)", flush=True)
+    inp = syn.generate(No, Nv, NG, "complex", 17)
+    g, e = _spin_orbitals(inp.gamma, np.asarray(inp.eps), No)
+    so = syn.CcsdInputs(gamma=g, eps=e, No=2 * No, field="complex", scale=inp.scale,
+                        fermi_energy=inp.fermi_energy)
+    syn.write_cc4s_inputs(so, tmp_path)
+    spins = [0] * No + [1] * No + [0] * Nv + [1] * Nv
+    (tmp_path / "State.yaml").write_text(
+        yaml.safe_dump({"propertyIndices": {"Spin": {i: s for i, s in enumerate(spins)}}}))
+    got = mp2_out(run(tmp_path, STEPS + MP2_STEP.format(fock="")))
+    want = O.second_order(O.CcsdProblem(inp.gamma, inp.eps, No, "complex"))
+    assert abs(got["correlation"] - want["correlation"]) < 1e-12 * abs(want["correlation"])

@@ -226,3 +226,69 @@ def test_ueg_known_answer_from_the_reference_test_suite():
     assert r["convergenceReached"]
     for key in ("correlation", "direct", "exchange", "secondOrder"):
         assert abs(r["energy"][key] - ref["Drccd"][key]) < 1e-10, key
+
+
+def test_second_order_singles_from_the_one_body_hamiltonian():
+    """The singles term of SecondOrderPerturbationTheory is the lambda^2
+    coefficient of the exact ground-state energy of the one-body Hamiltonian
+    diag(e) + lambda (f_ph + f_ph^dagger) (g times the sum of its No lowest
+    eigenvalues), taken here by Richardson-extrapolated finite differences."""
+    No, Nv, NG = 3, 6, 8
+    inp = syn.generate(No, Nv, NG, "complex", seed=21)
+    P = O.CcsdProblem(inp.gamma, inp.eps, No, "complex")
+    rng = np.random.default_rng(5)
+    f = 0.05 * (rng.standard_normal((Nv, No)) + 1j * rng.standard_normal((Nv, No)))
+    Np = No + Nv
+
+    def ground(lam, g=2.0):
+        h = np.diag(np.asarray(inp.eps, dtype=complex))
+        h[No:, :No] += lam * f
+        h[:No, No:] += lam * np.conj(f).T
+        return g * np.sum(np.linalg.eigvalsh(h)[:No])
+
+    assert Np == inp.gamma.shape[1]
+    e0 = ground(0.0)
+    d = lambda lam: (ground(lam) - e0) / lam ** 2
+    lam = 1e-2
+    e2 = (4.0 * d(lam / 2) - d(lam)) / 3.0
+    s = O.second_order(P, fph=f)["singles"]
+    assert abs(s - e2) < 1e-6 * abs(e2)
+    assert O.second_order(P, fph=f, degeneracy=1.0)["singles"] == pytest.approx(s / 2, rel=1e-15)
+
+
+def _spin_orbitals(gamma, eps, No):
+    """The same closed-shell system in spin orbitals: holes (up, down), then
+    particles (up, down); the vertex is diagonal in spin."""
+    NG, Np = gamma.shape[0], gamma.shape[1]
+    Nv = Np - No
+    up = list(range(No)) + [None] * No + list(range(No, Np)) + [None] * Nv
+    dn = [None] * No + list(range(No)) + [None] * Nv + list(range(No, Np))
+    g = np.zeros((NG, 2 * Np, 2 * Np), dtype=gamma.dtype)
+    for m in (up, dn):
+        idx = [k for k, p in enumerate(m) if p is not None]
+        src = [m[k] for k in idx]
+        g[np.ix_(range(NG), idx, idx)] = gamma[np.ix_(range(NG), src, src)]
+    e = np.concatenate([eps[:No], eps[:No], eps[No:], eps[No:]])
+    return g, e
+
+
+def test_second_order_degeneracy_by_spin_orbitals():
+    """Degeneracy 1 (open shell) on the spin-orbital restatement of a
+    closed-shell system gives the closed-shell (degeneracy 2) energy: g^2 on
+    the direct term and g on the exchange term are the only consistent
+    choice, with and without Fock singles."""
+    No, Nv, NG = 2, 4, 6
+    inp = syn.generate(No, Nv, NG, "complex", seed=23)
+    P = O.CcsdProblem(inp.gamma, inp.eps, No, "complex")
+    g, e = _spin_orbitals(inp.gamma, np.asarray(inp.eps), No)
+    Pso = O.CcsdProblem(g, e, 2 * No, "complex")
+    rng = np.random.default_rng(8)
+    f = 0.1 * (rng.standard_normal((Nv, No)) + 1j * rng.standard_normal((Nv, No)))
+    fso = np.zeros((2 * Nv, 2 * No), dtype=complex)
+    fso[:Nv, :No] = f
+    fso[Nv:, No:] = f
+    for fc, fs in ((None, None), (f, fso)):
+        closed = O.second_order(P, fph=fc)
+        so = O.second_order(Pso, fph=fs, degeneracy=1.0)
+        assert abs(so["correlation"] - closed["correlation"]) < 1e-13 * abs(closed["correlation"])
+        assert abs(so["singles"] - closed["singles"]) < 1e-15 + 1e-13 * abs(closed["singles"])
