@@ -450,8 +450,20 @@ class Runner {
     const int64_t NGg = grid->tensor->lens[1];
     if (pot->tensor->lens[0] != NGg) throw std::runtime_error("coulombPotential and gridVectors differ in G");
     const HostTensor &t1 = amps->set->at("ph"), &t2 = amps->set->at("pphh");
-    if (!t1.complex || !t2.complex || t1.elements() != Nv * No || t2.elements() != Nv * Nv * No * No)
-      throw std::runtime_error("the finite-size correction needs complex amplitudes of the vertex's sizes");
+    if (t1.complex != t2.complex || t1.elements() != Nv * No || t2.elements() != Nv * Nv * No * No)
+      throw std::runtime_error("the finite-size correction needs amplitudes of the vertex's sizes");
+    // real amplitudes (real Coulomb integrals) are taken to the complex field,
+    // the vertex being complex in either case (FiniteSizeCorrection.cxx:103-117)
+    std::vector<double> c1, c2;
+    const double *T1 = t1.data.data(), *T2 = t2.data.data();
+    if (!t1.complex) {
+      c1.assign(2 * t1.data.size(), 0.0);
+      c2.assign(2 * t2.data.size(), 0.0);
+      for (size_t k = 0; k < t1.data.size(); ++k) c1[2 * k] = t1.data[k];
+      for (size_t k = 0; k < t2.data.size(); ++k) c2[2 * k] = t2.data[k];
+      T1 = c1.data();
+      T2 = c2.data();
+    }
     const double *U = nullptr;
     if (sv) {
       const HostTensor &u = *sv->tensor;
@@ -501,7 +513,7 @@ class Runner {
       };
       check(cc4s_ccsd_create(&p, &ctx));
       check(cc4s_ccsd_set_inputs(ctx, gamma, e.data()));
-      check(cc4s_ccsd_set_amplitudes(ctx, t1.data.data(), t2.data.data()));
+      check(cc4s_ccsd_set_amplitudes(ctx, T1, T2));
       check(cc4s_ccsd_transition_structure_factor(ctx, U, NGg, pot->tensor->data.data(), S.data()));
       cc4s_ccsd_destroy(ctx);
       if (cc4s_finite_size_correction(grid->tensor->data.data(), NGg, B, S.data(), N,

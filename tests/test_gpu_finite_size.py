@@ -128,3 +128,92 @@ def test_driver_finite_size_correction_ueg(tmp_path):
     # the structure factor decomposes the correlation energy over G
     ccsd = next(s for s in steps if s["name"] == "CoupledCluster")
     assert abs(np.sum(V * S) - ccsd["out"]["energy"]["correlation"]) < 1e-10
+
+
+def test_driver_finite_size_correction_real_amplitudes(tmp_path):
+    """Real Coulomb integrals (halfGrid) give real amplitudes; the
+    FiniteSizeCorrection step takes them to the complex field
+    (FiniteSizeCorrection.cxx:103-117) and brings the vertex to a momentum
+    grid with singular vectors [G, F]; structure factor and energies match
+    the oracle."""
+    from tests.test_finite_size import _grid, _lattice
+    No, Nv, NF, N = 4, 12, 20, 3
+    print("finite-size correction, real amplitudes with singular vectors", flush=True)
+    inp = syn.generate(No, Nv, NF, "real", 31)
+    syn.write_cc4s_inputs(inp, tmp_path)
+    B = _lattice("bcc")
+    grid = _grid(B, 2)
+    NGg = grid.shape[0]
+    g2 = np.sum(grid * grid, axis=1)
+    V = np.where(g2 > 0, 4.0 * np.pi / np.where(g2 > 0, g2, 1.0), 2.5)
+    rng = np.random.default_rng(12)
+    U = (rng.standard_normal((NGg, NF)) + 1j * rng.standard_normal((NGg, NF))) / np.sqrt(NF)
+    syn.write_tensor(tmp_path, "GridVectors", grid.T, [(3, "Vector"), (NGg, "Momentum")],
+                     meta={"Gi": B[0].tolist(), "Gj": B[1].tolist(), "Gk": B[2].tolist()})
+    syn.write_tensor(tmp_path, "CoulombPotential", V, [(NGg, "Momentum")])
+    syn.write_tensor(tmp_path, "SingularVectors", U, [(NGg, "Momentum"), (NF, "AuxiliaryField")])
+    ops = STEPS + f"""
+- name: Read
+  in:
+    fileName: "GridVectors.yaml"
+  out:
+    destination: GridVectors
+- name: Read
+  in:
+    fileName: "CoulombPotential.yaml"
+  out:
+    destination: CoulombPotential
+- name: Read
+  in:
+    fileName: "SingularVectors.yaml"
+  out:
+    destination: SingularVectors
+- name: CoupledCluster
+  in:
+    method: Ccsd
+    slicedEigenEnergies: EigenEnergies
+    coulombIntegrals: CoulombIntegrals
+    slicedCoulombVertex: CoulombVertex
+    maxIterations: 4
+  out:
+    amplitudes: Amplitudes
+- name: FiniteSizeCorrection
+  in:
+    slicedCoulombVertex: CoulombVertex
+    amplitudes: Amplitudes
+    coulombVertexSingularVectors: SingularVectors
+    coulombPotential: CoulombPotential
+    gridVectors: GridVectors
+    interpolationGridSize: {N}
+  out:
+    transitionStructureFactor: SF
+- name: Write
+  in:
+    source: Amplitudes
+    fileName: "Amplitudes.yaml"
+  out: {{}}
+- name: Write
+  in:
+    source: SF
+    fileName: "SF.yaml"
+  out: {{}}
+"""
+    (tmp_path / "cc4s.in").write_text(ops)
+    r = subprocess.run([str(DRIVER), "-i", "cc4s.in"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=600)
+    print(r.stdout[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    rep = yaml.safe_load((tmp_path / "cc4s.out.yaml").read_text())
+    steps = [rep["steps"][k] for k in sorted(rep["steps"], key=int)]
+    fsc = next(s for s in steps if s["name"] == "FiniteSizeCorrection")
+    h = yaml.safe_load((tmp_path / "Amplitudes.yaml").read_text())["components"]
+    assert h["pphh"]["scalarType"] == "Real64"
+    T1 = read_tensor(tmp_path, "Amplitudes.components.ph", h["ph"]).astype(complex)
+    T2 = read_tensor(tmp_path, "Amplitudes.components.pphh", h["pphh"]).astype(complex)
+    S_ref = F.transition_structure_factor(inp.gamma, No, T1, T2, V, U)
+    S = read_tensor(tmp_path, "SF", yaml.safe_load((tmp_path / "SF.yaml").read_text()))
+    assert rel_err(S, S_ref) < 1e-12
+    corrected, uncorrected = F.finite_size_correction(grid, B, S_ref, N=N)
+    e = fsc["out"]["energy"]
+    assert abs(e["uncorrected"] - uncorrected) <= 1e-10 * abs(uncorrected)
+    assert abs(e["corrected"] - corrected) <= 1e-10 * abs(corrected)
