@@ -171,7 +171,7 @@ class Runner {
     } else if (name == "Write") {
       write(args, out_report);
     } else if (name == "PerturbativeTriplesReference" || name == "PerturbativeTriples") {
-      triples(args, results);
+      triples(args, results, name == "PerturbativeTriples");
     } else if (name == "FiniteSizeCorrection") {
       finite_size_correction(args, results);
     } else if (name == "SecondOrderPerturbationTheory") {
@@ -378,12 +378,19 @@ class Runner {
   // PerturbativeTriplesReference / PerturbativeTriples: the (T) energy of the
   // amplitudes (PerturbativeTriplesReference.cxx:55-180; both names give the
   // same energy, PerturbativeTriples.cxx being the distributed variant)
-  void triples(const NodePtr &args, std::map<std::string, ObjectPtr> &results) {
+  void triples(const NodePtr &args, std::map<std::string, ObjectPtr> &results, bool distributed) {
     auto ints = symbol(args, "coulombIntegrals");
     auto amps = symbol(args, "amplitudes");
     auto eps = symbol(args, "slicedEigenEnergies");
     if (ints->kind != Object::Kind::Integrals) throw std::runtime_error("expecting Coulomb integrals as 'coulombIntegrals'");
     if (yaml::get_integer(args, "cT", 0) != 0) throw std::runtime_error("cT (complete triples) is not supported");
+    // (T*) of PerturbativeTriplesStar.hpp:34-103: the (T) energy times the sum
+    // of the given MP2 pair energies over the finite-basis MP2 energy
+    auto pairs = symbol(args, "mp2PairEnergies", false);
+    if (args->has("mp2PairEnergies") && !pairs)
+      throw std::runtime_error("'mp2PairEnergies' does not name a stored object");
+    if (pairs && (pairs->kind != Object::Kind::Tensor || pairs->tensor->complex))
+      throw std::runtime_error("expecting a Real64 tensor as 'mp2PairEnergies'");
     auto vtx = ints->sliced_vertex;
     int64_t No, Nv;
     sliced_counts(eps, No, Nv);
@@ -396,7 +403,7 @@ class Runner {
       throw std::runtime_error("amplitudes do not match the field or the sizes");
     std::vector<double> compact, e;
     const double *gamma = step_inputs(vtx, eps, compact, e);
-    double et = 0.0;
+    double et = 0.0, mp2 = 0.0;
     if (!opt_.dry_run) {
       cc4s_ccsd_params p;
       cc4s_ccsd_default_params(&p);
@@ -416,6 +423,13 @@ class Runner {
       };
       check(cc4s_ccsd_create(&p, &ctx));
       check(cc4s_ccsd_set_inputs(ctx, gamma, e.data()));
+      if (pairs) {
+        // the first pass from no amplitudes: T2 = conj(V) / (e_i + e_j - e_a - e_b),
+        // its energy 2 sum M_abij V_abij - M_abij V_abji is the finite-basis MP2 energy
+        cc4s_ccsd_iteration it;
+        check(cc4s_ccsd_iterate(ctx, &it));
+        mp2 = it.energy;
+      }
       check(cc4s_ccsd_set_amplitudes(ctx, t1.data.data(), t2.data.data()));
       check(cc4s_ccsd_triples(ctx, &et));
       cc4s_ccsd_destroy(ctx);
@@ -423,7 +437,18 @@ class Runner {
     }
     auto energy = Node::map();
     energy->set("correlation", et);
+    auto star = [&]() {
+      double cbs = 0.0;
+      for (double x : pairs->tensor->data) cbs += x;
+      const double v = mp2 != 0.0 ? cbs / mp2 * et : 0.0;
+      std::printf("(T*)-Bsie energy correction: %23.15f\n", v);
+      return v;
+    };
+    // key order as the two steps write them (PerturbativeTriples.cxx:238-254,
+    // PerturbativeTriplesReference.cxx:201-213)
+    if (pairs && distributed) energy->set("starCorrection", star());
     energy->set("unit", eps->set->at("h").unit);
+    if (pairs && !distributed) energy->set("starCorrection", star());
     auto o = make_object(Object::Kind::Data);
     o->data = energy;
     results["energy"] = o;

@@ -623,6 +623,24 @@ def vertex_gradient(gamma, amplitudes, No, field):
 # PerturbativeTriplesReference (the (T) energy of converged amplitudes)
 # ---------------------------------------------------------------------------
 
+def mp2_pair_energies(P):
+    """MP2 pair energies of the finite basis as PerturbativeTriplesStar.hpp:
+    74-88 forms them: M_abij = conj(V_abij) / (e_i + e_j - e_a - e_b),
+    e_ij = sum_ab 2 M_abij V_abij - M_abij V_abji (complex, [i, j])."""
+    V = P.V["pphh"]
+    dph = P.eps_p[:, None] - P.eps_h[None, :]
+    M = np.conj(V) / -(dph[:, None, :, None] + dph[None, :, None, :])
+    return ein("abij,abij->ij", M, 2.0 * V - V.transpose(0, 1, 3, 2))
+
+
+def triples_star(P, e_triples, pair_energies_cbs):
+    """The (T*) correction of PerturbativeTriplesStar.hpp:34-103: the (T)
+    energy scaled by sum_ij of the given (basis-set limit) MP2 pair energies
+    over the finite-basis MP2 energy sum_ij e_ij."""
+    ratio = float(np.sum(pair_energies_cbs)) / float(np.real(np.sum(mp2_pair_energies(P))))
+    return ratio * e_triples
+
+
 def perturbative_triples(gamma, eps, No, field, amplitudes):
     """(T) correlation energy as PerturbativeTriplesReference.cxx:55-180
     forms it (cT off):

@@ -291,3 +291,59 @@ def test_second_order_open_shell_spin_orbitals(tmp_path):
     got = mp2_out(run(tmp_path, STEPS + MP2_STEP.format(fock="")))
     want = O.second_order(O.CcsdProblem(inp.gamma, inp.eps, No, "complex"))
     assert abs(got["correlation"] - want["correlation"]) < 1e-12 * abs(want["correlation"])
+
+
+@pytest.mark.parametrize("step", ["PerturbativeTriplesReference", "PerturbativeTriples"])
+def test_triples_star_correction(tmp_path, step):
+    """(T) with mp2PairEnergies on the rs = 1, No = 7, Nv = 26 electron gas:
+    given 1.25 times the finite-basis MP2 pair energies, starCorrection is
+    1.25 times the reference's recorded (T) energy (and the oracle's (T*))."""
+    from oracle import ueg
+    gamma, eps, fermi = ueg.generate(7, 26, 1.0)
+    print(f"{step} with mp2PairEnergies, UEG", flush=True)
+    inp = syn.CcsdInputs(gamma=gamma, eps=eps, No=7, field="complex", scale=1.0,
+                         fermi_energy=fermi)
+    syn.write_cc4s_inputs(inp, tmp_path)
+    P = O.CcsdProblem(gamma, eps, 7, "complex")
+    pairs = 1.25 * O.mp2_pair_energies(P).real
+    syn.write_tensor(tmp_path, "Mp2PairEnergies", pairs, [(7, "State"), (7, "State")])
+    s = ueg.UEG_RS1_NO7_NV26["settings"]
+    ops = STEPS + f"""
+- name: Read
+  in:
+    fileName: "Mp2PairEnergies.yaml"
+  out:
+    destination: Mp2PairEnergies
+- name: CoupledCluster
+  in:
+    method: Ccsd
+    slicedEigenEnergies: EigenEnergies
+    coulombIntegrals: CoulombIntegrals
+    slicedCoulombVertex: CoulombVertex
+    integralsSliceSize: 100
+    maxIterations: {s['maxIterations']}
+    energyConvergence: 1.0E-8
+    amplitudesConvergence: 1.0E-8
+    mixer:
+      type: DiisMixer
+      maxResidua: {s['maxResidua']}
+  out:
+    amplitudes: Amplitudes
+- name: {step}
+  in:
+    coulombIntegrals: CoulombIntegrals
+    amplitudes: Amplitudes
+    slicedEigenEnergies: EigenEnergies
+    mp2PairEnergies: Mp2PairEnergies
+  out:
+    {{}}
+"""
+    rep = run(tmp_path, ops)
+    e = rep["steps"][7]["out"]["energy"]
+    ref_t = ueg.UEG_RS1_NO7_NV26["Triples"]["correlation"]
+    assert abs(e["correlation"] - ref_t) < 1e-10
+    assert abs(e["starCorrection"] - 1.25 * ref_t) < 1e-10
+    assert abs(e["starCorrection"] - O.triples_star(P, e["correlation"], pairs)) < 1e-13
+    keys = list(e)
+    assert keys.index("starCorrection") < keys.index("unit") if step == "PerturbativeTriples" \
+        else keys.index("starCorrection") > keys.index("unit")

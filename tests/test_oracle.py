@@ -292,3 +292,21 @@ def test_second_order_degeneracy_by_spin_orbitals():
         so = O.second_order(Pso, fph=fs, degeneracy=1.0)
         assert abs(so["correlation"] - closed["correlation"]) < 1e-13 * abs(closed["correlation"])
         assert abs(so["singles"] - closed["singles"]) < 1e-15 + 1e-13 * abs(closed["singles"])
+
+
+@pytest.mark.parametrize("field", ["complex", "real"])
+def test_mp2_pair_energies_and_triples_star(field):
+    """The finite-basis MP2 pair energies of PerturbativeTriplesStar.hpp sum to
+    the second-order energy of SecondOrderPerturbationTheory (another
+    denominator sign convention and the exchange read as V_abji), so given
+    those pair energies the (T*) correction is the (T) energy itself, and it
+    scales with the given pair energies."""
+    No, Nv, NG = 3, 8, 10
+    inp = syn.generate(No, Nv, NG, field, seed=29)
+    P = O.CcsdProblem(inp.gamma, inp.eps, No, field)
+    e = O.mp2_pair_energies(P)
+    assert e.shape == (No, No)
+    assert abs(np.sum(e).real - O.mp2_energy(P)) < 1e-14 * abs(O.mp2_energy(P))
+    assert abs(np.sum(e).imag) < 1e-14 * abs(O.mp2_energy(P))
+    assert O.triples_star(P, -0.0123, e.real) == pytest.approx(-0.0123, rel=1e-14)
+    assert O.triples_star(P, -0.0123, 1.25 * e.real) == pytest.approx(1.25 * -0.0123, rel=1e-14)
