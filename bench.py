@@ -11,12 +11,13 @@ with the Coulomb vertex already resident in HBM.  The FP64 context does W
 untimed passes (pass 1 has no residuum contraction: R = V), then K timed
 passes: `value` is their mean wall time.  With --converge the FP64 context
 keeps iterating to the convergence criteria, and one further context per
-mixed switch value runs the mixed-precision path (split-bf16 contractions
-until dR < switch, then FP64) to the same criteria; every time to
-convergence, its ratio to the FP64 one and |E_mixed - E_FP64| are reported.
-The first switch value is the workload's written 1e-5; the second (1e-6)
-was the fastest of 1e-5, 1e-6 and 1e-7 at No=32, Nv=256 (the split-bf16
-residuum's rounding floor is dR ~1e-8).  At the FP64 fixed point of the
+mixed switch value and refinement mode runs the mixed-precision path (split
+16-bit contractions until dR < switch, an FP64 anchor pass, split passes
+corrected by its defect; refinement 1 lets an FP64 pass declare convergence,
+3 the corrected pass that meets the criteria) to the same criteria; every
+time to convergence, its ratio to the FP64 one and |E_mixed - E_FP64| are
+reported.  The first switch value is the workload's written 1e-5, the
+second 1e-6.  At the FP64 fixed point of the
 No=32, Nv=256, NG=1500 workload the energy adjoint (gradient with respect to the vertex and the
 structure factor S(G), FP64 and FP32) is timed and checked against central
 differences of the forward energy.  Prints one JSON line.
@@ -254,10 +255,11 @@ def main():
                          "when the switch comes at dR < 2e-6)")
     ap.add_argument("--mixed-restart-keep", default="0",
                     help="comma-separated numbers of recent DIIS pairs a restart keeps")
-    ap.add_argument("--mixed-refine", default=None,
+    ap.add_argument("--mixed-refine", default="1,3",
                     help="comma-separated refinement modes after the split passes "
                          "(0 FP64 passes, 1/2 FP64 anchor + defect-corrected split "
-                         "passes); default: the library's")
+                         "passes + FP64 passes, 3 the same, a corrected pass may "
+                         "declare convergence)")
     ap.add_argument("--mixed-split", default=None, choices=("bf16", "half"),
                     help="split format of the mixed passes (default: the library's)")
     ap.add_argument("--mixed-max-k", type=int, default=None,

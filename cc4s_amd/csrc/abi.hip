@@ -102,8 +102,8 @@ int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
     CC4S_ASSERT(params->mixed_restart_keep >= 0 && params->mixed_restart_keep < params->max_residua,
                 "mixed_restart_keep must lie in [0, maxResidua)");
     p.mixed_restart_keep = params->mixed_restart_keep;
-    CC4S_ASSERT(params->mixed_refine >= 0 && params->mixed_refine <= 2,
-                "mixed_refine must be 0, 1 or 2");
+    CC4S_ASSERT(params->mixed_refine >= 0 && params->mixed_refine <= 3,
+                "mixed_refine must be 0, 1, 2 or 3");
     p.mixed_refine = params->mixed_refine;
     CC4S_ASSERT(params->mixed_split == 0 || params->mixed_split == 1,
                 "mixed_split must be 0 (bf16) or 1 (half)");

@@ -518,6 +518,10 @@ IterationReport CoupledCluster::iterate() {
       // the noise of the split residuum
       const bool meets = std::abs(rep.energy - previous_e_) < p_.energy_convergence &&
                          rep.residuum_norm < p_.amplitudes_convergence;
+      if (meets && p_.mixed_refine == 3) {
+        rep.converged = true;
+        break;
+      }
       if (split_stalled(rep.residuum_norm) || meets) {
         phase_ = Phase::Fp64;
         if (p_.mixed_refine == 2) mixer_->restart(0);

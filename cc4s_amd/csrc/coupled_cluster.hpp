@@ -39,7 +39,9 @@ struct CoupledClusterParams {
   // that also evaluates the split residuum at the same amplitudes, giving
   // its defect d = R64(T) - Rsplit(T) (the DIIS history restarts there), then
   // split passes with Rsplit + d until they stop gaining, then FP64 passes
-  // (2 restarts the DIIS history once more before them, 1 keeps it)
+  // (2 restarts the DIIS history once more before them, 1 keeps it; 3 as 1,
+  // but a corrected pass that meets the convergence test ends the run: the
+  // defect, measured in FP64 at the anchor, stands in for the FP64 pass)
   int mixed_refine = 1;
   bool mixed_half = true;         // split parts in power-of-two scaled IEEE half
   int64_t mixed_max_k = 2048;     // longest FP32 sum of a split contraction

@@ -53,9 +53,10 @@ enum { CC4S_FIELD_REAL = 0, CC4S_FIELD_COMPLEX = 1 };
  * mixed_refine (default), one FP64 anchor pass also evaluates the split
  * residuum at the same amplitudes; its defect d = R64 - Rsplit corrects the
  * split passes that follow until they meet the convergence test or stall,
- * and FP64 passes confirm; without it FP64 passes follow directly and
+ * and FP64 passes confirm (mixed_refine 3: a corrected pass that meets the
+ * convergence test ends the run); without it FP64 passes follow directly and
  * mixed_restart decides whether the DIIS history restarts at the switch.
- * Only FP64 passes can declare convergence.  Denominators, energy, DIIS and
+ * Otherwise only FP64 passes can declare convergence.  Denominators, energy, DIIS and
  * every small contraction stay FP64.  FP32 is accepted by the energy
  * gradient only. */
 enum { CC4S_PRECISION_FP64 = 0, CC4S_PRECISION_MIXED = 1, CC4S_PRECISION_FP32 = 2 };
@@ -90,7 +91,8 @@ typedef struct {
                                      at the same amplitudes (defect d = R64 - Rsplit, DIIS
                                      restarted), split passes with Rsplit + d until they stop
                                      gaining, then FP64 passes (2: DIIS restarted before
-                                     them, 1: kept) (1) */
+                                     them, 1: kept); 3 as 1, but a corrected pass
+                                     meeting the convergence test ends the run (1) */
   int mixed_split;                /* MIXED: split parts 0 bf16, 1 IEEE half with a
                                      power-of-two scale per operand (1) */
   int64_t mixed_max_k;            /* MIXED: longest FP32 sum of a split contraction; longer

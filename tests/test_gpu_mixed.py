@@ -150,12 +150,13 @@ def test_restart_keeping_recent_pairs(placement, monkeypatch):
     assert abs(mixed["energy"]["correlation"] - fp64["energy"]["correlation"]) < 1e-8
 
 
-@pytest.mark.parametrize("refine", [1, 2])
+@pytest.mark.parametrize("refine", [1, 2, 3])
 def test_defect_corrected_refinement(refine):
     """After the split passes, one FP64 anchor pass measures the defect of
     the split residuum; split passes corrected by it continue, then FP64
     passes finish.  The run reaches the FP64 energy, the passes come in that
-    order, and only FP64 passes (anchor or plain) declare convergence."""
+    order, and only FP64 passes (anchor or plain) declare convergence; with
+    mode 3 a corrected pass may end the run."""
     No, Nv, NG = 12, 60, 300
     print(f"defect-corrected refinement, mode {refine}", flush=True)
     inp = syn.generate(No, Nv, NG, "complex", 42, scale_rule="mp2_per_occupied")
@@ -177,7 +178,10 @@ def test_defect_corrected_refinement(refine):
     assert abs(mixed["energy"]["correlation"] - fp64["energy"]["correlation"]) < 1e-8
     import re
     assert re.fullmatch(r"S+AC*F*", kinds), kinds
-    assert all(t["pass_kind"] in (0, 2) for t in mixed["trace"] if t["converged"])
+    declaring = (0, 2, 3) if refine == 3 else (0, 2)
+    assert all(t["pass_kind"] in declaring for t in mixed["trace"] if t["converged"])
+    if refine == 3:
+        assert mixed["trace"][-1]["converged"]
 
 
 @pytest.mark.parametrize("opts", [dict(method="Drccd"), dict(levelShift=0.3)],
