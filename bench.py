@@ -1,7 +1,8 @@
 """Benchmark: seconds per CCSD iteration on one MI355X, and mixed vs FP64
 time to convergence.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+                [--workload complex_no32_nv256_ng1500 | --config I]
                 [--scale mp2_per_occupied|max_v] [--converge both|fp64|mixed|none]
                 [--mixed-switch 1e-5,1e-7] [--no-cpu-baseline] [--adjoint auto|on|off]
 
@@ -23,8 +24,12 @@ structure factor S(G), FP64 and FP32) is timed and checked against central
 differences of the forward energy.  Prints one JSON line.
 
 Input scale: "mp2_per_occupied" (default) scales the vertex so that the MP2
-energy per occupied orbital equals that of the config-0 workload; "max_v" is
-max|V_abij| = 0.05 Eh, for which the iteration diverges beyond config 0.
+energy per occupied orbital equals that of the real No=8, Nv=40, NG=200
+workload; "max_v" is max|V_abij| = 0.05 Eh, for which the iteration diverges
+at larger No, Nv.  --config I selects the I-th workload of
+synthetic.WORKLOADS (0 real No=8 Nv=40 NG=200, 1 complex No=32 Nv=256
+NG=1500, 2 complex No=64 Nv=512 NG=3000, 3 the adjoint workload, 4 complex
+No=64 Nv=640 NG=4000).
 Per-pass cost does not depend on the scale.
 """
 
@@ -241,7 +246,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--workload", default="complex_no32_nv256_ng1500",
+                    choices=tuple(syn.WORKLOADS))
+    ap.add_argument("--config", type=int, default=None,
+                    help="index into the workload list (overrides --workload)")
     ap.add_argument("--scale", default="mp2_per_occupied",
                     choices=("mp2_per_occupied", "max_v"))
     ap.add_argument("--converge", default="both",
@@ -287,7 +295,9 @@ def main():
                if args.mixed_refine is not None else [None])
     if args.steps < 1 and args.converge != "none":
         ap.error("--steps must be at least 1 unless --converge none")
-    cfg = syn.CONFIGS[args.config]
+    wname = (list(syn.WORKLOADS)[args.config] if args.config is not None
+             else args.workload)
+    cfg = syn.WORKLOADS[wname]
     log(f"generating inputs: {cfg.name}, scale rule {args.scale}")
     t = time.perf_counter()
     big = cfg.No * cfg.Nv > 16384
@@ -339,7 +349,8 @@ def main():
                 "converged": bool(passes[-1]["converged"]),
             }
             want = args.adjoint == "on" or (args.adjoint == "auto"
-                                            and args.config in (1, 3))
+                                            and wname in ("complex_no32_nv256_ng1500",
+                                                          "adjoint_no32_nv256_ng1500"))
             if want and passes[-1]["converged"]:
                 adjoint = adjoint_report(cc, inp.gamma, passes[-1]["energy"])
             if args.triples:

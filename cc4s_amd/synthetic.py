@@ -102,16 +102,18 @@ class CcsdConfig:
         return self.No + self.Nv
 
 
-CONFIGS = {
-    0: CcsdConfig("real No=8 Nv=40 NG=200", 8, 40, 200, "real", seed=100),
-    1: CcsdConfig("complex No=32 Nv=256 NG=1500", 32, 256, 1500, "complex",
-                  seed=101),
-    2: CcsdConfig("complex No=64 Nv=512 NG=3000", 64, 512, 3000, "complex",
-                  seed=102),
-    3: CcsdConfig("complex No=32 Nv=256 NG=1500 (adjoint)", 32, 256, 1500,
-                  "complex", seed=101),
-    4: CcsdConfig("complex No=64 Nv=640 NG=4000", 64, 640, 4000, "complex",
-                  seed=104, max_residua=6),
+# the workloads, by name (insertion order is the order of bench.py --config)
+WORKLOADS = {
+    "real_no8_nv40_ng200": CcsdConfig("real No=8 Nv=40 NG=200", 8, 40, 200, "real",
+                                      seed=100),
+    "complex_no32_nv256_ng1500": CcsdConfig("complex No=32 Nv=256 NG=1500", 32, 256,
+                                            1500, "complex", seed=101),
+    "complex_no64_nv512_ng3000": CcsdConfig("complex No=64 Nv=512 NG=3000", 64, 512,
+                                            3000, "complex", seed=102),
+    "adjoint_no32_nv256_ng1500": CcsdConfig("complex No=32 Nv=256 NG=1500 (adjoint)",
+                                            32, 256, 1500, "complex", seed=101),
+    "complex_no64_nv640_ng4000": CcsdConfig("complex No=64 Nv=640 NG=4000", 64, 640,
+                                            4000, "complex", seed=104, max_residua=6),
 }
 
 
@@ -203,8 +205,8 @@ def mp2_of_vertex(gamma: np.ndarray, eps: np.ndarray, No: int,
     return float(np.real(-2.0 * np.sum(V * T) + np.sum(V.transpose(0, 1, 3, 2) * T)))
 
 
-# MP2 energy per occupied orbital of the config-0 workload (No=8, Nv=40,
-# NG=200, seed 100): -0.5527070232566246 / 8
+# MP2 energy per occupied orbital of the real No=8, Nv=40, NG=200 workload
+# (seed 100): -0.5527070232566246 / 8
 MP2_PER_OCCUPIED = -0.5527070232566246 / 8
 
 
@@ -248,8 +250,8 @@ def generate(No: int, Nv: int, NG: int, field: str, seed: int,
 
     scale_rule "max_v" (the workload definition): c such that
     max|V_abij| = target_max_v.  scale_rule "mp2_per_occupied": c such that
-    the second-order energy per occupied orbital equals that of the config-0
-    workload (E2 is proportional to c^4); at No, Nv beyond config 0 the max_v
+    the second-order energy per occupied orbital equals that of the real No=8,
+    Nv=40, NG=200 workload (E2 is proportional to c^4); at larger No, Nv the max_v
     rule gives couplings for which the CCSD iteration does not converge, this
     rule keeps the correlation strength per electron fixed.  `mp2_fn(gamma,
     eps, No, field)` may replace the host computation of the second-order

@@ -1,12 +1,12 @@
 """Writes the golden fixtures of tests/golden from the FP64 oracle.
 
-config0.json  config-0 workload (real, No=8, Nv=40, NG=200): SHA-256 of the
+real_no8_nv40_ng200.json  the real No=8, Nv=40, NG=200 workload: SHA-256 of the
               generated inputs, the oracle's per-iteration (E, dE, dR) trace,
               final energies and checksums of the converged amplitudes.
 small_complex.json  a small complex workload (No=6, Nv=24, NG=60) run the
               same way, plus the residuum at seeded amplitudes.
 
-Usage: python tests/golden/make_golden.py [config0|small_complex|config1 ...]
+Usage: python tests/golden/make_golden.py [real_no8_nv40_ng200|small_complex ...]
 """
 
 from __future__ import annotations
@@ -83,10 +83,11 @@ def residuum_fixture(P, inp):
 
 
 def main(which):
-    if "config0" in which:
-        data = run_config(syn.CONFIGS[0], residuum_fixture)
-        (HERE / "config0.json").write_text(json.dumps(data, indent=1))
-        print("config0", data["energy"], data["iterations"], data["oracle_seconds"])
+    if "real_no8_nv40_ng200" in which:
+        data = run_config(syn.WORKLOADS["real_no8_nv40_ng200"], residuum_fixture)
+        (HERE / "real_no8_nv40_ng200.json").write_text(json.dumps(data, indent=1))
+        print("real_no8_nv40_ng200", data["energy"], data["iterations"],
+              data["oracle_seconds"])
     if "small_complex" in which:
         cfg = syn.CcsdConfig("complex No=6 Nv=24 NG=60", 6, 24, 60, "complex",
                              seed=42)
@@ -97,4 +98,4 @@ def main(which):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["config0", "small_complex"])
+    main(sys.argv[1:] or ["real_no8_nv40_ng200", "small_complex"])

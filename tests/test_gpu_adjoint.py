@@ -176,7 +176,7 @@ def test_adjoint_at_converged_amplitudes_no32_nv256():
     oracle's host gradient, and S(G) against central differences of the
     forward energy (relative step 1e-5 of one G slice) at 20 seeded G, to
     1e-7 of max |S|."""
-    cfg = syn.CONFIGS[3]
+    cfg = syn.WORKLOADS["adjoint_no32_nv256_ng1500"]
     print("generating No=32 Nv=256 NG=1500", flush=True)
     inp = syn.generate_config(cfg, scale_rule="mp2_per_occupied",
                               mp2_fn=None)

@@ -135,17 +135,17 @@ def _check_run(gpu, T_gpu, ref):
     assert rel_err(T_gpu[1], ref["amplitudes"][1]) < T_TOL
 
 
-def test_run_parity_config0():
-    """Config 0 (real, No=8, Nv=40, NG=200) to convergence, DIIS 8."""
-    cfg = syn.CONFIGS[0]
-    print("run parity config 0", flush=True)
+def test_run_parity_real_no8_nv40_ng200():
+    """The real No=8, Nv=40, NG=200 workload to convergence, DIIS 8."""
+    cfg = syn.WORKLOADS["real_no8_nv40_ng200"]
+    print("run parity, real No=8 Nv=40 NG=200", flush=True)
     kw = dict(maxIterations=cfg.max_iterations,
               energyConvergence=cfg.energy_convergence,
               amplitudesConvergence=cfg.amplitudes_convergence,
               maxResidua=cfg.max_residua)
     gpu, T_gpu, ref = _run_both(cfg.No, cfg.Nv, cfg.NG, cfg.field, cfg.seed, **kw)
     _check_run(gpu, T_gpu, ref)
-    gold = json.loads((GOLDEN / "config0.json").read_text())
+    gold = json.loads((GOLDEN / "real_no8_nv40_ng200.json").read_text())
     assert gpu["iterations"] == gold["iterations"]
     assert abs(gpu["energy"]["correlation"] - gold["energy"]["correlation"]) < E_TOL
     for g, r in zip(gpu["trace"], gold["trace"]):
@@ -201,11 +201,11 @@ def test_zero_iterations_without_amplitudes_fails():
 
 
 def test_config1_properties():
-    """Config-1 shape (complex, No=32, Nv=256, NG=1500): properties that do
+    """The complex No=32, Nv=256, NG=1500 workload: properties that do
     not need the oracle at this size.  The residuum of symmetric amplitudes
     is pair-symmetric (R_abij = R_baji), independent of the ladder slice
     size, and its first iteration reproduces the MP2 energy."""
-    cfg = syn.CONFIGS[1]
+    cfg = syn.WORKLOADS["complex_no32_nv256_ng1500"]
     print("config1 properties: generating inputs", flush=True)
     inp = syn.generate_config(cfg)
     T = _amps(cfg.No, cfg.Nv, "complex", 5, scale=0.01)

@@ -69,8 +69,9 @@ def amplitudes(directory, name="Amplitudes"):
             read_tensor(directory, f"{name}.components.pphh", c["pphh"]))
 
 
-@pytest.mark.parametrize("golden,binary", [("config0", True), ("small_complex", False)],
-                         ids=["config0_real_binary", "small_complex_text"])
+@pytest.mark.parametrize("golden,binary", [("real_no8_nv40_ng200", True),
+                                           ("small_complex", False)],
+                         ids=["real_no8_nv40_ng200_binary", "small_complex_text"])
 def test_operation_file_matches_golden_and_oracle(tmp_path, golden, binary):
     gold = json.loads((GOLDEN / f"{golden}.json").read_text())
     c = gold["config"]

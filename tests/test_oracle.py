@@ -179,13 +179,13 @@ def test_define_holes_and_particles():
         O.define_holes_and_particles(eps, 2.0)
 
 
-def test_golden_config0_generator_bytes():
-    """SHA-256 of the config-0 input bytes (cc4s layout) is fixed."""
-    cfg = syn.CONFIGS[0]
+def test_golden_real_workload_generator_bytes():
+    """SHA-256 of the real No=8, Nv=40, NG=200 input bytes (cc4s layout) is fixed."""
+    cfg = syn.WORKLOADS["real_no8_nv40_ng200"]
     inp = syn.generate_config(cfg)
     import json, pathlib
     gold = json.loads((pathlib.Path(__file__).parent / "golden" /
-                       "config0.json").read_text())
+                       "real_no8_nv40_ng200.json").read_text())
     h = hashlib.sha256(np.asfortranarray(inp.gamma).tobytes(order="F")).hexdigest()
     he = hashlib.sha256(inp.eps.tobytes()).hexdigest()
     assert h == gold["gamma_sha256"]
