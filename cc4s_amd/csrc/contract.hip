@@ -807,6 +807,18 @@ void ContractionEngine::apply_env_overrides() {
   }
 }
 
+// a timing event from the pool of recorded and read ones, else a new one
+hipEvent_t ContractionEngine::take_event() {
+  hipEvent_t e;
+  if (!event_pool_.empty()) {
+    e = event_pool_.back();
+    event_pool_.pop_back();
+  } else {
+    CC4S_HIP(hipEventCreate(&e));
+  }
+  return e;
+}
+
 ContractionEngine::~ContractionEngine() {
   for (auto &p : pending_) {
     (void)hipEventDestroy(p.start);
@@ -1005,17 +1017,7 @@ void ContractionEngine::contract(double alpha, const View &A, const View &B,
     return;
   }
   if (profiling_) {
-    auto take = [this]() {
-      hipEvent_t e;
-      if (!event_pool_.empty()) {
-        e = event_pool_.back();
-        event_pool_.pop_back();
-      } else {
-        CC4S_HIP(hipEventCreate(&e));
-      }
-      return e;
-    };
-    Pending p{take(), take(), gett_variant(L), flops};
+    Pending p{take_event(), take_event(), gett_variant(L), flops};
     launch_gett(L, stream_, p.start, p.stop);
     pending_.push_back(p);
     if (pending_.size() > 4096) (void)timings();
@@ -1103,8 +1105,8 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (profiling_) {
-    CC4S_HIP(hipEventCreate(&e0));
-    CC4S_HIP(hipEventCreate(&e1));
+    e0 = take_event();
+    e1 = take_event();
     CC4S_HIP(hipEventRecord(e0, stream_));
   }
   for (int64_t c = 0; c < chunks; ++c) {
@@ -1132,6 +1134,7 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
   if (profiling_) {
     CC4S_HIP(hipEventRecord(e1, stream_));
     pending_.push_back(Pending{e0, e1, "mixed_c64", flops});
+    if (pending_.size() > 4096) (void)timings();
   }
 }
 

@@ -107,6 +107,7 @@ class ContractionEngine {
   bool profiling_ = false;
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> event_pool_;
+  hipEvent_t take_event();
   std::unordered_map<std::string, KernelTiming> timings_;
   hipStream_t stream_;
   std::unordered_map<std::string, std::unique_ptr<GettPlan>> plans_;
