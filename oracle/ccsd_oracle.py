@@ -31,6 +31,9 @@ binary, which needs CTF and yaml-cpp that are not available offline):
 * ``oracle/ccsd_vbased.py`` restates CcsdReference.cxx:36-239, an independent
   formulation from explicit V slices (including V_pppp); tests require the two
   residua to agree to 1e-12 for real and complex inputs;
+* textbook spin-orbital CCSD (Stanton, Gauss, Watts and Bartlett 1991) on the
+  spin-orbital restatement of a closed-shell system converges to the same
+  energy (1e-12 Eh) and amplitudes;
 * the first iteration reproduces the MP2 closed form
   (SecondOrderPerturbationTheory.cxx:108-127); that second-order energy's
   singles term equals the second derivative of the exact ground-state energy

@@ -1,0 +1,153 @@
+"""The closed-shell CCSD of the oracle against textbook spin-orbital CCSD.
+
+The spin-orbital equations are the ones of J. F. Stanton, J. Gauss, J. D.
+Watts and R. J. Bartlett, J. Chem. Phys. 94, 4334 (1991), in the form given by
+T. D. Crawford and H. F. Schaefer III, Rev. Comput. Chem. 14, 33 (2000), with
+canonical orbitals (the Fock matrix is diag(e)).  They share nothing with the
+Hirata-style closed-shell factorisation of Ccsd.cxx:76-633 that the oracle
+restates: antisymmetrised spin-orbital integrals, other intermediates, other
+index bookkeeping.  Spin adaptation is exact for a closed-shell reference, so
+the converged correlation energies must agree, and the spin-orbital amplitudes
+must map onto the closed-shell ones: t_{i a} = T1_ai for either spin,
+t_{i(up) j(down)}^{a(up) b(down)} = T2_abij, and the same-spin amplitude is
+T2_abij - T2_baij.
+
+Physicist-notation integrals <pq|rs> follow from the vertex as the oracle's V
+slices do (VertexCoulombIntegrals.cxx:68-102,143-171): complex field
+<pq|rs> = sum_G conj(Gamma[G,r,p]) Gamma[G,q,s]; real field
+<pq|rs> = sum_G Re Gamma[G,p,r] Re Gamma[G,q,s] + Im Gamma[G,p,r] Im Gamma[G,q,s].
+"""
+
+import numpy as np
+import pytest
+
+from cc4s_amd import synthetic as syn
+from oracle import ccsd_oracle as O
+
+
+def physicist_integrals(gamma, field):
+    if field == "complex":
+        return np.einsum("Grp,Gqs->pqrs", np.conj(gamma), gamma)
+    return (np.einsum("Gpr,Gqs->pqrs", gamma.real, gamma.real)
+            + np.einsum("Gpr,Gqs->pqrs", gamma.imag, gamma.imag))
+
+
+def spin_orbital_integrals(g):
+    """<PQ||RS> over spin orbitals P = 2 p + spin (occupied ones first)."""
+    n = g.shape[0]
+    spin = np.arange(2 * n) % 2
+    sp = np.arange(2 * n) // 2
+    G = g[np.ix_(sp, sp, sp, sp)]
+    same = spin[:, None] == spin[None, :]
+    G = G * same[:, None, :, None] * same[None, :, None, :]
+    return G - G.transpose(0, 1, 3, 2)
+
+
+def spin_orbital_ccsd(A, e, nocc, tol=1e-13, max_iter=400):
+    """Canonical spin-orbital CCSD (Crawford and Schaefer, eqs. for T1, T2 and
+    the intermediates F and W); Jacobi updates with a plain DIIS on the
+    amplitude changes.  Returns (energy, t1[i,a], t2[i,j,a,b])."""
+    o, v = slice(0, nocc), slice(nocc, len(e))
+    eo, ev = e[o], e[v]
+    Dia = eo[:, None] - ev[None, :]
+    Dijab = eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] \
+        - ev[None, None, None, :]
+    oovv, vvoo = A[o, o, v, v], A[v, v, o, o]
+    ovvv, vovv, oovo = A[o, v, v, v], A[v, o, v, v], A[o, o, v, o]
+    ovov, ovvo, oooo, vvvv = A[o, v, o, v], A[o, v, v, o], A[o, o, o, o], A[v, v, v, v]
+    ooov, vvvo, ovoo = A[o, o, o, v], A[v, v, v, o], A[o, v, o, o]
+    t1 = np.zeros_like(Dia, dtype=A.dtype)
+    t2 = np.transpose(vvoo, (2, 3, 0, 1)) / Dijab
+    ein = lambda s, *x: np.einsum(s, *x, optimize=True)
+
+    def energy(t1, t2):
+        return 0.25 * ein("ijab,ijab", oovv, t2) + 0.5 * ein("ijab,ia,jb", oovv, t1, t1)
+
+    def update(t1, t2):
+        tau = t2 + ein("ia,jb->ijab", t1, t1) - ein("ib,ja->ijab", t1, t1)
+        taut = t2 + 0.5 * (ein("ia,jb->ijab", t1, t1) - ein("ib,ja->ijab", t1, t1))
+        Fae = ein("mf,mafe->ae", t1, A[o, v, v, v]) - 0.5 * ein("mnaf,mnef->ae", taut, oovv)
+        Fmi = ein("ne,mnie->mi", t1, ooov) + 0.5 * ein("inef,mnef->mi", taut, oovv)
+        Fme = ein("nf,mnef->me", t1, oovv)
+        Wmnij = oooo + ein("je,mnie->mnij", t1, ooov) - ein("ie,mnje->mnij", t1, ooov) \
+            + 0.25 * ein("ijef,mnef->mnij", tau, oovv)
+        Wabef = vvvv - ein("mb,amef->abef", t1, vovv) + ein("ma,bmef->abef", t1, vovv) \
+            + 0.25 * ein("mnab,mnef->abef", tau, oovv)
+        Wmbej = ovvo + ein("jf,mbef->mbej", t1, ovvv) - ein("nb,mnej->mbej", t1, oovo) \
+            - ein("jnfb,mnef->mbej", 0.5 * t2 + ein("jf,nb->jnfb", t1, t1), oovv)
+        r1 = ein("ie,ae->ia", t1, Fae) - ein("ma,mi->ia", t1, Fmi) \
+            + ein("imae,me->ia", t2, Fme) - ein("nf,naif->ia", t1, ovov) \
+            - 0.5 * ein("imef,maef->ia", t2, ovvv) - 0.5 * ein("mnae,nmei->ia", t2, oovo)
+        Fbe = Fae - 0.5 * ein("mb,me->be", t1, Fme)
+        Fmj = Fmi + 0.5 * ein("je,me->mj", t1, Fme)
+        x = ein("ijae,be->ijab", t2, Fbe)
+        r2 = np.transpose(vvoo, (2, 3, 0, 1)).copy()
+        r2 += x - x.transpose(0, 1, 3, 2)
+        x = ein("imab,mj->ijab", t2, Fmj)
+        r2 -= x - x.transpose(1, 0, 2, 3)
+        r2 += 0.5 * ein("mnab,mnij->ijab", tau, Wmnij)
+        r2 += 0.5 * ein("ijef,abef->ijab", tau, Wabef)
+        x = ein("imae,mbej->ijab", t2, Wmbej) - ein("ie,ma,mbej->ijab", t1, t1, ovvo)
+        r2 += x - x.transpose(1, 0, 2, 3) - x.transpose(0, 1, 3, 2) + x.transpose(1, 0, 3, 2)
+        x = ein("ie,abej->ijab", t1, vvvo)
+        r2 += x - x.transpose(1, 0, 2, 3)
+        x = ein("ma,mbij->ijab", t1, ovoo)
+        r2 -= x - x.transpose(0, 1, 3, 2)
+        return r1 / Dia, r2 / Dijab
+
+    hist_t, hist_r = [], []
+    e_old = energy(t1, t2)
+    for _ in range(max_iter):
+        n1, n2 = update(t1, t2)
+        r = np.concatenate([(n1 - t1).ravel(), (n2 - t2).ravel()])
+        t = np.concatenate([n1.ravel(), n2.ravel()])
+        hist_t.append(t)
+        hist_r.append(r)
+        hist_t, hist_r = hist_t[-8:], hist_r[-8:]
+        k = len(hist_r)
+        B = -np.ones((k + 1, k + 1), dtype=complex)
+        B[0, 0] = 0
+        for p in range(k):
+            for q in range(k):
+                B[p + 1, q + 1] = np.vdot(hist_r[p], hist_r[q])
+        rhs = np.zeros(k + 1, dtype=complex)
+        rhs[0] = -1
+        c = np.linalg.solve(B, rhs)[1:]
+        t = sum(ci * ti for ci, ti in zip(c, hist_t))
+        if np.isrealobj(A):
+            t = t.real
+        t1 = t[:t1.size].reshape(t1.shape)
+        t2 = t[t1.size:].reshape(t2.shape)
+        e_new = energy(t1, t2)
+        if abs(e_new - e_old) < tol and np.max(np.abs(r)) < 1e-11:
+            return e_new, t1, t2
+        e_old = e_new
+    raise RuntimeError("spin-orbital CCSD did not converge")
+
+
+@pytest.mark.parametrize("field", ["complex", "real"])
+def test_closed_shell_ccsd_equals_spin_orbital_ccsd(field):
+    No, Nv, NG = 2, 4, 7
+    # strong coupling: CCSD lies 20 % above MP2, |T1| ~ 1e-2
+    inp = syn.generate(No, Nv, NG, field, seed=61, scale_rule="mp2_per_occupied")
+    P = O.CcsdProblem(inp.gamma, inp.eps, No, field)
+    ref = O.coupled_cluster(P, max_iterations=200, energy_convergence=1e-14,
+                            amplitudes_convergence=1e-12, max_residua=8)
+    assert ref["convergenceReached"]
+    T1, T2 = ref["amplitudes"]
+    g = physicist_integrals(inp.gamma, field)
+    A = spin_orbital_integrals(g)
+    e = np.repeat(np.asarray(inp.eps, dtype=np.float64), 2)
+    E, t1, t2 = spin_orbital_ccsd(A, e, 2 * No)
+    E_closed = ref["energy"]["correlation"]
+    assert abs(E.real - E_closed) < 1e-12, (E, E_closed)
+    # amplitudes: spin orbitals 2p (up) and 2p + 1 (down)
+    up_o, dn_o = np.arange(0, 2 * No, 2), np.arange(1, 2 * No, 2)
+    up_v, dn_v = np.arange(0, 2 * Nv, 2), np.arange(1, 2 * Nv, 2)
+    scale = np.max(np.abs(T2))
+    assert np.max(np.abs(t1[np.ix_(up_o, up_v)].T - T1)) < 1e-10 * scale
+    assert np.max(np.abs(t1[np.ix_(dn_o, dn_v)].T - T1)) < 1e-10 * scale
+    mixed = t2[np.ix_(up_o, dn_o, up_v, dn_v)].transpose(2, 3, 0, 1)      # [a,b,i,j]
+    assert np.max(np.abs(mixed - T2)) < 1e-10 * scale
+    same = t2[np.ix_(up_o, up_o, up_v, up_v)].transpose(2, 3, 0, 1)
+    assert np.max(np.abs(same - (T2 - T2.transpose(1, 0, 2, 3)))) < 1e-10 * scale
