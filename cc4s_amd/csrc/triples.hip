@@ -11,9 +11,10 @@
 //   Re(W_abc Zs) / D,  Zs = 8 S_abc - 4 (S_bac + S_cba + S_acb) + 2 (S_bca + S_cab)
 // (the sum of Z over the six orderings, written at the ordering i <= j <= k);
 // i <= j <= k runs once, weighted 1 for distinct holes, 1/2 for two equal,
-// 1/6 for three equal.  HBM-bound: W is read at six permutations per
-// element.  Each block keeps its running sum across launches in
-// `acc[block]` (one stream: a fixed order), so the energy is reproducible.
+// 1/6 for three equal.  Bandwidth-bound: W is read at six permutations of
+// each particle tile, all six from LDS.  Each block keeps its running sum
+// across launches in `acc[block]` (one stream: a fixed order), so the energy
+// is reproducible.
 #include "triples.hpp"
 
 namespace cc4s_amd {
@@ -47,6 +48,18 @@ struct TS<true> {
   __device__ static T plus(T a, T b) { return make_double2(a.x + b.x, a.y + b.y); }
 };
 
+// The particle triples are taken in tiles of kTile^3 (a, b, c); a block
+// loads the tile's W at all six orderings of its index ranges into LDS with
+// coalesced rows (x1 fastest), so every permuted read of W comes from LDS.
+constexpr int kTile = 8;
+constexpr int kTileElems = kTile * kTile * kTile;
+
+// ordering p = abc, bac, cba, acb, bca, cab: position d of (x1, x2, x3) holds
+// particle kOrder[p][d] of (a, b, c); kCoef[p] is its weight in Zs
+__constant__ int kOrder[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0},
+                                 {0, 2, 1}, {1, 2, 0}, {2, 0, 1}};
+__constant__ double kCoef[6] = {8.0, -4.0, -4.0, -4.0, 2.0, 2.0};
+
 template <bool CPLX>
 __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
     const typename TS<CPLX>::T *__restrict__ W, const typename TS<CPLX>::T *__restrict__ V,
@@ -55,33 +68,54 @@ __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
     int64_t k0, int64_t kc, double *acc) {
   using S = TS<CPLX>;
   using T = typename S::T;
+  __shared__ T tile[6][kTileElems];
+  __shared__ double sh[kBlock];
   const int64_t v = Nv, o = No;
-  const int64_t n = v * v * v * kc;
+  const int64_t nt = (v + kTile - 1) / kTile;
+  const int64_t ntiles = nt * nt * nt * kc;
   // V_pphh[p,q,s,r] at p + v (q + v (s + o r)); T1[a,i] at a + v i
   auto Vx = [&](int64_t p, int64_t q, int64_t s, int64_t r) {
     return V[p + v * (q + v * (s + o * r))];
   };
+  T zero;
+  if constexpr (CPLX) zero = make_double2(0.0, 0.0); else zero = 0.0;
   double sum = 0.0;
-  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
-       e += int64_t(gridDim.x) * kBlock) {
-    const int64_t a = e % v, b = (e / v) % v, c = (e / (v * v)) % v, kl = e / (v * v * v);
-    const int64_t k = k0 + kl;
+  for (int64_t tid = blockIdx.x; tid < ntiles; tid += gridDim.x) {
+    const int64_t r0[3] = {(tid % nt) * kTile, ((tid / nt) % nt) * kTile,
+                           ((tid / (nt * nt)) % nt) * kTile};
+    const int64_t kl = tid / (nt * nt * nt);
     const int64_t base = v * v * v * kl;
-    // S at the particle tuple (x1, x2, x3) and the holes (i, j, k)
-    auto Sx = [&](int64_t x1, int64_t x2, int64_t x3) {
-      const T w = W[base + x1 + v * (x2 + v * x3)];
-      return S::plus(w, S::plus(S::mul(T1[x1 + v * i], Vx(x2, x3, j, k)),
-                                S::plus(S::mul(T1[x2 + v * j], Vx(x1, x3, i, k)),
-                                        S::mul(T1[x3 + v * k], Vx(x1, x2, i, j)))));
-    };
-    const T zs = S::lin(8.0, Sx(a, b, c),
-                        -4.0, S::plus(Sx(b, a, c), S::plus(Sx(c, b, a), Sx(a, c, b))),
-                        2.0, S::plus(Sx(b, c, a), Sx(c, a, b)));
+    __syncthreads();  // the previous tile's LDS reads are done
+    for (int e = threadIdx.x; e < 6 * kTileElems; e += kBlock) {
+      const int p = e / kTileElems, t = e % kTileElems;
+      const int64_t x1 = r0[kOrder[p][0]] + t % kTile;
+      const int64_t x2 = r0[kOrder[p][1]] + (t / kTile) % kTile;
+      const int64_t x3 = r0[kOrder[p][2]] + t / (kTile * kTile);
+      tile[p][t] = (x1 < v && x2 < v && x3 < v) ? W[base + x1 + v * (x2 + v * x3)] : zero;
+    }
+    __syncthreads();
+    const int64_t k = k0 + kl;
     const double weight = (i == j && j == k) ? 1.0 / 6.0 : (i == j || j == k) ? 0.5 : 1.0;
-    const double d = ((eh[i] + eh[j]) + eh[k]) - ((ep[a] + ep[b]) + ep[c]);
-    sum += weight * S::re_mul(W[e], zs) / d;
+    for (int t = threadIdx.x; t < kTileElems; t += kBlock) {
+      const int l[3] = {t % kTile, (t / kTile) % kTile, t / (kTile * kTile)};
+      const int64_t x[3] = {r0[0] + l[0], r0[1] + l[1], r0[2] + l[2]};
+      if (x[0] >= v || x[1] >= v || x[2] >= v) continue;
+      // S at the ordering (x1, x2, x3) and the holes (i, j, k)
+      T zs = zero;
+#pragma unroll
+      for (int p = 0; p < 6; ++p) {
+        const int q0 = kOrder[p][0], q1 = kOrder[p][1], q2 = kOrder[p][2];
+        const int64_t x1 = x[q0], x2 = x[q1], x3 = x[q2];
+        const T w = tile[p][l[q0] + kTile * (l[q1] + kTile * l[q2])];
+        const T sp = S::plus(w, S::plus(S::mul(T1[x1 + v * i], Vx(x2, x3, j, k)),
+                                        S::plus(S::mul(T1[x2 + v * j], Vx(x1, x3, i, k)),
+                                                S::mul(T1[x3 + v * k], Vx(x1, x2, i, j)))));
+        zs = S::lin(1.0, zs, kCoef[p], sp, 0.0, zero);
+      }
+      const double d = ((eh[i] + eh[j]) + eh[k]) - ((ep[x[0]] + ep[x[1]]) + ep[x[2]]);
+      sum += weight * S::re_mul(tile[0][t], zs) / d;
+    }
   }
-  __shared__ double sh[kBlock];
   sh[threadIdx.x] = sum;
   __syncthreads();
   for (int w = kBlock / 2; w > 0; w >>= 1) {
