@@ -11,8 +11,8 @@
 //   Re(W_abc Zs) / D,  Zs = 8 S_abc - 4 (S_bac + S_cba + S_acb) + 2 (S_bca + S_cab)
 // (the sum of Z over the six orderings, written at the ordering i <= j <= k);
 // i <= j <= k runs once, weighted 1 for distinct holes, 1/2 for two equal,
-// 1/6 for three equal.  Bandwidth-bound: W is read at six permutations of
-// each particle tile, all six from LDS.  Each block keeps its running sum
+// 1/6 for three equal.  Bandwidth-bound: S is formed at the six orderings
+// of each particle tile in LDS.  Each block keeps its running sum
 // across launches in `acc[block]` (one stream: a fixed order), so the energy
 // is reproducible.
 #include "triples.hpp"
@@ -49,8 +49,9 @@ struct TS<true> {
 };
 
 // The particle triples are taken in tiles of kTile^3 (a, b, c); a block
-// loads the tile's W at all six orderings of its index ranges into LDS with
-// coalesced rows (x1 fastest), so every permuted read of W comes from LDS.
+// forms S at all six orderings of the tile's index ranges in LDS, reading W,
+// V and T1 with x1 fastest (coalesced rows), so every permuted read of S
+// comes from LDS.
 constexpr int kTile = 8;
 constexpr int kTileElems = kTile * kTile * kTile;
 
@@ -68,7 +69,8 @@ __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
     int64_t k0, int64_t kc, double *acc) {
   using S = TS<CPLX>;
   using T = typename S::T;
-  __shared__ T tile[6][kTileElems];
+  // tiles 0-5: S at the six orderings of the tile's index ranges; tile 6: W
+  __shared__ T tile[7][kTileElems];
   __shared__ double sh[kBlock];
   const int64_t v = Nv, o = No;
   const int64_t nt = (v + kTile - 1) / kTile;
@@ -85,35 +87,43 @@ __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
                            ((tid / (nt * nt)) % nt) * kTile};
     const int64_t kl = tid / (nt * nt * nt);
     const int64_t base = v * v * v * kl;
+    const int64_t k = k0 + kl;
     __syncthreads();  // the previous tile's LDS reads are done
+    // S at the ordering (x1, x2, x3) and the holes (i, j, k): W plus the
+    // disconnected part; x1 runs fastest, so the V and T1 reads along x1 are
+    // coalesced and the others are broadcasts
     for (int e = threadIdx.x; e < 6 * kTileElems; e += kBlock) {
       const int p = e / kTileElems, t = e % kTileElems;
       const int64_t x1 = r0[kOrder[p][0]] + t % kTile;
       const int64_t x2 = r0[kOrder[p][1]] + (t / kTile) % kTile;
       const int64_t x3 = r0[kOrder[p][2]] + t / (kTile * kTile);
-      tile[p][t] = (x1 < v && x2 < v && x3 < v) ? W[base + x1 + v * (x2 + v * x3)] : zero;
+      T sv = zero;
+      if (x1 < v && x2 < v && x3 < v) {
+        const T w = W[base + x1 + v * (x2 + v * x3)];
+        if (p == 0) tile[6][t] = w;
+        sv = S::plus(w, S::plus(S::mul(T1[x1 + v * i], Vx(x2, x3, j, k)),
+                                S::plus(S::mul(T1[x2 + v * j], Vx(x1, x3, i, k)),
+                                        S::mul(T1[x3 + v * k], Vx(x1, x2, i, j)))));
+      } else if (p == 0) {
+        tile[6][t] = zero;
+      }
+      tile[p][t] = sv;
     }
     __syncthreads();
-    const int64_t k = k0 + kl;
     const double weight = (i == j && j == k) ? 1.0 / 6.0 : (i == j || j == k) ? 0.5 : 1.0;
     for (int t = threadIdx.x; t < kTileElems; t += kBlock) {
       const int l[3] = {t % kTile, (t / kTile) % kTile, t / (kTile * kTile)};
       const int64_t x[3] = {r0[0] + l[0], r0[1] + l[1], r0[2] + l[2]};
       if (x[0] >= v || x[1] >= v || x[2] >= v) continue;
-      // S at the ordering (x1, x2, x3) and the holes (i, j, k)
       T zs = zero;
 #pragma unroll
       for (int p = 0; p < 6; ++p) {
         const int q0 = kOrder[p][0], q1 = kOrder[p][1], q2 = kOrder[p][2];
-        const int64_t x1 = x[q0], x2 = x[q1], x3 = x[q2];
-        const T w = tile[p][l[q0] + kTile * (l[q1] + kTile * l[q2])];
-        const T sp = S::plus(w, S::plus(S::mul(T1[x1 + v * i], Vx(x2, x3, j, k)),
-                                        S::plus(S::mul(T1[x2 + v * j], Vx(x1, x3, i, k)),
-                                                S::mul(T1[x3 + v * k], Vx(x1, x2, i, j)))));
-        zs = S::lin(1.0, zs, kCoef[p], sp, 0.0, zero);
+        zs = S::lin(1.0, zs, kCoef[p], tile[p][l[q0] + kTile * (l[q1] + kTile * l[q2])],
+                    0.0, zero);
       }
       const double d = ((eh[i] + eh[j]) + eh[k]) - ((ep[x[0]] + ep[x[1]]) + ep[x[2]]);
-      sum += weight * S::re_mul(tile[0][t], zs) / d;
+      sum += weight * S::re_mul(tile[6][t], zs) / d;
     }
   }
   sh[threadIdx.x] = sum;
