@@ -11,10 +11,10 @@
 //   Re(W_abc Zs) / D,  Zs = 8 S_abc - 4 (S_bac + S_cba + S_acb) + 2 (S_bca + S_cab)
 // (the sum of Z over the six orderings, written at the ordering i <= j <= k);
 // i <= j <= k runs once, weighted 1 for distinct holes, 1/2 for two equal,
-// 1/6 for three equal.  Bandwidth-bound: S is formed at the six orderings
-// of each particle tile in LDS.  Each block keeps its running sum
-// across launches in `acc[block]` (one stream: a fixed order), so the energy
-// is reproducible.
+// 1/6 for three equal.  Bandwidth-bound: each element of S is formed once,
+// in LDS, and read there at its six orderings.  Each block keeps its running
+// sum across launches in `acc[block]` (one stream: a fixed order), so the
+// energy is reproducible.
 #include "triples.hpp"
 
 namespace cc4s_amd {
@@ -48,10 +48,12 @@ struct TS<true> {
   __device__ static T plus(T a, T b) { return make_double2(a.x + b.x, a.y + b.y); }
 };
 
-// The particle triples are taken in tiles of kTile^3 (a, b, c); a block
-// forms S at all six orderings of the tile's index ranges in LDS, reading W,
-// V and T1 with x1 fastest (coalesced rows), so every permuted read of S
-// comes from LDS.
+// The particle triples are taken in tiles of kTile^3 (a, b, c).  A block
+// takes a canonical tile triple tA <= tB <= tC and forms S at the six
+// orderings of its three index ranges in LDS, reading W, V and T1 with x1
+// fastest (coalesced rows); every distinct permutation of the tile triple is
+// then evaluated from those six LDS tiles, so each element of S is formed
+// once and every permuted read of it comes from LDS.
 constexpr int kTile = 8;
 constexpr int kTileElems = kTile * kTile * kTile;
 
@@ -60,6 +62,12 @@ constexpr int kTileElems = kTile * kTile * kTile;
 __constant__ int kOrder[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0},
                                  {0, 2, 1}, {1, 2, 0}, {2, 0, 1}};
 __constant__ double kCoef[6] = {8.0, -4.0, -4.0, -4.0, 2.0, 2.0};
+// kCompose[s][p] = q with kOrder[q][d] = kOrder[s][kOrder[p][d]]: in the
+// output tile whose (a, b, c) ranges are the canonical ranges kOrder[s],
+// ordering p reads the LDS tile formed for ordering q
+__constant__ int kCompose[6][6] = {{0, 1, 2, 3, 4, 5}, {1, 0, 5, 4, 3, 2},
+                                   {2, 4, 0, 5, 1, 3}, {3, 5, 4, 0, 2, 1},
+                                   {4, 2, 3, 1, 5, 0}, {5, 3, 1, 2, 0, 4}};
 
 template <bool CPLX>
 __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
@@ -69,8 +77,8 @@ __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
     int64_t k0, int64_t kc, double *acc) {
   using S = TS<CPLX>;
   using T = typename S::T;
-  // tiles 0-5: S at the six orderings of the tile's index ranges; tile 6: W
-  __shared__ T tile[7][kTileElems];
+  // S at the six orderings of the canonical tile triple's index ranges
+  __shared__ T tile[6][kTileElems];
   __shared__ double sh[kBlock];
   const int64_t v = Nv, o = No;
   const int64_t nt = (v + kTile - 1) / kTile;
@@ -83,8 +91,9 @@ __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
   if constexpr (CPLX) zero = make_double2(0.0, 0.0); else zero = 0.0;
   double sum = 0.0;
   for (int64_t tid = blockIdx.x; tid < ntiles; tid += gridDim.x) {
-    const int64_t r0[3] = {(tid % nt) * kTile, ((tid / nt) % nt) * kTile,
-                           ((tid / (nt * nt)) % nt) * kTile};
+    const int64_t ti[3] = {tid % nt, (tid / nt) % nt, (tid / (nt * nt)) % nt};
+    if (ti[0] > ti[1] || ti[1] > ti[2]) continue;  // not canonical (uniform per block)
+    const int64_t r0[3] = {ti[0] * kTile, ti[1] * kTile, ti[2] * kTile};
     const int64_t kl = tid / (nt * nt * nt);
     const int64_t base = v * v * v * kl;
     const int64_t k = k0 + kl;
@@ -98,32 +107,39 @@ __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
       const int64_t x2 = r0[kOrder[p][1]] + (t / kTile) % kTile;
       const int64_t x3 = r0[kOrder[p][2]] + t / (kTile * kTile);
       T sv = zero;
-      if (x1 < v && x2 < v && x3 < v) {
-        const T w = W[base + x1 + v * (x2 + v * x3)];
-        if (p == 0) tile[6][t] = w;
-        sv = S::plus(w, S::plus(S::mul(T1[x1 + v * i], Vx(x2, x3, j, k)),
-                                S::plus(S::mul(T1[x2 + v * j], Vx(x1, x3, i, k)),
-                                        S::mul(T1[x3 + v * k], Vx(x1, x2, i, j)))));
-      } else if (p == 0) {
-        tile[6][t] = zero;
-      }
+      if (x1 < v && x2 < v && x3 < v)
+        sv = S::plus(W[base + x1 + v * (x2 + v * x3)],
+                     S::plus(S::mul(T1[x1 + v * i], Vx(x2, x3, j, k)),
+                             S::plus(S::mul(T1[x2 + v * j], Vx(x1, x3, i, k)),
+                                     S::mul(T1[x3 + v * k], Vx(x1, x2, i, j)))));
       tile[p][t] = sv;
     }
     __syncthreads();
     const double weight = (i == j && j == k) ? 1.0 / 6.0 : (i == j || j == k) ? 0.5 : 1.0;
-    for (int t = threadIdx.x; t < kTileElems; t += kBlock) {
+    for (int e = threadIdx.x; e < 6 * kTileElems; e += kBlock) {
+      const int sg = e / kTileElems, t = e % kTileElems;
+      // output tile: (a, b, c) over the canonical ranges kOrder[sg]; a
+      // permutation that repeats an earlier tile triple is skipped
+      bool repeat = false;
+      for (int s2 = 0; s2 < sg; ++s2)
+        repeat = repeat || (ti[kOrder[s2][0]] == ti[kOrder[sg][0]] &&
+                            ti[kOrder[s2][1]] == ti[kOrder[sg][1]] &&
+                            ti[kOrder[s2][2]] == ti[kOrder[sg][2]]);
+      if (repeat) continue;
       const int l[3] = {t % kTile, (t / kTile) % kTile, t / (kTile * kTile)};
-      const int64_t x[3] = {r0[0] + l[0], r0[1] + l[1], r0[2] + l[2]};
+      const int64_t x[3] = {r0[kOrder[sg][0]] + l[0], r0[kOrder[sg][1]] + l[1],
+                            r0[kOrder[sg][2]] + l[2]};
       if (x[0] >= v || x[1] >= v || x[2] >= v) continue;
       T zs = zero;
 #pragma unroll
       for (int p = 0; p < 6; ++p) {
         const int q0 = kOrder[p][0], q1 = kOrder[p][1], q2 = kOrder[p][2];
-        zs = S::lin(1.0, zs, kCoef[p], tile[p][l[q0] + kTile * (l[q1] + kTile * l[q2])],
-                    0.0, zero);
+        zs = S::lin(1.0, zs, kCoef[p],
+                    tile[kCompose[sg][p]][l[q0] + kTile * (l[q1] + kTile * l[q2])], 0.0, zero);
       }
       const double d = ((eh[i] + eh[j]) + eh[k]) - ((ep[x[0]] + ep[x[1]]) + ep[x[2]]);
-      sum += weight * S::re_mul(tile[6][t], zs) / d;
+      const T w = W[base + x[0] + v * (x[1] + v * x[2])];
+      sum += weight * S::re_mul(w, zs) / d;
     }
   }
   sh[threadIdx.x] = sum;
