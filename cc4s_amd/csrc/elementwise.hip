@@ -60,30 +60,44 @@ __global__ __launch_bounds__(kBlock) void tau_kernel(typename Sc<CPLX>::T *X,
   }
 }
 
-// one thread per unordered pair {(a,i),(b,j)}: both elements read first
+// (a, b) tiles of the pphh layout: a tile at fixed (i, j) and its partner
+// (the transposed tile at (j, i)) move through LDS together, so both are
+// read and written along their contiguous index
+constexpr int kPairTile = 32;
+
+// R_abij <- R_abij + R_baji (old values on the right): one block per
+// unordered pair of tiles {(A, B, i, j), (B, A, j, i)}, taken where
+// (i, tile A) <= (j, tile B); both tiles are read before either is written
 template <bool CPLX>
 __global__ __launch_bounds__(kBlock) void symmetrize_kernel(typename Sc<CPLX>::T *R,
                                                             int64_t No, int64_t Nv) {
   using S = Sc<CPLX>;
-  const int64_t n = Nv * Nv * No * No;
-  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
-       e += int64_t(gridDim.x) * kBlock) {
-    int64_t r = e;
-    const int64_t a = r % Nv; r /= Nv;
-    const int64_t b = r % Nv; r /= Nv;
-    const int64_t i = r % No;
-    const int64_t j = r / No;
-    const int64_t p = a + Nv * i, q = b + Nv * j;  // pair indices (a,i), (b,j)
-    if (p > q) continue;
-    const int64_t t = b + Nv * (a + Nv * (j + No * i));
-    const typename S::T x = R[e];
-    if (p == q) {
-      R[e] = S::add(x, x);
-    } else {
-      const typename S::T y = R[t];
-      const typename S::T s2 = S::add(x, y);
-      R[e] = s2;
-      R[t] = S::add(y, x);
+  using T = typename S::T;
+  constexpr int kt = kPairTile;
+  __shared__ T xs[kt][kt + 1], ys[kt][kt + 1];
+  const int64_t nt = (Nv + kt - 1) / kt;
+  const int64_t ntiles = nt * nt * No * No;
+  for (int64_t tid = blockIdx.x; tid < ntiles; tid += gridDim.x) {
+    const int64_t ta = tid % nt, tb = (tid / nt) % nt, ij = tid / (nt * nt);
+    const int64_t i = ij % No, j = ij / No;
+    const int64_t P = i * nt + ta, Q = j * nt + tb;
+    if (P > Q) continue;  // the partner's block does it (uniform per block)
+    const int64_t a0 = ta * kt, b0 = tb * kt;
+    const int64_t offX = Nv * Nv * (i + No * j), offY = Nv * Nv * (j + No * i);
+    __syncthreads();  // the previous pair's LDS reads are done
+    for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
+      const int u = e % kt, w = e / kt;
+      // xs[a - a0][b - b0] = R_abij, ys[b - b0][a - a0] = R_baji
+      xs[u][w] = (a0 + u < Nv && b0 + w < Nv) ? R[offX + (a0 + u) + Nv * (b0 + w)] : S::zero();
+      ys[u][w] = (b0 + u < Nv && a0 + w < Nv) ? R[offY + (b0 + u) + Nv * (a0 + w)] : S::zero();
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
+      const int u = e % kt, w = e / kt;
+      if (a0 + u < Nv && b0 + w < Nv)
+        R[offX + (a0 + u) + Nv * (b0 + w)] = S::add(xs[u][w], ys[w][u]);
+      if (P != Q && b0 + u < Nv && a0 + w < Nv)
+        R[offY + (b0 + u) + Nv * (a0 + w)] = S::add(ys[u][w], xs[w][u]);
     }
   }
 }
@@ -136,32 +150,44 @@ __device__ double2 block_sum(double2 v, double2 *sh) {
   return r;
 }
 
-// energy partials: per block (D, X) as two double2
+// energy partials: per block (D, X) as two double2.  (a, b) tiles at fixed
+// (i, j): V_baij of the tile is staged through LDS, read along b
 template <bool CPLX>
 __global__ __launch_bounds__(kBlock) void energy_kernel(const typename Sc<CPLX>::T *T,
                                                         const typename Sc<CPLX>::T *V,
                                                         int64_t No, int64_t Nv,
                                                         double2 *partials) {
   using S = Sc<CPLX>;
+  constexpr int kt = kPairTile;
   __shared__ double2 sh[kBlock];
+  __shared__ typename S::T vt[kt][kt + 1];  // vt[b - b0][a - a0] = V_baij
   const typename S::T *T1 = T;
   const typename S::T *T2 = T + Nv * No;
-  const int64_t n = Nv * Nv * No * No;
+  const int64_t nt = (Nv + kt - 1) / kt;
+  const int64_t ntiles = nt * nt * No * No;
   double2 d = make_double2(0.0, 0.0), x = make_double2(0.0, 0.0);
-  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
-       e += int64_t(gridDim.x) * kBlock) {
-    int64_t r = e;
-    const int64_t a = r % Nv; r /= Nv;
-    const int64_t b = r % Nv; r /= Nv;
-    const int64_t i = r % No;
-    const int64_t j = r / No;
-    const typename S::T tau = S::add(T2[e], S::mul(T1[a + Nv * i], T1[b + Nv * j]));
-    const int64_t eba = b + Nv * (a + Nv * (i + No * j));
-    // tau * conj(V) = conj(conj(tau) * V)
-    const double2 dv = S::cdot(V[e], tau);
-    const double2 xv = S::cdot(V[eba], tau);
-    d.x += dv.x; d.y += dv.y;
-    x.x += xv.x; x.y += xv.y;
+  for (int64_t tid = blockIdx.x; tid < ntiles; tid += gridDim.x) {
+    const int64_t ta = tid % nt, tb = (tid / nt) % nt, ij = tid / (nt * nt);
+    const int64_t i = ij % No, j = ij / No;
+    const int64_t a0 = ta * kt, b0 = tb * kt, off = Nv * Nv * (i + No * j);
+    __syncthreads();
+    for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
+      const int u = e % kt, w = e / kt;
+      vt[u][w] = (b0 + u < Nv && a0 + w < Nv) ? V[off + (b0 + u) + Nv * (a0 + w)] : S::zero();
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
+      const int u = e % kt, w = e / kt;
+      const int64_t a = a0 + u, b = b0 + w;
+      if (a >= Nv || b >= Nv) continue;
+      const int64_t idx = off + a + Nv * b;
+      const typename S::T tau = S::add(T2[idx], S::mul(T1[a + Nv * i], T1[b + Nv * j]));
+      // tau * conj(V) = conj(conj(tau) * V)
+      const double2 dv = S::cdot(V[idx], tau);
+      const double2 xv = S::cdot(vt[w][u], tau);
+      d.x += dv.x; d.y += dv.y;
+      x.x += xv.x; x.y += xv.y;
+    }
   }
   const double2 ds = block_sum(d, sh);
   const double2 xs = block_sum(x, sh);
@@ -264,11 +290,10 @@ void tau(hipStream_t s, Field f, void *X, const void *T2, const void *T1,
 }
 
 void symmetrize_pairs(hipStream_t s, Field f, void *R2, int64_t No, int64_t Nv) {
-  const int64_t n = Nv * Nv * No * No;
   if (f == Field::Complex)
-    symmetrize_kernel<true><<<grid_for(n), kBlock, 0, s>>>(static_cast<double2 *>(R2), No, Nv);
+    symmetrize_kernel<true><<<kReduceBlocks, kBlock, 0, s>>>(static_cast<double2 *>(R2), No, Nv);
   else
-    symmetrize_kernel<false><<<grid_for(n), kBlock, 0, s>>>(static_cast<double *>(R2), No, Nv);
+    symmetrize_kernel<false><<<kReduceBlocks, kBlock, 0, s>>>(static_cast<double *>(R2), No, Nv);
   CC4S_HIP(hipGetLastError());
 }
 
@@ -456,35 +481,62 @@ __global__ __launch_bounds__(kBlock) void dots_packed_kernel(DotArgs p, int coun
 }
 
 // full-layout output; the norm runs over full elements, i.e. it is the norm
-// of the full symmetric tensor
+// of the full symmetric tensor.  T2 is taken in (a, b) tiles at fixed
+// (i, j): for i < j the packed index of (a,i),(b,j) runs along a, for i > j
+// along b, so there the tile is combined along b and written out along a
+// through LDS; i = j tiles (a 1/No share) are combined element by element.
 template <bool CPLX>
 __global__ __launch_bounds__(kBlock) void combine_packed_kernel(DotArgs p, int count,
                                                                 typename Sc<CPLX>::T *out,
                                                                 int64_t No, int64_t Nv,
                                                                 double2 *partials) {
   using S = Sc<CPLX>;
+  using T = typename S::T;
+  constexpr int kt = kPairTile;
   __shared__ double2 sh[kBlock];
+  __shared__ T at[kt][kt + 1];
   const int64_t n1 = Nv * No;
-  const int64_t n = n1 + n1 * n1;
   double2 nrm = make_double2(0.0, 0.0);
-  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
-       e += int64_t(gridDim.x) * kBlock) {
-    int64_t t = e;
-    if (e >= n1) {
-      int64_t r = e - n1;
-      const int64_t a = r % Nv; r /= Nv;
-      const int64_t b = r % Nv; r /= Nv;
-      const int64_t i = r % No;
-      const int64_t j = r / No;
-      t = n1 + pair_index(a + Nv * i, b + Nv * j);
-    }
-    typename S::T av = S::zero(), rv = S::zero();
+  // the combination at packed index t, accumulating |R|^2 of it
+  auto combine_at = [&](int64_t t) {
+    T av = S::zero(), rv = S::zero();
     for (int k = 0; k < count; ++k) {
-      av = S::add(av, S::scale(static_cast<const typename S::T *>(p.A[k])[t], p.c[k]));
-      rv = S::add(rv, S::scale(static_cast<const typename S::T *>(p.R[k])[t], p.c[k]));
+      av = S::add(av, S::scale(static_cast<const T *>(p.A[k])[t], p.c[k]));
+      rv = S::add(rv, S::scale(static_cast<const T *>(p.R[k])[t], p.c[k]));
     }
-    out[e] = av;
     nrm.x += S::cdot(rv, rv).x;
+    return av;
+  };
+  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n1;
+       e += int64_t(gridDim.x) * kBlock)
+    out[e] = combine_at(e);
+  const int64_t nt = (Nv + kt - 1) / kt;
+  const int64_t ntiles = nt * nt * No * No;
+  for (int64_t tid = blockIdx.x; tid < ntiles; tid += gridDim.x) {
+    const int64_t ta = tid % nt, tb = (tid / nt) % nt, ij = tid / (nt * nt);
+    const int64_t i = ij % No, j = ij / No;
+    const int64_t a0 = ta * kt, b0 = tb * kt, off = n1 + Nv * Nv * (i + No * j);
+    __syncthreads();  // the previous tile's LDS reads are done
+    for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
+      const int u = e % kt, w = e / kt;
+      if (i > j) {
+        // u runs along b: packed (P, Q) = (a + Nv i, b + Nv j), P > Q
+        const int64_t a = a0 + w, b = b0 + u;
+        if (a < Nv && b < Nv) at[u][w] = combine_at(n1 + pair_index(a + Nv * i, b + Nv * j));
+      } else {
+        const int64_t a = a0 + u, b = b0 + w;
+        if (a < Nv && b < Nv)
+          out[off + a + Nv * b] = combine_at(n1 + pair_index(a + Nv * i, b + Nv * j));
+      }
+    }
+    __syncthreads();
+    if (i > j) {
+      for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
+        const int u = e % kt, w = e / kt;
+        const int64_t a = a0 + u, b = b0 + w;
+        if (a < Nv && b < Nv) out[off + a + Nv * b] = at[w][u];
+      }
+    }
   }
   const double2 s = block_sum(nrm, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
