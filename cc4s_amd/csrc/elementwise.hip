@@ -482,9 +482,11 @@ __global__ __launch_bounds__(kBlock) void dots_packed_kernel(DotArgs p, int coun
 
 // full-layout output; the norm runs over full elements, i.e. it is the norm
 // of the full symmetric tensor.  T2 is taken in (a, b) tiles at fixed
-// (i, j): for i < j the packed index of (a,i),(b,j) runs along a, for i > j
-// along b, so there the tile is combined along b and written out along a
-// through LDS; i = j tiles (a 1/No share) are combined element by element.
+// (i, j); with P = a + Nv i and Q = b + Nv j, a tile below the diagonal of
+// the (P, Q) tile grid has its packed index pair_index(P, Q) running along
+// b, so it is combined along b and written out along a through LDS; a tile
+// above runs along a; only the diagonal tiles are combined element by
+// element.
 template <bool CPLX>
 __global__ __launch_bounds__(kBlock) void combine_packed_kernel(DotArgs p, int count,
                                                                 typename Sc<CPLX>::T *out,
@@ -516,11 +518,12 @@ __global__ __launch_bounds__(kBlock) void combine_packed_kernel(DotArgs p, int c
     const int64_t ta = tid % nt, tb = (tid / nt) % nt, ij = tid / (nt * nt);
     const int64_t i = ij % No, j = ij / No;
     const int64_t a0 = ta * kt, b0 = tb * kt, off = n1 + Nv * Nv * (i + No * j);
+    const bool below = i * nt + ta > j * nt + tb;  // every P > Q in the tile
     __syncthreads();  // the previous tile's LDS reads are done
     for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
       const int u = e % kt, w = e / kt;
-      if (i > j) {
-        // u runs along b: packed (P, Q) = (a + Nv i, b + Nv j), P > Q
+      if (below) {
+        // u runs along b: P > Q, so pair_index(P, Q) = P (P + 1) / 2 + Q
         const int64_t a = a0 + w, b = b0 + u;
         if (a < Nv && b < Nv) at[u][w] = combine_at(n1 + pair_index(a + Nv * i, b + Nv * j));
       } else {
@@ -530,7 +533,7 @@ __global__ __launch_bounds__(kBlock) void combine_packed_kernel(DotArgs p, int c
       }
     }
     __syncthreads();
-    if (i > j) {
+    if (below) {
       for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
         const int u = e % kt, w = e / kt;
         const int64_t a = a0 + u, b = b0 + w;
