@@ -805,6 +805,12 @@ void ContractionEngine::apply_env_overrides() {
     CC4S_ASSERT(v >= 16, "CC4S_MIXED_MAX_K must be at least 16");
     mixed_max_k = v;
   }
+  // CC4S_MIXED_MIN_FLOPS: smaller complex contractions stay FP64
+  if (const char *e = std::getenv("CC4S_MIXED_MIN_FLOPS")) {
+    const double v = std::atof(e);
+    CC4S_ASSERT(v >= 0.0, "CC4S_MIXED_MIN_FLOPS must not be negative");
+    mixed_min_flops = v;
+  }
 }
 
 // a timing event from the pool of recorded and read ones, else a new one
