@@ -310,3 +310,25 @@ def test_mp2_pair_energies_and_triples_star(field):
     assert abs(np.sum(e).imag) < 1e-14 * abs(O.mp2_energy(P))
     assert O.triples_star(P, -0.0123, e.real) == pytest.approx(-0.0123, rel=1e-14)
     assert O.triples_star(P, -0.0123, 1.25 * e.real) == pytest.approx(1.25 * -0.0123, rel=1e-14)
+
+
+@pytest.mark.parametrize("seed", [3, 9])
+def test_drccd_direct_energy_is_the_rpa_plasmon_formula(seed):
+    """Direct ring CCD is the random-phase approximation (Scuseria,
+    Henderson and Sorensen, J. Chem. Phys. 129, 231101 (2008)): for real
+    orbitals the converged direct energy of the Drccd method equals
+    (sum_n w_n - tr A) / 2 with A = D + K, B = K, K_(ai),(bj) = 2 V_abij and
+    w_n^2 the eigenvalues of (A - B)^1/2 (A + B) (A - B)^1/2."""
+    No, Nv, NG = 3, 8, 10
+    inp = syn.generate(No, Nv, NG, "real", seed=seed, scale_rule="mp2_per_occupied")
+    P = O.CcsdProblem(inp.gamma, inp.eps, No, "real", method="Drccd")
+    r = O.coupled_cluster(P, max_iterations=400, energy_convergence=1e-14,
+                          amplitudes_convergence=1e-13, max_residua=8)
+    assert r["convergenceReached"]
+    n = No * Nv
+    K = 2.0 * np.real(P.V["pphh"]).transpose(0, 2, 1, 3).reshape(n, n)
+    D = (P.eps_p[:, None] - P.eps_h[None, :]).reshape(n)
+    M = np.sqrt(D)[:, None] * (np.diag(D) + 2.0 * K) * np.sqrt(D)[None, :]
+    w = np.sqrt(np.linalg.eigvalsh(M))
+    e_rpa = 0.5 * (w.sum() - D.sum() - np.trace(K))
+    assert abs(r["energy"]["direct"] - e_rpa) < 1e-12
