@@ -151,3 +151,48 @@ def test_closed_shell_ccsd_equals_spin_orbital_ccsd(field):
     assert np.max(np.abs(mixed - T2)) < 1e-10 * scale
     same = t2[np.ix_(up_o, up_o, up_v, up_v)].transpose(2, 3, 0, 1)
     assert np.max(np.abs(same - (T2 - T2.transpose(1, 0, 2, 3)))) < 1e-10 * scale
+
+
+def spin_orbital_triples(A, e, nocc, t1, t2):
+    """Canonical spin-orbital (T) (Raghavachari, Trucks, Pople and
+    Head-Gordon 1989, as written by Crawford and Schaefer):
+      D t_c = P(i/jk) P(a/bc) [sum_e t_jk^ae <ei||bc> - sum_m t_im^bc <ma||jk>]
+      D t_d = P(i/jk) P(a/bc) t_i^a <jk||bc>
+      E(T)  = 1/36 sum conj(t_c) D (t_c + t_d)
+    with P(i/jk) f(ijk) = f(ijk) - f(jik) - f(kji)."""
+    o, v = slice(0, nocc), slice(nocc, len(e))
+    eo, ev = e[o], e[v]
+    D = (eo[:, None, None, None, None, None] + eo[None, :, None, None, None, None]
+         + eo[None, None, :, None, None, None] - ev[None, None, None, :, None, None]
+         - ev[None, None, None, None, :, None] - ev[None, None, None, None, None, :])
+    ein = lambda s, *x: np.einsum(s, *x, optimize=True)
+    X = ein("jkae,eibc->ijkabc", t2, A[v, o, v, v]) - ein("imbc,majk->ijkabc", t2, A[o, v, o, o])
+    Y = ein("ia,jkbc->ijkabc", t1, A[o, o, v, v])
+
+    def P(x):
+        x = x - x.transpose(1, 0, 2, 3, 4, 5) - x.transpose(2, 1, 0, 3, 4, 5)
+        return x - x.transpose(0, 1, 2, 4, 3, 5) - x.transpose(0, 1, 2, 5, 4, 3)
+
+    tc, td = P(X) / D, P(Y) / D
+    return np.sum(np.conj(tc) * D * (tc + td)) / 36.0
+
+
+@pytest.mark.parametrize("field", ["real"])
+def test_closed_shell_triples_equals_spin_orbital_triples(field):
+    """(T) of PerturbativeTriplesReference (the oracle's restatement) on the
+    converged closed-shell amplitudes equals textbook spin-orbital (T) on the
+    spin-orbital amplitudes of the same system (real field: for complex
+    integrals the textbook expression depends on where the conjugations sit,
+    and the reference's complex (T) is pinned by its recorded UEG energy
+    instead)."""
+    No, Nv, NG = 2, 4, 7
+    inp = syn.generate(No, Nv, NG, field, seed=61, scale_rule="mp2_per_occupied")
+    P = O.CcsdProblem(inp.gamma, inp.eps, No, field)
+    ref = O.coupled_cluster(P, max_iterations=200, energy_convergence=1e-14,
+                            amplitudes_convergence=1e-12, max_residua=8)
+    closed = O.perturbative_triples(inp.gamma, inp.eps, No, field, ref["amplitudes"])
+    A = spin_orbital_integrals(physicist_integrals(inp.gamma, field))
+    e = np.repeat(np.asarray(inp.eps, dtype=np.float64), 2)
+    _, t1, t2 = spin_orbital_ccsd(A, e, 2 * No)
+    so = spin_orbital_triples(A, e, 2 * No, t1, t2)
+    assert abs(so.real - closed) < 1e-10 * abs(closed), (so, closed)
