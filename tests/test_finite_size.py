@@ -123,3 +123,28 @@ def test_finite_size_integration_properties():
     two = F.finite_size_correction(grid, B, 2.0 * S, N=N, grid_unit=1.1)
     one = F.finite_size_correction(grid, B, S, N=N, grid_unit=1.1)
     assert np.allclose(two, 2.0 * np.array(one), rtol=1e-13)
+
+
+def test_ueg_momentum_grid_matches_the_vertex():
+    """The momentum grid and Coulomb potential of the electron-gas vertex
+    (UegVertexGenerator.cxx:213-296): every nonzero Gamma^G_pq has
+    |Gamma^G_pq|^2 = V(G), and the momenta of the pair it couples differ by
+    the grid vector of G, k_p - k_q = G (the vertex itself reproduces the
+    reference's recorded energies, test_oracle.py)."""
+    from oracle import ueg
+    No, Nv = 7, 26
+    gamma, eps, fermi = ueg.generate(No, Nv, 1.0)
+    grid, B, V = ueg.momentum_grid(No, Nv, 1.0)
+    assert gamma.shape[0] == grid.shape[0] == V.shape[0]
+    b = B[0, 0]
+    Np = No + Nv
+    maxG = int((5.0 * Np) ** (1.0 / 3.0))
+    import itertools
+    k = sorted(itertools.product(range(-maxG, maxG + 1), repeat=3),
+               key=lambda g: g[0] ** 2 + g[1] ** 2 + g[2] ** 2)[:Np]
+    k = b * np.array(k, dtype=float)
+    nz = np.argwhere(np.abs(gamma) > 0)
+    assert len(nz) > 0
+    for G, p, q in nz:
+        assert abs(abs(gamma[G, p, q]) ** 2 - V[G]) <= 1e-14 * V[G]
+        assert np.allclose(k[p] - k[q], grid[G])
