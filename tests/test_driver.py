@@ -276,3 +276,41 @@ def test_second_order_singles_and_fock_slices(built, tmp_path, field, spin):
     for part in ("hh", "hp", "ph", "pp"):
         got = read_tensor(tmp_path, f"SlicedFock.components.{part}", out["components"][part])
         assert np.array_equal(got, F[rng[part[0]], rng[part[1]]]), part
+
+
+@pytest.mark.parametrize("case,message", [
+    ("fock_not_sliced", "expecting a TensorSet {hh, hp, ph, pp} as 'slicedFockOperator'"),
+    ("fock_missing", "'slicedFockOperator' does not name a stored object"),
+    ("pairs_missing", "'mp2PairEnergies' does not name a stored object"),
+    ("cT", "cT (complete triples) is not supported"),
+])
+def test_second_order_and_triples_argument_errors(built, tmp_path, case, message):
+    """Argument errors of SecondOrderPerturbationTheory and
+    PerturbativeTriples stop the run before any GPU work (dry run)."""
+    inp = syn.generate(2, 5, 3, "complex", 1)
+    syn.write_cc4s_inputs(inp, tmp_path)
+    if case.startswith("fock"):
+        fock = "EigenEnergies" if case == "fock_not_sliced" else "NoSuchFock"
+        steps = STEPS + f"""
+- name: SecondOrderPerturbationTheory
+  in:
+    coulombIntegrals: CoulombIntegrals
+    slicedEigenEnergies: EigenEnergies
+    slicedFockOperator: {fock}
+  out: {{}}
+"""
+    else:
+        extra = "    mp2PairEnergies: NoSuchPairs\n" if case == "pairs_missing" else "    cT: 1\n"
+        # any stored object passes the name lookup; the argument checks
+        # come before the amplitudes are inspected
+        steps = STEPS + f"""
+- name: PerturbativeTriples
+  in:
+    coulombIntegrals: CoulombIntegrals
+    amplitudes: EigenEnergies
+    slicedEigenEnergies: EigenEnergies
+{extra}  out: {{}}
+"""
+    r = run_driver(tmp_path, steps, "--dry-run")
+    assert r.returncode == 1
+    assert message in r.stderr
