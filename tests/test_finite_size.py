@@ -148,3 +148,10 @@ def test_ueg_momentum_grid_matches_the_vertex():
     for G, p, q in nz:
         assert abs(abs(gamma[G, p, q]) ** 2 - V[G]) <= 1e-14 * V[G]
         assert np.allclose(k[p] - k[q], grid[G])
+
+
+def test_library_interpolation_refuses_an_empty_fine_grid():
+    B = _lattice("cubic")
+    grid = _grid(B, 2)
+    with pytest.raises(api.CcsdError, match="N > 0"):
+        api.finite_size_correction(grid, B, np.ones(len(grid)), N=0)
