@@ -283,8 +283,8 @@ def main():
                          "for the No=32, Nv=256, NG=1500 workloads)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; "
-                          "--gpus must be 1"}))
+        print(json.dumps({"error": "the CCSD iteration runs on one GPU per process "
+                          "(no sharding across GPUs); --gpus must be 1"}))
         sys.exit(2)
     switches = [float(x) for x in str(args.mixed_switch).split(",") if x]
     if args.mixed_split is not None:
