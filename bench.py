@@ -74,15 +74,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gpu_mp2(gamma, eps, No, field):
-    """Second-order energy of a vertex from the first pass of the GPU path
-    (R = V, T = -V/Delta; the energy of that pass is the closed form)."""
-    NG, Np, _ = gamma.shape
-    with api.CoupledCluster(No, Np - No, NG, field=field) as cc:
-        cc.set_inputs(gamma, eps)
-        return cc.iterate()["energy"]
-
-
 def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
     """Times the oracle's particle-particle ladder statements
     (oracle/ccsd_oracle.py, Ccsd.cxx:588-590) on slice pairs of the same
@@ -140,6 +131,96 @@ def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
 
 MIXED_OPTIONS = {}
 
+# Pass sequences of full runs of the complex No=64, Nv=512, NG=3000 workload
+# (scale rule mp2_per_occupied, DIIS 8, convergence |dE| < 1e-10 and
+# dR < 1e-8) to convergence on one MI355X: FP64 converges at pass 17; the
+# mixed run (half split, 2048-term FP32 sums, switch 1e-5, refinement 1)
+# does 8 split passes (the first is the R = V pass), the FP64 anchor, 7
+# corrected split passes and one FP64 pass.  F = FP64, S = split,
+# A = anchor (an FP64 and a split residuum), C = corrected split.
+RECORDED_PASSES_NO64_NV512 = {"fp64": "F" * 17, "mixed": "S" * 8 + "A" + "C" * 7 + "F"}
+
+
+def dominant_kernel(stats):
+    """The contraction-kernel variant with the largest summed duration, its
+    algorithmic rate and fraction of the FP64 matrix peak."""
+    dom = max(stats, key=lambda v: stats[v]["ms"])
+    d = stats[dom]
+    achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
+    return {
+        "kernel": dom, "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
+        "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS,
+        "launches": d["launches"], "avg_launch_ms": d["ms"] / max(1, d["launches"]),
+        "flops_per_launch": d["flops"] / max(1, d["launches"]),
+    }
+
+
+def large_workload_report(wname, scale):
+    """One FP64 pass and one split (mixed) pass of a large workload, timed in
+    this run after the passes of the main workload: the first pass of each
+    context (R = V, no residuum contraction) is untimed, the second evaluates
+    the full residuum.  Reports s/iteration, the iteration's and the
+    particle-particle ladder's algorithmic rate against the FP64 matrix peak,
+    the dominant kernel, the split pass's speed-up, and the time to
+    convergence of both paths projected from these per-pass costs and the
+    pass sequences of recorded full runs (RECORDED_PASSES_NO64_NV512)."""
+    cfg = syn.WORKLOADS[wname]
+    log(f"large workload: generating inputs: {cfg.name}, scale rule {scale}")
+    t = time.perf_counter()
+    inp = syn.generate_config(cfg, scale_rule=scale, mp2_fn=syn.mp2_on_device)
+    log(f"  {time.perf_counter() - t:.1f} s, scale c = {inp.scale:.6e}")
+    out = {"workload": f"{cfg.name}, DIIS {cfg.max_residua}", "No": cfg.No,
+           "Nv": cfg.Nv, "NG": cfg.NG, "scale_rule": scale, "scale": inp.scale}
+    with make_context(cfg, api.PRECISION_FP64, 1e-5) as cc:
+        cc.set_inputs(inp.gamma, inp.eps)
+        first = cc.iterate()
+        cc.set_profiling(True)
+        r = cc.iterate()
+        stats = {v: cc.kernel_stats(v) for v in VARIANTS}
+        lad_ms, lad_flops, _ = cc.ladder_stats()
+        log(f"large workload FP64 pass: E={r['energy']:.12f} {r['seconds']:.3f} s "
+            f"{r['flops'] / r['seconds'] / 1e12:.2f} TFLOP/s")
+    ladder = lad_flops / (lad_ms * 1e-3) / 1e12 if lad_ms else None
+    out["fp64"] = {
+        "seconds_per_iteration": r["seconds"],
+        "first_pass_seconds": first["seconds"],
+        "residuum_seconds": r["residuum_seconds"],
+        "update_and_diis_seconds": r["mixer_seconds"],
+        "flops_per_iteration": r["flops"],
+        "tflops_per_s": r["flops"] / r["seconds"] / 1e12,
+        "frac_fp64_mfma_peak_iteration": r["flops"] / r["seconds"] / 1e12
+        / FP64_MFMA_PEAK_TFLOPS,
+        "ladder_tflops_per_s": ladder,
+        "ladder_frac_fp64_mfma_peak": ladder / FP64_MFMA_PEAK_TFLOPS if ladder else None,
+        "dominant_kernel": dominant_kernel(stats),
+    }
+    with make_context(cfg, api.PRECISION_MIXED, 1e-5) as cc:
+        cc.set_inputs(inp.gamma, inp.eps)
+        cc.iterate()
+        m = cc.iterate()
+        log(f"large workload split pass: E={m['energy']:.12f} {m['seconds']:.3f} s")
+    out["mixed"] = {
+        "seconds_per_iteration": m["seconds"],
+        "split": {0: "bf16", 1: "half"}[cc.params.mixed_split],
+        "max_k": cc.params.mixed_max_k,
+        "algorithmic_tflops_per_s": r["flops"] / m["seconds"] / 1e12,
+        "speedup_vs_fp64_pass": r["seconds"] / m["seconds"],
+    }
+    if wname == "complex_no64_nv512_ng3000":
+        cost = {"F": r["seconds"], "S": m["seconds"], "C": m["seconds"],
+                "A": r["seconds"] + m["seconds"]}
+
+        def projected(seq):
+            # the first pass of a run is the R = V pass
+            return first["seconds"] + sum(cost[k] for k in seq[1:])
+        t64 = projected(RECORDED_PASSES_NO64_NV512["fp64"])
+        tmx = projected(RECORDED_PASSES_NO64_NV512["mixed"])
+        out["time_to_convergence_projected"] = {
+            "fp64_seconds": t64, "mixed_seconds": tmx, "speedup": t64 / tmx,
+            "pass_sequences": RECORDED_PASSES_NO64_NV512,
+        }
+    return out
+
 
 def make_context(cfg, precision, switch, restart=None, keep=None, refine=None):
     """A context of the workload; mixed contexts take the split settings of
@@ -156,10 +237,25 @@ def make_context(cfg, precision, switch, restart=None, keep=None, refine=None):
                               mixedRestartKeep=keep, **extra)
 
 
+def time_to_convergence(passes):
+    """Passes, wall time and energy up to and including the first pass that
+    meets the convergence test (all passes if none does): passes timed
+    beyond convergence do not count."""
+    n = next((k + 1 for k, p in enumerate(passes) if p["converged"]), len(passes))
+    cut = passes[:n]
+    return {
+        "iterations": n,
+        "seconds": float(sum(p["seconds"] for p in cut)),
+        "energy": cut[-1]["energy"] if cut else None,
+        "converged": bool(cut and cut[-1]["converged"]),
+    }
+
+
 def converge(cc, label, first_passes):
-    """Iterates to convergence; returns (passes, seconds, energy, converged)."""
+    """Iterates until a pass meets the convergence test or maxIterations;
+    returns every pass (those given first included)."""
     passes = list(first_passes)
-    while not (passes and passes[-1]["converged"]) and \
+    while not any(p["converged"] for p in passes) and \
             len(passes) < cc.params.max_iterations:
         r = cc.iterate()
         passes.append(r)
@@ -276,6 +372,10 @@ def main():
                     help="also time this many mixed-precision passes (after "
                          "--warmup untimed ones) without converging")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--large", default="complex_no64_nv512_ng3000",
+                    choices=("none",) + tuple(syn.WORKLOADS),
+                    help="after the main workload, time one FP64 and one split "
+                         "pass of this workload (none: skip)")
     ap.add_argument("--triples", action="store_true",
                     help="time the (T) correction at the FP64 fixed point")
     ap.add_argument("--adjoint", default="auto", choices=("auto", "on", "off"),
@@ -302,7 +402,7 @@ def main():
     t = time.perf_counter()
     big = cfg.No * cfg.Nv > 16384
     inp = syn.generate_config(cfg, scale_rule=args.scale,
-                              mp2_fn=gpu_mp2 if big else None)
+                              mp2_fn=syn.mp2_on_device if big else None)
     log(f"  {time.perf_counter() - t:.1f} s, scale c = {inp.scale:.6e}")
 
     out_conv = {}
@@ -342,16 +442,11 @@ def main():
         cc.set_profiling(False)
         if args.converge in ("both", "fp64"):
             passes = converge(cc, "fp64", passes)
-            out_conv["fp64"] = {
-                "iterations": len(passes),
-                "seconds": float(sum(p["seconds"] for p in passes)),
-                "energy": passes[-1]["energy"],
-                "converged": bool(passes[-1]["converged"]),
-            }
+            out_conv["fp64"] = time_to_convergence(passes)
             want = args.adjoint == "on" or (args.adjoint == "auto"
                                             and wname in ("complex_no32_nv256_ng1500",
                                                           "adjoint_no32_nv256_ng1500"))
-            if want and passes[-1]["converged"]:
+            if want and out_conv["fp64"]["converged"]:
                 adjoint = adjoint_report(cc, inp.gamma, passes[-1]["energy"])
             if args.triples:
                 t = time.perf_counter()
@@ -401,19 +496,21 @@ def main():
         cc = make_context(cfg, api.PRECISION_MIXED, sw, rs, kp, rf)
         cc.set_inputs(inp.gamma, inp.eps)
         passes = converge(cc, key, [])
+        ttc = time_to_convergence(passes)
+        passes = passes[:ttc["iterations"]]
         mixed_times = [p["seconds"] for p in passes[1:] if p["precision_used"]]
         fp64_times = [p["seconds"] for p in passes[1:] if not p["precision_used"]]
         out_conv[key] = {
-            "iterations": len(passes),
+            "iterations": ttc["iterations"],
             "mixed_iterations": sum(1 for p in passes if p["precision_used"]),
             "fp64_passes": len(fp64_times),
-            "seconds": float(sum(p["seconds"] for p in passes)),
+            "seconds": ttc["seconds"],
             "seconds_per_mixed_iteration": (float(np.mean(mixed_times))
                                             if mixed_times else None),
             "seconds_per_fp64_pass": (float(np.mean(fp64_times))
                                       if fp64_times else None),
-            "energy": passes[-1]["energy"],
-            "converged": bool(passes[-1]["converged"]),
+            "energy": ttc["energy"],
+            "converged": ttc["converged"],
             "switch": sw,
             "diis_restart_at_switch": rs,
             "diis_pairs_kept": kp,
@@ -428,6 +525,10 @@ def main():
                 out_conv["fp64"]["seconds"] / out_conv[key]["seconds"])
             out_conv[key]["abs_energy_difference"] = abs(
                 out_conv["fp64"]["energy"] - out_conv[key]["energy"])
+
+    large = None
+    if args.large != "none" and args.large != wname:
+        large = large_workload_report(args.large, args.scale)
 
     out = {
         "metric": METRIC,
@@ -458,10 +559,10 @@ def main():
         out["config"]["adjoint"] = adjoint
     if triples is not None:
         out["config"]["triples"] = triples
+    if large is not None:
+        out["config"]["large_workload"] = large
     if value is not None:
-        dom = max(stats, key=lambda v: stats[v]["ms"])
-        d = stats[dom]
-        achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
+        dk = dominant_kernel(stats)
         lad_ms, lad_flops, _ = lad
         out["config"].update({
             "flops_per_iteration": iter_flops,
@@ -471,17 +572,23 @@ def main():
             "ladder_tflops_per_s": (lad_flops / (lad_ms * 1e-3) / 1e12
                                     if lad_ms else None),
         })
+        # the complex FP64 kernels run the three-product complex multiply
+        # (6 real flops of MFMA work per complex MAC; generation 2: four
+        # products, 8); `achieved` credits 8 per complex MAC
+        gen = int(os.environ.get("CC4S_GETT_GENERATION", "3"))
+        executed = (6.0 if gen >= 3 else 8.0) / 8.0 if cfg.field == "complex" else 1.0
         out["roofline"] = {
-            "kernel": dom,
+            "kernel": dk["kernel"],
             "bound": "mfma",
-            "achieved": achieved,
+            "achieved": dk["achieved"],
             "peak": FP64_MFMA_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / FP64_MFMA_PEAK_TFLOPS,
-            "traffic": measured_traffic(dom, cfg),
-            "launches": d["launches"],
-            "avg_launch_ms": d["ms"] / max(1, d["launches"]),
-            "flops_per_launch": d["flops"] / max(1, d["launches"]),
+            "frac": dk["frac"],
+            "traffic": measured_traffic(dk["kernel"], cfg),
+            "frac_executed_mfma": dk["frac"] * executed,
+            "launches": dk["launches"],
+            "avg_launch_ms": dk["avg_launch_ms"],
+            "flops_per_launch": dk["flops_per_launch"],
         }
         log(f"kernel stats: {json.dumps(stats)}")
         if not args.no_cpu_baseline:

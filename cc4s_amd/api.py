@@ -266,16 +266,33 @@ class CoupledCluster:
         Np = self.No + self.Nv
         if g.shape != (self.NG, Np, Np):
             raise ValueError(f"gamma shape {g.shape} != {(self.NG, Np, Np)}")
-        e = np.ascontiguousarray(eps, dtype=np.float64)
+        e = self._eps(eps)
         self._check(self.lib.cc4s_ccsd_set_inputs(self._ctx, _ptr(g), _ptr(e)))
 
     def set_inputs_device(self, gamma_device_ptr: int, eps):
-        e = np.ascontiguousarray(eps, dtype=np.float64)
+        e = self._eps(eps)
         self._check(self.lib.cc4s_ccsd_set_inputs_device(
             self._ctx, C.c_void_p(gamma_device_ptr), _ptr(e)))
 
-    def set_amplitudes(self, T1, T2):
+    def _eps(self, eps):
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        if e.shape != (self.No + self.Nv,):
+            raise ValueError(f"eps shape {e.shape} != {(self.No + self.Nv,)}")
+        return e
+
+    def _amplitude_arrays(self, T1, T2):
+        """(T1, T2) as Fortran arrays of the context's field, shapes checked:
+        the library reads Nv*No and Nv*Nv*No*No values."""
         t1, t2 = _fortran(T1, self.dtype), _fortran(T2, self.dtype)
+        if t1.shape != (self.Nv, self.No):
+            raise ValueError(f"T1 shape {t1.shape} != {(self.Nv, self.No)}")
+        if t2.shape != (self.Nv, self.Nv, self.No, self.No):
+            raise ValueError(f"T2 shape {t2.shape} != "
+                             f"{(self.Nv, self.Nv, self.No, self.No)}")
+        return t1, t2
+
+    def set_amplitudes(self, T1, T2):
+        t1, t2 = self._amplitude_arrays(T1, T2)
         self._check(self.lib.cc4s_ccsd_set_amplitudes(self._ctx, _ptr(t1), _ptr(t2)))
 
     def iterate(self):
@@ -310,15 +327,13 @@ class CoupledCluster:
         if amplitudes is None:
             t1 = t2 = None
         else:
-            t1 = _fortran(amplitudes[0], self.dtype)
-            t2 = _fortran(amplitudes[1], self.dtype)
+            t1, t2 = self._amplitude_arrays(*amplitudes)
         self._check(self.lib.cc4s_ccsd_residuum(self._ctx, _ptr(t1), _ptr(t2),
                                                 _ptr(R1), _ptr(R2)))
         return R1, R2
 
     def energy(self, amplitudes):
-        t1 = _fortran(amplitudes[0], self.dtype)
-        t2 = _fortran(amplitudes[1], self.dtype)
+        t1, t2 = self._amplitude_arrays(*amplitudes)
         out = (C.c_double * 6)()
         self._check(self.lib.cc4s_ccsd_energy(self._ctx, _ptr(t1), _ptr(t2), out))
         return complex(out[0], out[1]), complex(out[2], out[3]), complex(out[4], out[5])

@@ -361,3 +361,15 @@ def write_tensor(directory, name, values, dims, meta=None, unit=1.0, binary=True
             "".join(f"({v.real:.17g},{v.imag:.17g})\n" for v in flat))
     else:
         (d / f"{name}.elements").write_text("".join(f"{v:.17g}\n" for v in flat))
+
+
+def mp2_on_device(gamma: np.ndarray, eps: np.ndarray, No: int, field: str) -> float:
+    """Second-order energy of a vertex from the first pass of the GPU path
+    (R = V, T = -V/Delta; the energy of that pass is the closed form of
+    SecondOrderPerturbationTheory.cxx:108-127).  Usable as `mp2_fn` of
+    `generate` where the pphh slice is too large for the host computation."""
+    from cc4s_amd import api
+    NG, Np, _ = gamma.shape
+    with api.CoupledCluster(No, Np - No, NG, field=field) as cc:
+        cc.set_inputs(gamma, eps)
+        return cc.iterate()["energy"]

@@ -644,7 +644,7 @@ def triples_star(P, e_triples, pair_energies_cbs):
     return ratio * e_triples
 
 
-def perturbative_triples(gamma, eps, No, field, amplitudes):
+def perturbative_triples(gamma, eps, No, field, amplitudes, invariant=False):
     """(T) correlation energy as PerturbativeTriplesReference.cxx:55-180
     forms it (cT off):
       T'_abcijk = V_bcdk T_adij - conj(V_jklc) T_abil             (:106-107)
@@ -654,7 +654,20 @@ def perturbative_triples(gamma, eps, No, field, amplitudes):
       D = e_i + e_j + e_k - e_a - e_b - e_c                        (:128-134)
       E = sum conj(W)/D * conj(Z)/D * D = sum conj(W Z) / D       (:136-140,163)
     with V_ppph, V_hhhp, V_pphh from VertexCoulombIntegrals.  Returns Re E.
-    Memory O(o^3 v^3): small cases only."""
+
+    W and Z both pick up conj(u_a u_b u_c) u_i u_j u_k under an orbital
+    phase change (Gamma^G_pq -> conj(u_p) Gamma^G_pq u_q), so the reference's
+    conj(W Z) changes with the phases whenever V is not real.  invariant=True
+    returns Re sum conj(W) Z / D instead, which does not change and equals
+    the reference's value for real V.  Memory O(o^3 v^3): small cases only."""
+    W, Z, D = triples_parts(gamma, eps, No, field, amplitudes)
+    if invariant:
+        return float(np.real(np.sum(np.conj(W) * Z / D)))
+    return float(np.real(np.sum(np.conj(W * Z) / D)))
+
+
+def triples_parts(gamma, eps, No, field, amplitudes):
+    """(W, Z, D) of ``perturbative_triples`` as [a,b,c,i,j,k] arrays."""
     slices = slice_vertex(gamma, No)
     V = vertex_integrals(slices, field, ("pphh", "ppph", "hhhp"))
     T1, T2 = amplitudes
@@ -674,4 +687,4 @@ def perturbative_triples(gamma, eps, No, field, amplitudes):
     D = (eh[None, None, None, :, None, None] + eh[None, None, None, None, :, None]
          + eh[None, None, None, None, None, :] - ep[:, None, None, None, None, None]
          - ep[None, :, None, None, None, None] - ep[None, None, :, None, None, None])
-    return float(np.real(np.sum(np.conj(W * Z) / D)))
+    return W, Z, D

@@ -5,8 +5,13 @@ real_no8_nv40_ng200.json  the real No=8, Nv=40, NG=200 workload: SHA-256 of the
               final energies and checksums of the converged amplitudes.
 small_complex.json  a small complex workload (No=6, Nv=24, NG=60) run the
               same way, plus the residuum at seeded amplitudes.
+complex_no32_nv256_ng1500.json  the complex No=32, Nv=256, NG=1500 workload
+              with the vertex scaled by the "mp2_per_occupied" rule (the scale
+              at which the iteration converges), run the same way (about an
+              hour of host time on 8 cores; about 35 GB of memory).
 
-Usage: python tests/golden/make_golden.py [real_no8_nv40_ng200|small_complex ...]
+Usage: python tests/golden/make_golden.py [real_no8_nv40_ng200|small_complex|
+                                           complex_no32_nv256_ng1500 ...]
 """
 
 from __future__ import annotations
@@ -44,19 +49,25 @@ def checksums(T):
     return out
 
 
-def run_config(cfg, extra=None):
-    inp = syn.generate_config(cfg)
+def _progress(i, e, dR):
+    print(f"  iteration {i + 1}: E={np.real(e):.15f} dR={dR:.3e} "
+          f"({time.strftime('%H:%M:%S')})", flush=True)
+
+
+def run_config(cfg, extra=None, scale_rule="max_v"):
+    inp = syn.generate_config(cfg, scale_rule=scale_rule)
     P = O.CcsdProblem(inp.gamma, inp.eps, inp.No, cfg.field)
     t = time.time()
     r = O.coupled_cluster(P, max_iterations=cfg.max_iterations,
                           energy_convergence=cfg.energy_convergence,
                           amplitudes_convergence=cfg.amplitudes_convergence,
                           level_shift=cfg.level_shift,
-                          max_residua=cfg.max_residua)
+                          max_residua=cfg.max_residua, callback=_progress)
     dt = time.time() - t
     g = np.asfortranarray(inp.gamma).tobytes(order="F")
     out = {
         "config": cfg.__dict__,
+        "scale_rule": scale_rule,
         "scale": inp.scale,
         "gamma_sha256": hashlib.sha256(g).hexdigest(),
         "eps_sha256": hashlib.sha256(inp.eps.tobytes()).hexdigest(),
@@ -94,6 +105,12 @@ def main(which):
         data = run_config(cfg, residuum_fixture)
         (HERE / "small_complex.json").write_text(json.dumps(data, indent=1))
         print("small_complex", data["energy"], data["iterations"],
+              data["oracle_seconds"])
+    if "complex_no32_nv256_ng1500" in which:
+        data = run_config(syn.WORKLOADS["complex_no32_nv256_ng1500"],
+                          scale_rule="mp2_per_occupied")
+        (HERE / "complex_no32_nv256_ng1500.json").write_text(json.dumps(data, indent=1))
+        print("complex_no32_nv256_ng1500", data["energy"], data["iterations"],
               data["oracle_seconds"])
 
 

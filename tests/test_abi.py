@@ -72,3 +72,19 @@ def test_ctypes_structs_follow_the_header(cname, pyclass):
         parts = decl.split(None, 1)
         names += [n.strip().lstrip("*") for n in parts[1].split(",")]
     assert [f for f, _ in getattr(api, pyclass)._fields_] == names
+
+
+def test_amplitude_and_energy_shapes_are_checked_before_the_library_reads_them():
+    """The library reads Nv*No, Nv^2*No^2 and No+Nv values; a short or
+    transposed array is refused in Python (no device call is made)."""
+    import numpy as np
+    cc = object.__new__(api.CoupledCluster)
+    cc.No, cc.Nv, cc.NG, cc.dtype = 2, 3, 4, np.complex128
+    cc._amplitude_arrays(np.zeros((3, 2)), np.zeros((3, 3, 2, 2)))
+    with pytest.raises(ValueError, match="T1 shape"):
+        cc._amplitude_arrays(np.zeros((2, 3)), np.zeros((3, 3, 2, 2)))
+    with pytest.raises(ValueError, match="T2 shape"):
+        cc._amplitude_arrays(np.zeros((3, 2)), np.zeros((3, 3, 2)))
+    with pytest.raises(ValueError, match="eps shape"):
+        cc._eps(np.zeros(4))
+    assert cc._eps(np.zeros(5)).shape == (5,)

@@ -316,3 +316,37 @@ def test_ueg_triples_of_the_reference_test_suite():
         assert cc.run()["convergenceReached"]
         e = cc.triples()
     assert abs(e - ref["Triples"]["correlation"]) < 1e-10
+
+
+def test_ueg_under_an_orbital_phase_gauge():
+    """The electron-gas vertex with its orbitals' phases changed
+    (Gamma^G_pq -> conj(u_p) Gamma^G_pq u_q): V becomes complex, the
+    physics stays.  The GPU's converged CCSD energies equal the recorded
+    ones (1e-9 Eh), and its (T) of those amplitudes equals the oracle's
+    restatement of the reference's expression (1e-12 relative), which for
+    complex V moves away from the recorded (T) energy: the reference's
+    Re sum conj(W Z)/D is not invariant under the gauge."""
+    from oracle import ueg
+    from tests import properties as PR
+    gamma, eps, _ = ueg.generate(7, 26, 1.0)
+    ref = ueg.UEG_RS1_NO7_NV26
+    s = ref["settings"]
+    g = np.asfortranarray(gamma.copy())
+    PR.gauge_vertex(g, PR.phases(g.shape[1], 11))
+    with api.CoupledCluster(7, 26, g.shape[0], maxIterations=s["maxIterations"],
+                            energyConvergence=s["energyConvergence"],
+                            amplitudesConvergence=s["amplitudesConvergence"],
+                            maxResidua=s["maxResidua"],
+                            integralsSliceSize=s["integralsSliceSize"]) as cc:
+        cc.set_inputs(g, eps)
+        out = cc.run()
+        e_t = cc.triples()
+        T = cc.amplitudes()
+    assert out["convergenceReached"]
+    for key in ("correlation", "direct", "exchange", "secondOrder"):
+        assert abs(out["energy"][key] - ref["Ccsd"][key]) < 1e-9, key
+    e_oracle = O.perturbative_triples(g, eps, 7, "complex", T)
+    print(f"gauged UEG (T): GPU {e_t:.15f} oracle {e_oracle:.15f} "
+          f"recorded (real V) {ref['Triples']['correlation']:.15f}", flush=True)
+    assert abs(e_t - e_oracle) <= 1e-12 * abs(e_oracle)
+    assert abs(e_t - ref["Triples"]["correlation"]) > 1e-3

@@ -1,5 +1,7 @@
 """The FP64 MFMA contraction kernels against numpy einsum (GPU)."""
 
+import zlib
+
 import numpy as np
 import pytest
 
@@ -44,7 +46,8 @@ def _rand(idx, cplx, rng):
 @pytest.mark.parametrize("case", CASES, ids=["%s,%s->%s" % c for c in CASES])
 def test_contract_matches_einsum(case, cplx):
     ia, ib, ic = case
-    rng = np.random.default_rng(abs(hash(case)) % 2**32)
+    # seeded from a stable digest of the case (str hashing is salted per process)
+    rng = np.random.default_rng(zlib.crc32(",".join(case).encode()))
     A, B, C = (_rand(i, cplx, rng) for i in (ia, ib, ic))
     for alpha, beta in ((1.0, 0.0), (-0.5, 1.0)):
         ref = alpha * np.einsum(f"{ia},{ib}->{ic}", A, B) + beta * C
