@@ -35,4 +35,19 @@ enum class Field : int { Real = 0, Complex = 1 };
 
 inline size_t scalar_bytes(Field f) { return f == Field::Complex ? 16 : 8; }
 
+// Sum and maximum over the 64 lanes of a wavefront by butterfly exchanges
+// across lanes (lane l combines with lane l^1, l^2, ..., l^32): no LDS, no
+// barrier; every lane ends with the same bits and the order is fixed, so
+// the result is reproducible.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
 }  // namespace cc4s_amd

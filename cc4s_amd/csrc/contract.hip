@@ -367,61 +367,74 @@ __global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
     bCol[i] = bColOk[i] ? p.colB[n] : 0;
   }
 
-  int64_t aK[NKA], aKn[NKA], bK[NKB], bKn[NKB];
-  bool aKok[NKA], aKokn[NKA], bKok[NKB], bKokn[NKB];
-  auto load_koff = [&](int s, int64_t *ak, bool *akok, int64_t *bk, bool *bkok) {
+  // K offsets of two stages: stage t's in buffer t & 1, indexed by
+  // constants (the loop below is unrolled by two), so no register copies
+  // wait for a load
+  int64_t aK[2][NKA], bK[2][NKB];
+  bool aKok[2][NKA], bKok[2][NKB];
+  // every load is unconditional (out-of-range lanes read a valid element,
+  // kbeg < kend or offset 0, and are zeroed when stored to LDS): a load
+  // under a branch makes the compiler wait for it at the branch's end, which
+  // would stall the wave a whole K-stage early
+  auto load_koff = [&](int s, auto par_c) {
+    constexpr int pb = decltype(par_c)::value;
+    int64_t *ak = aK[pb], *bk = bK[pb];
+    bool *akok = aKok[pb], *bkok = bKok[pb];
     const int64_t kb = kbeg + static_cast<int64_t>(s) * BK;
 #pragma unroll
     for (int i = 0; i < NKA; ++i) {
       const int64_t k = kb + a_k(i);
       akok[i] = k < kend;
-      ak[i] = akok[i] ? p.colA[k] : 0;
+      ak[i] = p.colA[akok[i] ? k : kbeg];
     }
 #pragma unroll
     for (int i = 0; i < NKB; ++i) {
       const int64_t k = kb + b_k(i);
       bkok[i] = k < kend;
-      bk[i] = bkok[i] ? p.rowB[k] : 0;
+      bk[i] = p.rowB[bkok[i] ? k : kbeg];
     }
-  };
-  auto advance_koff = [&]() {
-#pragma unroll
-    for (int i = 0; i < NKA; ++i) { aK[i] = aKn[i]; aKok[i] = aKokn[i]; }
-#pragma unroll
-    for (int i = 0; i < NKB; ++i) { bK[i] = bKn[i]; bKok[i] = bKokn[i]; }
   };
 
   // staging registers; DEEP uses two sets, always indexed by constants
   constexpr int NSET = DEEP ? 2 : 1;
   double2 ra[NSET][EA], rb[NSET][EB];
-  auto load_data = [&](auto set_c) {
+  bool oka[NSET][EA], okb[NSET][EB];
+  auto load_data = [&](auto set_c, auto par_c) {
     constexpr int q = decltype(set_c)::value;
+    constexpr int pb = decltype(par_c)::value;
     const double2 *A = reinterpret_cast<const double2 *>(p.A);
     const double2 *B = reinterpret_cast<const double2 *>(p.B);
 #pragma unroll
     for (int i = 0; i < EA; ++i) {
       const int ir = AKC ? i : 0, ik = AKC ? 0 : i;
-      const bool ok = aRowOk[ir] && aKok[ik];
-      ra[q][i] = ok ? A[aRow[ir] + aK[ik]] : make_double2(0.0, 0.0);
-      ra[q][i].y *= p.signA;
+      oka[q][i] = aRowOk[ir] && aKok[pb][ik];
+      ra[q][i] = A[oka[q][i] ? aRow[ir] + aK[pb][ik] : 0];
     }
 #pragma unroll
     for (int i = 0; i < EB; ++i) {
       const int ir = BKC ? i : 0, ik = BKC ? 0 : i;
-      const bool ok = bColOk[ir] && bKok[ik];
-      rb[q][i] = ok ? B[bCol[ir] + bK[ik]] : make_double2(0.0, 0.0);
-      rb[q][i].y *= p.signB;
+      okb[q][i] = bColOk[ir] && bKok[pb][ik];
+      rb[q][i] = B[okb[q][i] ? bCol[ir] + bK[pb][ik] : 0];
     }
   };
+  // out-of-range elements are zeroed and the conjugation signs applied here,
+  // a stage after the loads were issued
   auto store_lds = [&](int buf, auto set_c) {
     constexpr int q = decltype(set_c)::value;
+    const double2 zero = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int i = 0; i < EA; ++i) As[buf][a_k(i)][a_m(i)] = ra[q][i];
+    for (int i = 0; i < EA; ++i)
+      As[buf][a_k(i)][a_m(i)] =
+          oka[q][i] ? make_double2(ra[q][i].x, ra[q][i].y * p.signA) : zero;
 #pragma unroll
-    for (int i = 0; i < EB; ++i) Bs[buf][b_k(i)][b_n(i)] = rb[q][i];
+    for (int i = 0; i < EB; ++i)
+      Bs[buf][b_k(i)][b_n(i)] =
+          okb[q][i] ? make_double2(rb[q][i].x, rb[q][i].y * p.signB) : zero;
   };
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, DEEP ? 1 : 0>;
+  using P0 = std::integral_constant<int, 0>;
+  using P1 = std::integral_constant<int, 1>;
 
   const int lane = t & 63;
   const int w = t >> 6;
@@ -440,25 +453,24 @@ __global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
   // prologue: stage 0 in LDS buffer 0; stage 1 in registers (DEEP: stage 1
   // in set 1, stage 2 in set 0)
   if (nstages > 0) {
-    load_koff(0, aK, aKok, bK, bKok);
-    if (nstages > 1) load_koff(1, aKn, aKokn, bKn, bKokn);
-    load_data(S0{});
+    load_koff(0, P0{});
+    if (nstages > 1) load_koff(1, P1{});
+    load_data(S0{}, P0{});
     store_lds(0, S0{});
     if (nstages > 1) {
-      advance_koff();
-      if (nstages > 2) load_koff(2, aKn, aKokn, bKn, bKokn);
-      load_data(S1{});
+      if (nstages > 2) load_koff(2, P0{});
+      load_data(S1{}, P1{});
       if (DEEP && nstages > 2) {
-        advance_koff();
-        if (nstages > 3) load_koff(3, aKn, aKokn, bKn, bKokn);
-        load_data(S0{});
+        if (nstages > 3) load_koff(3, P1{});
+        load_data(S0{}, P0{});
       }
     }
     __syncthreads();
   }
   // one K-stage: multiply LDS buffer s&1, then move stage s+1 from its
   // registers to the other buffer and load the next stage into them
-  auto stage = [&](int s, auto set_c) {
+  auto stage = [&](int s, auto set_c, auto par_c) {
+    constexpr int par = decltype(par_c)::value;  // s & 1
     const int buf = s & 1;
 #pragma unroll
     for (int kq = 0; kq < BK / 4; ++kq) {
@@ -515,20 +527,21 @@ __global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
       }
     }
     constexpr int AHEAD = DEEP ? 3 : 2;  // stage loaded into the freed registers
+    using PD = std::integral_constant<int, (par + AHEAD) & 1>;      // stage s+AHEAD
+    using PN = std::integral_constant<int, (par + AHEAD + 1) & 1>;  // stage s+AHEAD+1
     if (s + 1 < nstages) {
       store_lds(buf ^ 1, set_c);  // stage s+1
       if (s + AHEAD < nstages) {
-        advance_koff();
-        if (s + AHEAD + 1 < nstages) load_koff(s + AHEAD + 1, aKn, aKokn, bKn, bKokn);
-        load_data(set_c);
+        if (s + AHEAD + 1 < nstages) load_koff(s + AHEAD + 1, PN{});
+        load_data(set_c, PD{});
       }
     }
     __syncthreads();
   };
   // stage s+1 sits in set (s+1)&1 (DEEP) or in the single set
   for (int s = 0; s < nstages; s += 2) {
-    stage(s, S1{});
-    if (s + 1 < nstages) stage(s + 1, S0{});
+    stage(s, S1{}, P0{});
+    if (s + 1 < nstages) stage(s + 1, S0{}, P1{});
   }
 
   int64_t cCol[NB];
@@ -800,6 +813,11 @@ void ContractionEngine::apply_env_overrides() {
   // CC4S_GETT_GENERATION selects the complex FP64 kernel (plans made later)
   if (const char *e = std::getenv("CC4S_GETT_GENERATION"))
     set_complex_gett_generation(std::atoi(e));
+  if (const char *e = std::getenv("CC4S_MIXED_TERMS")) {
+    const int v = std::atoi(e);
+    CC4S_ASSERT(v == 12 || v == 16, "CC4S_MIXED_TERMS must be 12 or 16");
+    mixed_terms = v;
+  }
   if (const char *e = std::getenv("CC4S_MIXED_MAX_K")) {
     const long long v = std::atoll(e);
     CC4S_ASSERT(v >= 16, "CC4S_MIXED_MAX_K must be at least 16");
@@ -1132,7 +1150,8 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
                   kc, Np, Bop.conj, P.b_kc, Bpk, useScB, stream_);
     const double bc = c == 0 ? beta : 1.0;
     launch_bgemm(Ause, Buse, static_cast<double *>(C.ptr), W, P.rowC, P.colC, P.M,
-                 P.N, Mp, Np, kc, splits, alpha, bc, useScA, useScB, stream_);
+                 P.N, Mp, Np, kc, splits, alpha, bc, useScA, useScB, mixed_terms,
+                 stream_);
     if (splits > 1)
       launch_splitk_reduce(Field::Complex, W, static_cast<double *>(C.ptr), P.rowC,
                            P.colC, P.M, P.N, splits, alpha, bc, stream_);

@@ -72,8 +72,13 @@ class ContractionEngine {
   // split parts in IEEE half with per-operand power-of-two scaling (about
   // 22 significant bits per real part) instead of bf16 (about 16)
   bool mixed_half = false;
-  // CC4S_MIXED_SPLIT, CC4S_MIXED_MAX_K and CC4S_GETT_GENERATION, when set,
-  // replace the settings above and the complex kernel generation
+  // 16-bit products per complex multiply-add of the split kernel: 12 (the
+  // hi*hi, hi*lo and lo*hi products of both real parts) or 16 (also lo*lo,
+  // which lies 2^-22 below the product, under the FP32 sums' rounding)
+  int mixed_terms = 12;
+  // CC4S_MIXED_SPLIT, CC4S_MIXED_MAX_K, CC4S_MIXED_TERMS and
+  // CC4S_GETT_GENERATION, when set, replace the settings above and the
+  // complex kernel generation
   void apply_env_overrides();
   // While set, the split-bf16 packing of an operand whose memory starts at
   // `ptr` is kept for the whole K range and reused by later contractions of
@@ -152,8 +157,10 @@ struct MixedTile {
 };
 MixedTile mixed_tile();
 // packs k in [k0, k0+Kc) of an operand (FP64 complex, addressed by
-// offMN[mn] + offK[k]) into out[Kc][MNp] split words, zero-padded: bf16
-// parts, or (scale != null) half parts of the operand times scale[0]
+// offMN[mn] + offK[k]) into split words, zero-padded: bf16 parts, or
+// (scale != null) half parts of the operand times scale[0].  Layout
+// [Kc/2][MNp] of 16-byte pairs: pair (p, mn) holds the words of k = 2p and
+// 2p+1, word = (Re hi | Re lo << 16, Im hi | Im lo << 16); Kc is even.
 void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
                  int64_t MN, int64_t K, int64_t k0, int64_t Kc, int64_t MNp,
                  bool conj, bool kcontig, void *out, const double *scale,
@@ -165,12 +172,13 @@ void launch_split_scale(const double *X, const int64_t *offMN, const int64_t *of
                         int64_t MN, int64_t K, bool kcontig, double *partials,
                         double *scale, hipStream_t s);
 size_t split_scale_partials_bytes();
-// C (or split-K partials W) from packed operands [Kc][Mp] and [Kc][Np];
-// scaleA/scaleB (both or neither): half parts, the sums are divided by both
+// C (or split-K partials W) from packed operands ([Kc/2][Mp] and [Kc/2][Np]
+// pairs); scaleA/scaleB (both or neither): half parts, the sums are divided
+// by both; terms: 16-bit products per complex multiply-add (12 or 16)
 void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
                   const int64_t *rowC, const int64_t *colC, int64_t M, int64_t N,
                   int64_t Mp, int64_t Np, int64_t Kc, int splits, double alpha,
                   double beta, const double *scaleA, const double *scaleB,
-                  hipStream_t s);
+                  int terms, hipStream_t s);
 
 }  // namespace cc4s_amd

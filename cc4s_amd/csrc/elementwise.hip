@@ -133,60 +133,84 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
   }
 }
 
-// block-level sum of double2 values held by every thread
+// block-level sum of double2 values held by every thread: a wavefront
+// reduction per wave, then the kBlock/64 wave sums in wave order; every
+// thread gets the total
 __device__ double2 block_sum(double2 v, double2 *sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
+  v.x = wave_sum(v.x);
+  v.y = wave_sum(v.y);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = v;
   __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (t < w) {
-      sh[t].x += sh[t + w].x;
-      sh[t].y += sh[t + w].y;
-    }
-    __syncthreads();
+  double2 r = sh[0];
+#pragma unroll
+  for (int k = 1; k < kBlock / 64; ++k) {
+    r.x += sh[k].x;
+    r.y += sh[k].y;
   }
-  const double2 r = sh[0];
-  __syncthreads();
+  __syncthreads();  // sh may be reused by the next call
   return r;
 }
 
-// energy partials: per block (D, X) as two double2.  (a, b) tiles at fixed
-// (i, j): V_baij of the tile is staged through LDS, read along b
+// energy partials: per block (D, X) as two double2.  One work item is an
+// unordered pair of (a, b) tiles {(A, B), (B, A)} at fixed (i, j): both V
+// tiles move through LDS, so every V_abij and T2_abij is read once, along
+// its contiguous index, and serves the direct term of its own element and
+// the exchange term of the mirrored one (the tile A = B is a pair by itself)
 template <bool CPLX>
 __global__ __launch_bounds__(kBlock) void energy_kernel(const typename Sc<CPLX>::T *T,
                                                         const typename Sc<CPLX>::T *V,
                                                         int64_t No, int64_t Nv,
                                                         double2 *partials) {
   using S = Sc<CPLX>;
+  using E = typename S::T;
   constexpr int kt = kPairTile;
   __shared__ double2 sh[kBlock];
-  __shared__ typename S::T vt[kt][kt + 1];  // vt[b - b0][a - a0] = V_baij
-  const typename S::T *T1 = T;
-  const typename S::T *T2 = T + Nv * No;
+  // vx[u][w] = V at (a0 + u, b0 + w), vy[u][w] = V at (b0 + u, a0 + w)
+  __shared__ E vx[kt][kt + 1], vy[kt][kt + 1];
+  const E *T1 = T;
+  const E *T2 = T + Nv * No;
   const int64_t nt = (Nv + kt - 1) / kt;
-  const int64_t ntiles = nt * nt * No * No;
+  const int64_t npairs = nt * (nt + 1) / 2;
+  const int64_t ntiles = npairs * No * No;
   double2 d = make_double2(0.0, 0.0), x = make_double2(0.0, 0.0);
   for (int64_t tid = blockIdx.x; tid < ntiles; tid += gridDim.x) {
-    const int64_t ta = tid % nt, tb = (tid / nt) % nt, ij = tid / (nt * nt);
+    const int64_t pr = tid % npairs, ij = tid / npairs;
+    // pr = tb (tb + 1) / 2 + ta, ta <= tb
+    int64_t tb = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(pr) + 1.0) - 1.0) * 0.5);
+    while (tb * (tb + 1) / 2 > pr) --tb;
+    while ((tb + 1) * (tb + 2) / 2 <= pr) ++tb;
+    const int64_t ta = pr - tb * (tb + 1) / 2;
+    const bool diag = ta == tb;
     const int64_t i = ij % No, j = ij / No;
     const int64_t a0 = ta * kt, b0 = tb * kt, off = Nv * Nv * (i + No * j);
     __syncthreads();
     for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
       const int u = e % kt, w = e / kt;
-      vt[u][w] = (b0 + u < Nv && a0 + w < Nv) ? V[off + (b0 + u) + Nv * (a0 + w)] : S::zero();
+      vx[u][w] = (a0 + u < Nv && b0 + w < Nv) ? V[off + (a0 + u) + Nv * (b0 + w)] : S::zero();
+      if (!diag)
+        vy[u][w] = (b0 + u < Nv && a0 + w < Nv) ? V[off + (b0 + u) + Nv * (a0 + w)] : S::zero();
     }
     __syncthreads();
     for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
       const int u = e % kt, w = e / kt;
-      const int64_t a = a0 + u, b = b0 + w;
-      if (a >= Nv || b >= Nv) continue;
-      const int64_t idx = off + a + Nv * b;
-      const typename S::T tau = S::add(T2[idx], S::mul(T1[a + Nv * i], T1[b + Nv * j]));
       // tau * conj(V) = conj(conj(tau) * V)
-      const double2 dv = S::cdot(V[idx], tau);
-      const double2 xv = S::cdot(vt[w][u], tau);
-      d.x += dv.x; d.y += dv.y;
-      x.x += xv.x; x.y += xv.y;
+      const int64_t a = a0 + u, b = b0 + w;
+      if (a < Nv && b < Nv) {
+        const E tau = S::add(T2[off + a + Nv * b], S::mul(T1[a + Nv * i], T1[b + Nv * j]));
+        const double2 dv = S::cdot(vx[u][w], tau);              // V_abij
+        const double2 xv = S::cdot(diag ? vx[w][u] : vy[w][u], tau);  // V_baij
+        d.x += dv.x; d.y += dv.y;
+        x.x += xv.x; x.y += xv.y;
+      }
+      const int64_t a2 = b0 + u, b2 = a0 + w;   // element of the tile (B, A)
+      if (!diag && a2 < Nv && b2 < Nv) {
+        const E tau = S::add(T2[off + a2 + Nv * b2], S::mul(T1[a2 + Nv * i], T1[b2 + Nv * j]));
+        const double2 dv = S::cdot(vy[u][w], tau);
+        const double2 xv = S::cdot(vx[w][u], tau);
+        d.x += dv.x; d.y += dv.y;
+        x.x += xv.x; x.y += xv.y;
+      }
     }
   }
   const double2 ds = block_sum(d, sh);

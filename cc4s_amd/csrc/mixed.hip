@@ -1,28 +1,36 @@
-// Split-bf16 complex contraction (the mixed-precision path).
+// Split-16-bit complex contraction (the mixed-precision path).
 //
-// Every FP64 complex operand element x = xr + i xi is split into four bf16
-// values xr = rh + rl, xi = ih + il (rh = bf16(xr), rl = bf16(xr - rh), the
-// same for xi), i.e. about 16 significant bits per part.  An operand is first
-// packed, through the same offset tables as the FP64 kernel, into a dense
-// zero-padded [k][m] array of 8-byte (rh|rl<<16, ih|il<<16) words.  The
-// multiply kernel then feeds v_mfma_f32_32x32x16_bf16 with, per lane, one
-// complex k expanded to eight bf16:
-//   A:    [ah  ah  al  al  aih aih ail ail]   (ah = rh of a, ...)
-//   B_re: [bh  bl  bh  bl  -bih -bil -bih -bil]
-//   B_im: [bih bil bih bil  bh  bl  bh  bl ]
-// so that the 8-term dot products are exactly Re(a b) and Im(a b) of the split
-// values, accumulated in FP32 (32x32x16 = two complex k per instruction;
-// lanes 0-31 carry k, lanes 32-63 k+1).  Results are written to C in FP64
+// Every FP64 complex operand element x = xr + i xi is split into four 16-bit
+// values xr = rh + rl, xi = ih + il (rh = h(xr), rl = h(xr - rh), the same
+// for xi; h rounds to bf16 or, with a per-operand power-of-two scale, to
+// IEEE half).  An operand is first packed, through the same offset tables as
+// the FP64 kernel, into a dense zero-padded array of 8-byte words
+// (rh|rl<<16, ih|il<<16), two consecutive k per 16-byte pair.
+//
+// The multiply kernel feeds v_mfma_f32_32x32x16_{f16,bf16}, whose A and B
+// operands carry 8 values per lane (lanes 0-31: k-slots 0-7 of their row or
+// column, lanes 32-63: slots 8-15).  Default (12 products per complex
+// multiply-add): the six products that keep about 22 (half) or 16 (bf16)
+// significant bits of each real part,
+//   Re(a b) = ah bh + ah bl + al bh - aih bih - aih bil - ail bih,
+//   Im(a b) = ah bih + ah bil + al bih + aih bh + aih bl + ail bh,
+// are laid along k as A = [ah ah al aih aih ail], B_re = [bh bl bh -bih -bil
+// -bih], B_im = [bih bil bih bh bl bh]; four complex k of a half-wave fill
+// 24 slots, i.e. three MFMAs for Re and three for Im per 8 complex k.  The
+// dropped lo*lo products lie 2^-22 (half) below the product, under the
+// rounding of the FP32 sums.  With 16 products (terms = 16) one complex k
+// fills the 8 slots of a lane, A = [ah ah al al aih aih ail ail],
+// B_re = [bh bl bh bl -bih -bil -bih -bil], B_im = [bih bil bih bil bh bl bh
+// bl], two complex k per MFMA.  Sums are FP32; results go to C in FP64
 // (alpha * acc + beta * C) or to FP64 split-K partials.  K is processed in
 // chunks, so FP32 sums never run over more than one chunk.
 //
-// FP16 variant: the same split with IEEE half parts (11 significant bits
-// each, about 22 per real part).  Half has a narrow exponent range, so each
-// operand is first multiplied by a power of two that brings its largest
-// magnitude just below 2^15 (a max-|x| reduction over the operand), and the
-// kernel multiplies the FP32 sums by the inverse of both scales; every
-// scaling is exact.  Parts below 2^-24 of the scaled range flush towards
-// zero, which is far below the kept precision of the large elements.
+// Half parts have a narrow exponent range, so each operand is first
+// multiplied by a power of two that brings its largest magnitude just below
+// 2^15 (a max-|x| reduction over the operand), and the kernel multiplies the
+// FP32 sums by the inverse of both scales; every scaling is exact.  Parts
+// below 2^-24 of the scaled range flush towards zero, which is far below the
+// kept precision of the large elements.
 #include "contract.hpp"
 
 #include <algorithm>
@@ -96,13 +104,10 @@ __global__ __launch_bounds__(256) void maxabs_kernel(const double2 *X, const int
       m = fmax(m, fmax(fabs(x.x), fabs(x.y)));
     }
   }
-  sh[threadIdx.x] = m;
+  m = wave_max(m);
+  if (lane == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
 // scale[0] = 2^(15 - e) for max = f 2^e, f in [0.5, 1): the scaled maximum
@@ -112,16 +117,14 @@ __global__ __launch_bounds__(256) void scale_kernel(const double *partials, int 
   __shared__ double sh[256];
   double m = 0.0;
   for (int b = threadIdx.x; b < nblocks; b += 256) m = fmax(m, partials[b]);
-  sh[threadIdx.x] = m;
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
+    const double mx = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
     int e = 0;
-    (void)frexp(sh[0], &e);
-    scale[0] = sh[0] > 0.0 ? ldexp(1.0, 15 - e) : 1.0;
+    (void)frexp(mx, &e);
+    scale[0] = mx > 0.0 ? ldexp(1.0, 15 - e) : 1.0;
   }
 }
 
@@ -135,12 +138,12 @@ struct PackArgs {
   int64_t MNp;           // padded row length of the packed array
   double sign;           // -1 conjugates
   bool kcontig;          // source contiguous along k
-  uint2 *out;            // [Kc][MNp]
+  uint4 *out;            // [Kc/2][MNp] pairs of k
   const double *scale;   // FP16: the operand's power-of-two scale
 };
 
 // packs a 64(k) x 64(mn) tile: reads in the source's contiguous order,
-// transposes through LDS, writes mn-fastest
+// transposes through LDS, writes pairs of k mn-fastest
 template <bool FP16>
 __global__ __launch_bounds__(256) void pack_kernel(PackArgs p) {
   __shared__ uint2 tile[kPackTile][kPackTile + 1];
@@ -176,16 +179,20 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs p) {
     tile[kk][mm] = v;
   }
   __syncthreads();
-  for (int e = t; e < kPackTile * kPackTile; e += 256) {
-    const int mm = e % kPackTile, kk = e / kPackTile;
-    const int64_t krow = kb + kk, mn = mnb + mm;
-    if (krow < p.Kc && mn < p.MNp) p.out[krow * p.MNp + mn] = tile[kk][mm];
+  // kb and Kc are even: a pair never straddles the chunk's end
+  for (int e = t; e < kPackTile * kPackTile / 2; e += 256) {
+    const int mm = e % kPackTile, kp = e / kPackTile;
+    const int64_t krow = kb + 2 * kp, mn = mnb + mm;
+    if (krow < p.Kc && mn < p.MNp) {
+      const uint2 w0 = tile[2 * kp][mm], w1 = tile[2 * kp + 1][mm];
+      p.out[(krow >> 1) * p.MNp + mn] = make_uint4(w0.x, w0.y, w1.x, w1.y);
+    }
   }
 }
 
 struct BgArgs {
-  const uint2 *A;  // [Kc][Mp]
-  const uint2 *B;  // [Kc][Np]
+  const uint4 *A;  // [Kc/2][Mp] pairs of k
+  const uint4 *B;  // [Kc/2][Np] pairs of k
   double *C;
   double *W;
   const int64_t *rowC, *colC;
@@ -195,6 +202,7 @@ struct BgArgs {
   const double *scaleA, *scaleB;  // FP16: the operands' scales
 };
 
+// --- 16 products per complex multiply-add: one complex k per lane ---------
 // the eight 16-bit parts of one lane's operand (bf16 or half bit patterns);
 // the sign flip is bit 15 of each part in both formats
 __device__ __forceinline__ u32x4 expand_a(uint2 c) {
@@ -215,6 +223,55 @@ __device__ __forceinline__ u32x4 expand_b_im(uint2 c) {
   return w;
 }
 
+// --- 12 products per complex multiply-add: four complex k per half-wave ---
+// 32-bit words hold two 16-bit k-slots, the lower half first.  Per complex
+// k (word (x, y) = (rh | rl<<16, ih | il<<16)) three words of each operand:
+//   A:    (ah, ah)  (al, aih)   (aih, ail)
+//   B_re: (bh, bl)  (bh, -bih)  (-bil, -bih)
+//   B_im: (bih, bil) (bih, bh)  (bl, bh)
+// Twelve words (four k) are the k-slots of three MFMAs (four words each).
+struct Ops12 {
+  u32x4 v[3];
+};
+__device__ __forceinline__ unsigned int lo_lo(unsigned int x) {
+  return __builtin_amdgcn_perm(x, x, 0x01000100u);
+}
+// (x >> 16) | (y << 16)
+__device__ __forceinline__ unsigned int hi_lo(unsigned int y, unsigned int x) {
+  return __builtin_amdgcn_alignbit(y, x, 16);
+}
+// (x & 0xffff) | (y << 16)
+__device__ __forceinline__ unsigned int lo_pair(unsigned int y, unsigned int x) {
+  return __builtin_amdgcn_perm(y, x, 0x05040100u);
+}
+__device__ __forceinline__ void put12(Ops12 &o, int w, unsigned int v) {
+  o.v[w >> 2][w & 3] = v;
+}
+// the four k of two pairs p = (k0, k1), q = (k2, k3)
+__device__ __forceinline__ Ops12 expand12_a(uint4 p, uint4 q) {
+  Ops12 o;
+  const unsigned int x[4] = {p.x, p.z, q.x, q.z}, y[4] = {p.y, p.w, q.y, q.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    put12(o, 3 * k, lo_lo(x[k]));
+    put12(o, 3 * k + 1, hi_lo(y[k], x[k]));
+    put12(o, 3 * k + 2, y[k]);
+  }
+  return o;
+}
+__device__ __forceinline__ void expand12_b(uint4 p, uint4 q, Ops12 &re, Ops12 &im) {
+  const unsigned int x[4] = {p.x, p.z, q.x, q.z}, y[4] = {p.y, p.w, q.y, q.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    put12(re, 3 * k, x[k]);
+    put12(re, 3 * k + 1, lo_pair(y[k], x[k]) ^ 0x80000000u);
+    put12(re, 3 * k + 2, hi_lo(y[k], y[k]) ^ 0x80008000u);
+    put12(im, 3 * k, y[k]);
+    put12(im, 3 * k + 1, lo_pair(x[k], y[k]));
+    put12(im, 3 * k + 2, hi_lo(x[k], x[k]));
+  }
+}
+
 template <bool FP16>
 __device__ __forceinline__ f32x16 mfma_split(u32x4 a, u32x4 b, f32x16 c) {
   if constexpr (FP16)
@@ -229,18 +286,20 @@ constexpr int kBThreads = 512;
 
 // 8 waves in a 2 x 4 grid over a BM x BN complex tile; each wave owns
 // WM x WN = (WM/32) x (WN/32) blocks of 32x32, re and im accumulators.
-template <int BM, int BN, int BK, int WM, int WN, bool FP16>
+// LDS holds two K-stages of pairs of k (16 bytes per element pair).
+template <int BM, int BN, int BK, int WM, int WN, bool FP16, int TERMS>
 __global__ __launch_bounds__(kBThreads, 1) void bgemm_kernel(BgArgs p) {
   constexpr int MB = WM / 32, NB = WN / 32;
   constexpr int WAVES_N = BN / WN;
+  constexpr int BP = BK / 2;                     // pairs of k per stage
   static_assert((BM / WM) * (BN / WN) * 64 == kBThreads, "eight waves");
-  static_assert(BK % 2 == 0, "two complex k per MFMA");
-  constexpr int QA = BK * BM / 2 / kBThreads;  // uint4 (two elements) per thread
-  constexpr int QB = BK * BN / 2 / kBThreads;
-  static_assert(QA >= 1 && QB >= 1, "tile too small");
+  static_assert(BK % 8 == 0, "eight complex k per 12-product step");
+  static_assert(TERMS == 12 || TERMS == 16, "12 or 16 products");
+  constexpr int QA = BP * BM / kBThreads;  // pairs per thread and stage
+  constexpr int QB = BP * BN / kBThreads;
 
-  __shared__ uint2 As[2][BK][BM];
-  __shared__ uint2 Bs[2][BK][BN];
+  __shared__ uint4 As[2][BP][BM];
+  __shared__ uint4 Bs[2][BP][BN];
 
   const int t = threadIdx.x;
   const int ntiles = p.tilesM * p.tilesN;
@@ -259,29 +318,29 @@ __global__ __launch_bounds__(kBThreads, 1) void bgemm_kernel(BgArgs p) {
   const int nstages = kend > kbeg ? static_cast<int>((kend - kbeg) / BK) : 0;
 
   // staging registers, named so that none of them is placed in LDS
-  static_assert(QA == 2 && QB == 4, "staging is written out for 2 + 4 words");
+  static_assert(QA == 2 && QB == 4, "staging is written out for 2 + 4 pairs");
   uint4 ra0, ra1, rb0, rb1, rb2, rb3;
-  const int akk = t / (BM / 2), amm = 2 * (t % (BM / 2));  // + q * (kBThreads / (BM/2)) rows
-  const int bkk = t / (BN / 2), bnn = 2 * (t % (BN / 2));
-  constexpr int AR = kBThreads / (BM / 2), BR = kBThreads / (BN / 2);
+  constexpr int AR = kBThreads / BM, BR = kBThreads / BN;  // pair rows per pass
+  const int ap = t / BM, am = t % BM;
+  const int bp = t / BN, bn = t % BN;
   auto load = [&](int s) {
-    const int64_t kb = kbeg + static_cast<int64_t>(s) * BK;
-    const uint2 *a = p.A + (kb + akk) * p.Mp + m0 + amm;
-    const uint2 *b = p.B + (kb + bkk) * p.Np + n0 + bnn;
-    ra0 = *reinterpret_cast<const uint4 *>(a);
-    ra1 = *reinterpret_cast<const uint4 *>(a + AR * p.Mp);
-    rb0 = *reinterpret_cast<const uint4 *>(b);
-    rb1 = *reinterpret_cast<const uint4 *>(b + BR * p.Np);
-    rb2 = *reinterpret_cast<const uint4 *>(b + 2 * BR * p.Np);
-    rb3 = *reinterpret_cast<const uint4 *>(b + 3 * BR * p.Np);
+    const int64_t pb = (kbeg + static_cast<int64_t>(s) * BK) >> 1;
+    const uint4 *a = p.A + (pb + ap) * p.Mp + m0 + am;
+    const uint4 *b = p.B + (pb + bp) * p.Np + n0 + bn;
+    ra0 = a[0];
+    ra1 = a[AR * p.Mp];
+    rb0 = b[0];
+    rb1 = b[BR * p.Np];
+    rb2 = b[2 * BR * p.Np];
+    rb3 = b[3 * BR * p.Np];
   };
   auto store = [&](int buf) {
-    *reinterpret_cast<uint4 *>(&As[buf][akk][amm]) = ra0;
-    *reinterpret_cast<uint4 *>(&As[buf][akk + AR][amm]) = ra1;
-    *reinterpret_cast<uint4 *>(&Bs[buf][bkk][bnn]) = rb0;
-    *reinterpret_cast<uint4 *>(&Bs[buf][bkk + BR][bnn]) = rb1;
-    *reinterpret_cast<uint4 *>(&Bs[buf][bkk + 2 * BR][bnn]) = rb2;
-    *reinterpret_cast<uint4 *>(&Bs[buf][bkk + 3 * BR][bnn]) = rb3;
+    As[buf][ap][am] = ra0;
+    As[buf][ap + AR][am] = ra1;
+    Bs[buf][bp][bn] = rb0;
+    Bs[buf][bp + BR][bn] = rb1;
+    Bs[buf][bp + 2 * BR][bn] = rb2;
+    Bs[buf][bp + 3 * BR][bn] = rb3;
   };
 
   const int lane = t & 63;
@@ -305,28 +364,55 @@ __global__ __launch_bounds__(kBThreads, 1) void bgemm_kernel(BgArgs p) {
     if (nstages > 1) load(1);
     __syncthreads();
   }
+  const int ra = wm * WM + (lane & 31), rb = wn * WN + (lane & 31);
   for (int s = 0; s < nstages; ++s) {
     const int buf = s & 1;
     static_assert(MB == 2 && NB == 2, "fragments are written out for 2 x 2 blocks");
+    if constexpr (TERMS == 12) {
 #pragma unroll
-    for (int kp = 0; kp < BK / 2; ++kp) {
-      const int kk = 2 * kp + (lane >> 5);
-      const int ra0 = wm * WM + (lane & 31), rb0 = wn * WN + (lane & 31);
-      // named registers (not arrays), so no fragment is ever placed in LDS
-      const u32x4 a0 = expand_a(As[buf][kk][ra0]);
-      const u32x4 a1 = expand_a(As[buf][kk][ra0 + 32]);
-      const uint2 c0 = Bs[buf][kk][rb0];
-      const uint2 c1 = Bs[buf][kk][rb0 + 32];
-      const u32x4 br0 = expand_b_re(c0), bi0 = expand_b_im(c0);
-      const u32x4 br1 = expand_b_re(c1), bi1 = expand_b_im(c1);
-      accR[0][0] = mfma_split<FP16>(a0, br0, accR[0][0]);
-      accI[0][0] = mfma_split<FP16>(a0, bi0, accI[0][0]);
-      accR[0][1] = mfma_split<FP16>(a0, br1, accR[0][1]);
-      accI[0][1] = mfma_split<FP16>(a0, bi1, accI[0][1]);
-      accR[1][0] = mfma_split<FP16>(a1, br0, accR[1][0]);
-      accI[1][0] = mfma_split<FP16>(a1, bi0, accI[1][0]);
-      accR[1][1] = mfma_split<FP16>(a1, br1, accR[1][1]);
-      accI[1][1] = mfma_split<FP16>(a1, bi1, accI[1][1]);
+      for (int q = 0; q < BK / 8; ++q) {
+        // lanes 0-31: k = 8q .. 8q+3, lanes 32-63: k = 8q+4 .. 8q+7
+        const int pl = 4 * q + 2 * (lane >> 5);
+        const Ops12 a0 = expand12_a(As[buf][pl][ra], As[buf][pl + 1][ra]);
+        const Ops12 a1 = expand12_a(As[buf][pl][ra + 32], As[buf][pl + 1][ra + 32]);
+        Ops12 br0, bi0, br1, bi1;
+        expand12_b(Bs[buf][pl][rb], Bs[buf][pl + 1][rb], br0, bi0);
+        expand12_b(Bs[buf][pl][rb + 32], Bs[buf][pl + 1][rb + 32], br1, bi1);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          accR[0][0] = mfma_split<FP16>(a0.v[m], br0.v[m], accR[0][0]);
+          accI[0][0] = mfma_split<FP16>(a0.v[m], bi0.v[m], accI[0][0]);
+          accR[0][1] = mfma_split<FP16>(a0.v[m], br1.v[m], accR[0][1]);
+          accI[0][1] = mfma_split<FP16>(a0.v[m], bi1.v[m], accI[0][1]);
+          accR[1][0] = mfma_split<FP16>(a1.v[m], br0.v[m], accR[1][0]);
+          accI[1][0] = mfma_split<FP16>(a1.v[m], bi0.v[m], accI[1][0]);
+          accR[1][1] = mfma_split<FP16>(a1.v[m], br1.v[m], accR[1][1]);
+          accI[1][1] = mfma_split<FP16>(a1.v[m], bi1.v[m], accI[1][1]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int kp = 0; kp < BP; ++kp) {
+        // lanes 0-31: k = 2kp (first word of the pair), lanes 32-63: 2kp+1
+        const int h = lane >> 5;
+        const uint2 *Ap = reinterpret_cast<const uint2 *>(&As[buf][kp][0]);
+        const uint2 *Bp = reinterpret_cast<const uint2 *>(&Bs[buf][kp][0]);
+        // named registers (not arrays), so no fragment is ever placed in LDS
+        const u32x4 a0 = expand_a(Ap[2 * ra + h]);
+        const u32x4 a1 = expand_a(Ap[2 * (ra + 32) + h]);
+        const uint2 c0 = Bp[2 * rb + h];
+        const uint2 c1 = Bp[2 * (rb + 32) + h];
+        const u32x4 br0 = expand_b_re(c0), bi0 = expand_b_im(c0);
+        const u32x4 br1 = expand_b_re(c1), bi1 = expand_b_im(c1);
+        accR[0][0] = mfma_split<FP16>(a0, br0, accR[0][0]);
+        accI[0][0] = mfma_split<FP16>(a0, bi0, accI[0][0]);
+        accR[0][1] = mfma_split<FP16>(a0, br1, accR[0][1]);
+        accI[0][1] = mfma_split<FP16>(a0, bi1, accI[0][1]);
+        accR[1][0] = mfma_split<FP16>(a1, br0, accR[1][0]);
+        accI[1][0] = mfma_split<FP16>(a1, bi0, accI[1][0]);
+        accR[1][1] = mfma_split<FP16>(a1, br1, accR[1][1]);
+        accI[1][1] = mfma_split<FP16>(a1, bi1, accI[1][1]);
+      }
     }
     if (s + 1 < nstages) {
       store(buf ^ 1);
@@ -404,7 +490,7 @@ void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
   a.MNp = MNp;
   a.sign = conj ? -1.0 : 1.0;
   a.kcontig = kcontig;
-  a.out = static_cast<uint2 *>(out);
+  a.out = static_cast<uint4 *>(out);
   a.scale = scale;
   dim3 grid(static_cast<unsigned>((MNp + kPackTile - 1) / kPackTile),
             static_cast<unsigned>((Kc + kPackTile - 1) / kPackTile));
@@ -419,14 +505,17 @@ void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
                   const int64_t *rowC, const int64_t *colC, int64_t M, int64_t N,
                   int64_t Mp, int64_t Np, int64_t Kc, int splits, double alpha,
                   double beta, const double *scaleA, const double *scaleB,
-                  hipStream_t s) {
+                  int terms, hipStream_t s) {
   CC4S_ASSERT((scaleA == nullptr) == (scaleB == nullptr),
               "split-half operands need both scales");
+  CC4S_ASSERT(terms == 12 || terms == 16, "split products per complex MAC: 12 or 16");
+  CC4S_ASSERT(Kc % kXBK == 0 && Mp % kXBM == 0 && Np % kXBN == 0,
+              "packed operands are padded to whole tiles");
   BgArgs a;
   a.scaleA = scaleA;
   a.scaleB = scaleB;
-  a.A = static_cast<const uint2 *>(Apack);
-  a.B = static_cast<const uint2 *>(Bpack);
+  a.A = static_cast<const uint4 *>(Apack);
+  a.B = static_cast<const uint4 *>(Bpack);
   a.C = C;
   a.W = splits > 1 ? W : nullptr;
   a.rowC = rowC;
@@ -439,10 +528,14 @@ void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
   a.tilesM = static_cast<int>(Mp / kXBM);
   a.tilesN = static_cast<int>(Np / kXBN);
   dim3 grid(a.tilesM * a.tilesN, splits);
-  if (scaleA)
-    bgemm_kernel<kXBM, kXBN, kXBK, kXWM, kXWN, true><<<grid, kBThreads, 0, s>>>(a);
+  if (scaleA && terms == 12)
+    bgemm_kernel<kXBM, kXBN, kXBK, kXWM, kXWN, true, 12><<<grid, kBThreads, 0, s>>>(a);
+  else if (scaleA)
+    bgemm_kernel<kXBM, kXBN, kXBK, kXWM, kXWN, true, 16><<<grid, kBThreads, 0, s>>>(a);
+  else if (terms == 12)
+    bgemm_kernel<kXBM, kXBN, kXBK, kXWM, kXWN, false, 12><<<grid, kBThreads, 0, s>>>(a);
   else
-    bgemm_kernel<kXBM, kXBN, kXBK, kXWM, kXWN, false><<<grid, kBThreads, 0, s>>>(a);
+    bgemm_kernel<kXBM, kXBN, kXBK, kXWM, kXWN, false, 16><<<grid, kBThreads, 0, s>>>(a);
   CC4S_HIP(hipGetLastError());
 }
 

@@ -142,13 +142,15 @@ __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
       sum += weight * S::re_mul(w, zs) / d;
     }
   }
-  sh[threadIdx.x] = sum;
+  // wavefront sums, then the wave sums in wave order
+  sum = wave_sum(sum);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sum;
   __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sh[0];
+    for (int k = 1; k < kBlock / 64; ++k) b += sh[k];
+    acc[blockIdx.x] += b;
   }
-  if (threadIdx.x == 0) acc[blockIdx.x] += sh[0];
 }
 
 }  // namespace
