@@ -35,7 +35,7 @@ EXPORTS = (
     "cc4s_ccsd_set_profiling",
     "cc4s_ccsd_ladder_stats", "cc4s_ccsd_kernel_stats",
     "cc4s_ccsd_last_error", "cc4s_ccsd_destroy",
-    "cc4s_contract",
+    "cc4s_contract", "cc4s_ccsd_energy_noalloc",
 )
 
 
@@ -140,12 +140,15 @@ def load_library(path: pathlib.Path | str | None = None):
     lib.cc4s_contract.argtypes = [C.c_int, C.c_double, vp, C.c_char_p, i64p, C.c_int,
                                   vp, C.c_char_p, i64p, C.c_int, C.c_double, vp,
                                   C.c_char_p, i64p, C.c_int, C.c_int]
+    lib.cc4s_ccsd_energy_noalloc.argtypes = [C.c_int, C.c_int64, C.c_int64, vp, vp,
+                                             vp, dp]
     for name in ("cc4s_ccsd_create", "cc4s_ccsd_set_inputs",
                  "cc4s_ccsd_set_inputs_device", "cc4s_ccsd_set_amplitudes",
                  "cc4s_ccsd_iterate", "cc4s_ccsd_run", "cc4s_ccsd_get_amplitudes",
                  "cc4s_ccsd_residuum", "cc4s_ccsd_energy",
                  "cc4s_ccsd_set_profiling", "cc4s_ccsd_ladder_stats",
-                 "cc4s_ccsd_kernel_stats", "cc4s_contract"):
+                 "cc4s_ccsd_kernel_stats", "cc4s_contract",
+                 "cc4s_ccsd_energy_noalloc"):
         getattr(lib, name).restype = C.c_int
     if path is None:
         _lib = lib
@@ -433,3 +436,18 @@ def contract(alpha, A, idx_a, B, idx_b, beta, C_, idx_c, conj_a=False,
     if rc != 0:
         raise CcsdError(lib.cc4s_ccsd_last_error(None).decode())
     return c
+
+
+def energy_noalloc(field, No, Nv, T_device, V_device, stream=None):
+    """CoupledCluster::getEnergy of device-resident buffers through the
+    allocation-free entry point: T_device is the address of one amplitude set
+    [T1; T2] and V_device that of V_abij, both in the cc4s layout on the
+    current device.  Returns (E, D, X) as complex numbers."""
+    lib = load_library()
+    f = {"real": FIELD_REAL, "complex": FIELD_COMPLEX}.get(field, field)
+    out = (C.c_double * 6)()
+    rc = lib.cc4s_ccsd_energy_noalloc(int(f), int(No), int(Nv), C.c_void_p(T_device),
+                                      C.c_void_p(V_device), C.c_void_p(stream), out)
+    if rc != 0:
+        raise CcsdError(lib.cc4s_ccsd_last_error(None).decode())
+    return complex(out[0], out[1]), complex(out[2], out[3]), complex(out[4], out[5])

@@ -54,6 +54,8 @@ void fill(cc4s_ccsd_iteration *out, const IterationReport &r) {
 
 }  // namespace
 
+void cc4s_amd::set_last_error(const std::string &message) { g_error = message; }
+
 extern "C" {
 
 void cc4s_ccsd_default_params(cc4s_ccsd_params *p) {

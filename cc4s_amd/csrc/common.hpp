@@ -50,4 +50,8 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// Sets the message cc4s_ccsd_last_error(NULL) returns (calls without a
+// context).
+void set_last_error(const std::string &message);
+
 }  // namespace cc4s_amd

@@ -244,8 +244,26 @@ int cc4s_ccsd_kernel_stats(cc4s_ccsd_context *ctx, const char *variant,
                            int64_t *launches, double *milliseconds,
                            double *flops);
 
+/* Allocation-free energy.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It computes what
+ * cc4s_ccsd_energy computes (CoupledCluster::getEnergy,
+ * CoupledCluster.cxx:194-253), with the same kernels and reduction order,
+ * from device-resident buffers: T_device is one amplitude set [T1 (Nv*No);
+ * T2 (Nv*Nv*No*No)] and Vpphh_device the V_abij slice [Nv,Nv,No,No] (complex
+ * interleaved for CC4S_FIELD_COMPLEX), both in the cc4s layout, on the current
+ * HIP device; stream is a hipStream_t (NULL: the default stream).  out[0..5] =
+ * Re E, Im E, Re D, Im D, Re X, Im X (host memory).  No hipMalloc/hipFree:
+ * the reduction partials live in a static 64 KiB device array per GPU (the
+ * energy needs 32,800 bytes of it for any No, Nv), kept until the process
+ * ends.  Calls that overlap on one GPU race on that array.  Errors:
+ * cc4s_ccsd_last_error(NULL). */
+int cc4s_ccsd_energy_noalloc(int field, int64_t No, int64_t Nv,
+                             const void *T_device, const void *Vpphh_device,
+                             void *stream, double *out);
+
 /* Message of the last failed call on ctx (ctx == NULL: of the last failed
- * cc4s_ccsd_create or cc4s_contract). */
+ * cc4s_ccsd_create, cc4s_contract or cc4s_ccsd_energy_noalloc). */
 const char *cc4s_ccsd_last_error(const cc4s_ccsd_context *ctx);
 
 void cc4s_ccsd_destroy(cc4s_ccsd_context *ctx);
