@@ -122,10 +122,59 @@ def cpu_baseline(cfg, inp, iteration_flops, seconds_budget=20.0):
         "unit": "s/iteration",
         "cores": cores,
         "kind": "port",
+        "cpu_model": cpu_model(),
         "sample": (f"oracle (numpy/OpenBLAS FP64) particle-particle ladder, "
                    f"{pairs} slice pairs of size {s} of this workload in "
                    f"{dt:.1f} s = {rate / 1e9:.1f} GFLOP/s, extrapolated to "
                    f"the iteration's {iteration_flops:.3e} algorithmic flops"),
+    }
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline_full(cfg, inp):
+    """Times one complete iteration of the oracle on the host: its
+    CoupledCluster loop (oracle/ccsd_oracle.py, CoupledCluster.cxx:129-175)
+    run for two passes, the first of which has no residuum contraction
+    (R = V, Ccsd.cxx:404-409); the second pass is the residuum of every
+    statement of Ccsd.cxx:381-633, the denominator update, the DIIS step and
+    the energy.  Takes minutes at No=32, Nv=256."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except Exception:  # pragma: no cover
+        threadpool_limits = None
+    from oracle import ccsd_oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    ctx = threadpool_limits(limits=cores) if threadpool_limits else None
+    stamps = []
+    try:
+        P = O.CcsdProblem(inp.gamma, inp.eps, cfg.No, cfg.field)
+        t0 = time.perf_counter()
+        O.coupled_cluster(P, max_iterations=2, energy_convergence=0.0,
+                          amplitudes_convergence=0.0, max_residua=cfg.max_residua,
+                          callback=lambda *a: stamps.append(time.perf_counter()))
+    finally:
+        if ctx is not None:
+            ctx.__exit__(None, None, None)
+    first, second = stamps[0] - t0, stamps[1] - stamps[0]
+    return {
+        "value": second,
+        "unit": "s/iteration",
+        "cores": cores,
+        "kind": "port",
+        "cpu_model": cpu_model(),
+        "sample": (f"one complete pass of the oracle's CCSD loop (numpy/OpenBLAS FP64: "
+                   f"residuum, update, DIIS, energy) on this workload, timed after its "
+                   f"first pass (R = V, {first:.1f} s)"),
     }
 
 
@@ -372,6 +421,9 @@ def main():
                     help="also time this many mixed-precision passes (after "
                          "--warmup untimed ones) without converging")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-full", action="store_true",
+                    help="time one complete oracle iteration on the host instead of "
+                         "the bounded ladder sample (minutes at No=32, Nv=256)")
     ap.add_argument("--large", default="complex_no64_nv512_ng3000",
                     choices=("none",) + tuple(syn.WORKLOADS),
                     help="after the main workload, time one FP64 and one split "
@@ -591,7 +643,10 @@ def main():
             "flops_per_launch": dk["flops_per_launch"],
         }
         log(f"kernel stats: {json.dumps(stats)}")
-        if not args.no_cpu_baseline:
+        if args.cpu_baseline_full:
+            log("cpu baseline (one complete oracle iteration) ...")
+            out["cpu_baseline"] = cpu_baseline_full(cfg, inp)
+        elif not args.no_cpu_baseline:
             log("cpu baseline (oracle sample) ...")
             out["cpu_baseline"] = cpu_baseline(cfg, inp, iter_flops)
     print(json.dumps(out), flush=True)

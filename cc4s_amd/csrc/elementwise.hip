@@ -508,9 +508,10 @@ __global__ __launch_bounds__(kBlock) void dots_packed_kernel(DotArgs p, int coun
 // of the full symmetric tensor.  T2 is taken in (a, b) tiles at fixed
 // (i, j); with P = a + Nv i and Q = b + Nv j, a tile below the diagonal of
 // the (P, Q) tile grid has its packed index pair_index(P, Q) running along
-// b, so it is combined along b and written out along a through LDS; a tile
-// above runs along a; only the diagonal tiles are combined element by
-// element.
+// b: each pair there is combined once and written to its element in the
+// tile (along a, through LDS) and to its mirror in the tile above (along
+// b), so every packed element is read once; only the diagonal tiles are
+// combined element by element.
 template <bool CPLX>
 __global__ __launch_bounds__(kBlock) void combine_packed_kernel(DotArgs p, int count,
                                                                 typename Sc<CPLX>::T *out,
@@ -523,46 +524,57 @@ __global__ __launch_bounds__(kBlock) void combine_packed_kernel(DotArgs p, int c
   __shared__ T at[kt][kt + 1];
   const int64_t n1 = Nv * No;
   double2 nrm = make_double2(0.0, 0.0);
-  // the combination at packed index t, accumulating |R|^2 of it
-  auto combine_at = [&](int64_t t) {
+  // the combination at packed index t, accumulating |R|^2 of it `mult`
+  // times (the number of full-layout elements it is written to)
+  auto combine_at = [&](int64_t t, double mult) {
     T av = S::zero(), rv = S::zero();
     for (int k = 0; k < count; ++k) {
       av = S::add(av, S::scale(static_cast<const T *>(p.A[k])[t], p.c[k]));
       rv = S::add(rv, S::scale(static_cast<const T *>(p.R[k])[t], p.c[k]));
     }
-    nrm.x += S::cdot(rv, rv).x;
+    nrm.x += mult * S::cdot(rv, rv).x;
     return av;
   };
   for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n1;
        e += int64_t(gridDim.x) * kBlock)
-    out[e] = combine_at(e);
+    out[e] = combine_at(e, 1.0);
   const int64_t nt = (Nv + kt - 1) / kt;
   const int64_t ntiles = nt * nt * No * No;
   for (int64_t tid = blockIdx.x; tid < ntiles; tid += gridDim.x) {
     const int64_t ta = tid % nt, tb = (tid / nt) % nt, ij = tid / (nt * nt);
     const int64_t i = ij % No, j = ij / No;
-    const int64_t a0 = ta * kt, b0 = tb * kt, off = n1 + Nv * Nv * (i + No * j);
-    const bool below = i * nt + ta > j * nt + tb;  // every P > Q in the tile
-    __syncthreads();  // the previous tile's LDS reads are done
-    for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
-      const int u = e % kt, w = e / kt;
-      if (below) {
-        // u runs along b: P > Q, so pair_index(P, Q) = P (P + 1) / 2 + Q
-        const int64_t a = a0 + w, b = b0 + u;
-        if (a < Nv && b < Nv) at[u][w] = combine_at(n1 + pair_index(a + Nv * i, b + Nv * j));
-      } else {
-        const int64_t a = a0 + u, b = b0 + w;
-        if (a < Nv && b < Nv)
-          out[off + a + Nv * b] = combine_at(n1 + pair_index(a + Nv * i, b + Nv * j));
-      }
-    }
-    __syncthreads();
-    if (below) {
+    const int64_t tp = i * nt + ta, tq = j * nt + tb;  // tile ids along P and Q
+    if (tp < tq) continue;  // the mirror of a tile below the diagonal (uniform per block)
+    const int64_t a0 = ta * kt, b0 = tb * kt;
+    const int64_t offX = n1 + Nv * Nv * (i + No * j), offY = n1 + Nv * Nv * (j + No * i);
+    if (tp == tq) {
+      // diagonal tile (i == j, ta == tb): element by element, both halves
       for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
         const int u = e % kt, w = e / kt;
         const int64_t a = a0 + u, b = b0 + w;
-        if (a < Nv && b < Nv) out[off + a + Nv * b] = at[w][u];
+        if (a < Nv && b < Nv)
+          out[offX + a + Nv * b] = combine_at(n1 + pair_index(a + Nv * i, b + Nv * j), 1.0);
       }
+      continue;
+    }
+    // every P > Q in the tile: pair_index(P, Q) = P (P + 1) / 2 + Q runs along
+    // b.  Each pair is combined once and written to both of its elements:
+    // (b, a, j, i) directly along b, (a, b, i, j) along a through LDS.
+    __syncthreads();  // the previous tile's LDS reads are done
+    for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
+      const int u = e % kt, w = e / kt;
+      const int64_t a = a0 + w, b = b0 + u;
+      if (a < Nv && b < Nv) {
+        const T v = combine_at(n1 + pair_index(a + Nv * i, b + Nv * j), 2.0);
+        at[u][w] = v;
+        out[offY + b + Nv * a] = v;
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kt * kt; e += kBlock) {
+      const int u = e % kt, w = e / kt;
+      const int64_t a = a0 + u, b = b0 + w;
+      if (a < Nv && b < Nv) out[offX + a + Nv * b] = at[w][u];
     }
   }
   const double2 s = block_sum(nrm, sh);
