@@ -7,11 +7,16 @@ small_complex.json  a small complex workload (No=6, Nv=24, NG=60) run the
               same way, plus the residuum at seeded amplitudes.
 complex_no32_nv256_ng1500.json  the complex No=32, Nv=256, NG=1500 workload
               with the vertex scaled by the "mp2_per_occupied" rule (the scale
-              at which the iteration converges), run the same way (about an
-              hour of host time on 8 cores; about 35 GB of memory).
+              at which the iteration converges), run the same way (about 1.5
+              hours of host time on 8 cores; about 35 GB of memory).
+complex_no32_nv256_ng1500_residuum.json  one oracle residuum of that workload
+              at seeded amplitudes: the scale, input SHA-256 values,
+              checksums of R1 and R2, R1 and R2 at seeded sample positions
+              (cc4s-layout flat indices) and the energy of the amplitudes.
 
 Usage: python tests/golden/make_golden.py [real_no8_nv40_ng200|small_complex|
-                                           complex_no32_nv256_ng1500 ...]
+                                           complex_no32_nv256_ng1500|
+                                           complex_no32_nv256_ng1500_residuum ...]
 """
 
 from __future__ import annotations
@@ -93,6 +98,55 @@ def residuum_fixture(P, inp):
         "energy": [E.real, E.imag, D.real, D.imag, X.real, X.imag]}}
 
 
+# samples of the config-size residuum fixture
+RESIDUUM_AMPLITUDE_SEED = 77
+RESIDUUM_SAMPLE_SEED = 2026
+RESIDUUM_SAMPLES_R1, RESIDUUM_SAMPLES_R2 = 256, 2048
+
+
+def sample_positions(n, count, seed):
+    """Sorted distinct flat positions in [0, n), from the SplitMix64 stream
+    (uniforms of stream 5), so the test draws the same positions."""
+    u = syn.uniforms(seed, 5, np.arange(4 * count))
+    draws = (u * n).astype(np.int64)
+    _, first = np.unique(draws, return_index=True)   # first draw of each value
+    pos = np.sort(draws[np.sort(first)[:count]])
+    assert len(pos) == count
+    return pos
+
+
+def residuum_at_config_size(cfg, scale_rule="mp2_per_occupied"):
+    inp = syn.generate_config(cfg, scale_rule=scale_rule)
+    P = O.CcsdProblem(inp.gamma, inp.eps, inp.No, cfg.field)
+    T = syn.random_amplitudes(cfg.No, cfg.Nv, cfg.field, seed=RESIDUUM_AMPLITUDE_SEED)
+    t = time.time()
+    R1, R2 = P.residuum(T)
+    dt = time.time() - t
+    E, D, X = O.energy(P, T)
+    f1 = np.asfortranarray(R1).ravel(order="F")
+    f2 = np.asfortranarray(R2).ravel(order="F")
+    p1 = sample_positions(f1.size, RESIDUUM_SAMPLES_R1, RESIDUUM_SAMPLE_SEED)
+    p2 = sample_positions(f2.size, RESIDUUM_SAMPLES_R2, RESIDUUM_SAMPLE_SEED + 1)
+    g = np.asfortranarray(inp.gamma).tobytes(order="F")
+    return {
+        "config": cfg.__dict__,
+        "scale_rule": scale_rule,
+        "scale": inp.scale,
+        "gamma_sha256": hashlib.sha256(g).hexdigest(),
+        "eps_sha256": hashlib.sha256(inp.eps.tobytes()).hexdigest(),
+        "amplitudes": {"generator": "synthetic.random_amplitudes", "seed":
+                       RESIDUUM_AMPLITUDE_SEED, "t1_scale": 0.02, "t2_scale": 0.01},
+        "energy": [E.real, E.imag, D.real, D.imag, X.real, X.imag],
+        "R1": checksums((R1, R1))["T1"], "R2": checksums((R2, R2))["T2"],
+        "R1_max_abs": float(np.max(np.abs(f1))), "R2_max_abs": float(np.max(np.abs(f2))),
+        "R1_samples": {"seed": RESIDUUM_SAMPLE_SEED, "positions": p1.tolist(),
+                       "re": f1[p1].real.tolist(), "im": f1[p1].imag.tolist()},
+        "R2_samples": {"seed": RESIDUUM_SAMPLE_SEED + 1, "positions": p2.tolist(),
+                       "re": f2[p2].real.tolist(), "im": f2[p2].imag.tolist()},
+        "oracle_seconds": dt,
+    }
+
+
 def main(which):
     if "real_no8_nv40_ng200" in which:
         data = run_config(syn.WORKLOADS["real_no8_nv40_ng200"], residuum_fixture)
@@ -111,6 +165,12 @@ def main(which):
                           scale_rule="mp2_per_occupied")
         (HERE / "complex_no32_nv256_ng1500.json").write_text(json.dumps(data, indent=1))
         print("complex_no32_nv256_ng1500", data["energy"], data["iterations"],
+              data["oracle_seconds"])
+    if "complex_no32_nv256_ng1500_residuum" in which:
+        data = residuum_at_config_size(syn.WORKLOADS["complex_no32_nv256_ng1500"])
+        (HERE / "complex_no32_nv256_ng1500_residuum.json").write_text(
+            json.dumps(data, indent=1))
+        print("complex_no32_nv256_ng1500_residuum", data["energy"][0],
               data["oracle_seconds"])
 
 
