@@ -472,8 +472,25 @@ __global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
   auto stage = [&](int s, auto set_c, auto par_c) {
     constexpr int par = decltype(par_c)::value;  // s & 1
     const int buf = s & 1;
+    constexpr int AHEAD = DEEP ? 3 : 2;  // stage loaded into the freed registers
+    using PD = std::integral_constant<int, (par + AHEAD) & 1>;      // stage s+AHEAD
+    using PN = std::integral_constant<int, (par + AHEAD + 1) & 1>;  // stage s+AHEAD+1
+    // stage s+1 moves from its registers to the other LDS buffer (read in
+    // stage s-1, released by that stage's barrier) and stage s+AHEAD is
+    // loaded half way through this stage's MFMAs, so the wave keeps issuing
+    // MFMAs around its memory instructions
+    auto advance = [&]() {
+      if (s + 1 < nstages) {
+        store_lds(buf ^ 1, set_c);  // stage s+1
+        if (s + AHEAD < nstages) {
+          if (s + AHEAD + 1 < nstages) load_koff(s + AHEAD + 1, PN{});
+          load_data(set_c, PD{});
+        }
+      }
+    };
 #pragma unroll
     for (int kq = 0; kq < BK / 4; ++kq) {
+      if (kq == BK / 8) advance();
       const int kr = kq * 4 + (lane >> 4);
       double2 a[MB], b[NB];
 #pragma unroll
@@ -524,16 +541,6 @@ __global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
 #pragma unroll
           for (int j = 0; j < NB; ++j)
             accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].x, accI[i][j], 0, 0, 0);
-      }
-    }
-    constexpr int AHEAD = DEEP ? 3 : 2;  // stage loaded into the freed registers
-    using PD = std::integral_constant<int, (par + AHEAD) & 1>;      // stage s+AHEAD
-    using PN = std::integral_constant<int, (par + AHEAD + 1) & 1>;  // stage s+AHEAD+1
-    if (s + 1 < nstages) {
-      store_lds(buf ^ 1, set_c);  // stage s+1
-      if (s + AHEAD < nstages) {
-        if (s + AHEAD + 1 < nstages) load_koff(s + AHEAD + 1, PN{});
-        load_data(set_c, PD{});
       }
     }
     __syncthreads();
