@@ -204,7 +204,7 @@ def dominant_kernel(stats):
     }
 
 
-def large_workload_report(wname, scale):
+def large_workload_report(wname, scale, fused_ladder=False):
     """One FP64 pass and one split (mixed) pass of a large workload, timed in
     this run after the passes of the main workload: the first pass of each
     context (R = V, no residuum contraction) is untimed, the second evaluates
@@ -232,6 +232,7 @@ def large_workload_report(wname, scale):
     ladder = lad_flops / (lad_ms * 1e-3) / 1e12 if lad_ms else None
     out["fp64"] = {
         "seconds_per_iteration": r["seconds"],
+        "ladder_ms": lad_ms,
         "first_pass_seconds": first["seconds"],
         "residuum_seconds": r["residuum_seconds"],
         "update_and_diis_seconds": r["mixer_seconds"],
@@ -243,6 +244,21 @@ def large_workload_report(wname, scale):
         "ladder_frac_fp64_mfma_peak": ladder / FP64_MFMA_PEAK_TFLOPS if ladder else None,
         "dominant_kernel": dominant_kernel(stats),
     }
+    if fused_ladder:
+        with make_context(cfg, api.PRECISION_FP64, 1e-5, ladder="fused") as cc:
+            cc.set_inputs(inp.gamma, inp.eps)
+            cc.iterate()
+            cc.set_profiling(True)
+            fr = cc.iterate()
+            f_ms, f_flops, _ = cc.ladder_stats()
+        f_rate = f_flops / (f_ms * 1e-3) / 1e12 if f_ms else None
+        out["fp64_fused_ladder"] = {
+            "seconds_per_iteration": fr["seconds"],
+            "ladder_ms": f_ms, "ladder_tflops_per_s": f_rate,
+            "ladder_frac_fp64_mfma_peak": f_rate / FP64_MFMA_PEAK_TFLOPS if f_rate else None,
+            "abs_energy_difference_vs_slab": abs(fr["energy"] - r["energy"]),
+        }
+        log(f"large workload FP64 pass, fused ladder: {json.dumps(out['fp64_fused_ladder'])}")
     with make_context(cfg, api.PRECISION_MIXED, 1e-5) as cc:
         cc.set_inputs(inp.gamma, inp.eps)
         cc.iterate()
@@ -271,10 +287,42 @@ def large_workload_report(wname, scale):
     return out
 
 
-def make_context(cfg, precision, switch, restart=None, keep=None, refine=None):
+def ladder_comparison(cfg, inp, slab_passes, lad_slab, steps=2):
+    """The particle-particle ladder of FP64 passes with the V slab (timed in
+    the main loop) and with the fused kernel (V formed in LDS, no slab): a
+    context with the fused ladder runs its first pass (R = V, no ladder)
+    untimed and `steps` passes with the ladder timed by HIP events around
+    it.  Its energies are compared with the slab context's passes of the same
+    index (same amplitudes up to rounding)."""
+    def rate(ms, flops, calls):
+        t = flops / (ms * 1e-3) / 1e12 if ms else None
+        return {"ladder_ms_per_pass": ms / max(1, calls), "ladder_tflops_per_s": t,
+                "ladder_frac_fp64_mfma_peak": t / FP64_MFMA_PEAK_TFLOPS if t else None,
+                "flops_per_pass": flops / max(1, calls)}
+    out = {"slab": rate(*lad_slab)}
+    with make_context(cfg, api.PRECISION_FP64, 1e-5, ladder="fused") as cc:
+        cc.set_inputs(inp.gamma, inp.eps)
+        passes = [cc.iterate()]
+        cc.set_profiling(True)
+        for _ in range(steps):
+            passes.append(cc.iterate())
+        out["fused"] = rate(*cc.ladder_stats())
+        out["fused"]["seconds_per_iteration"] = float(np.mean([p["seconds"] for p in passes[1:]]))
+    n = min(len(passes), len(slab_passes))
+    out["max_abs_energy_difference_vs_slab"] = max(
+        abs(a["energy"] - b["energy"]) for a, b in zip(passes[:n], slab_passes[:n]))
+    out["fused_over_slab_time"] = (out["fused"]["ladder_ms_per_pass"]
+                                   / out["slab"]["ladder_ms_per_pass"])
+    log(f"ladder comparison: {json.dumps(out)}")
+    return out
+
+
+def make_context(cfg, precision, switch, restart=None, keep=None, refine=None,
+                 ladder="slab"):
     """A context of the workload; mixed contexts take the split settings of
     the command line (MIXED_OPTIONS)."""
     extra = dict(MIXED_OPTIONS) if precision == api.PRECISION_MIXED else {}
+    extra["ladder"] = ladder
     if refine is not None and precision == api.PRECISION_MIXED:
         extra["mixedRefine"] = refine
     return api.CoupledCluster(cfg.No, cfg.Nv, cfg.NG, field=cfg.field,
@@ -428,6 +476,11 @@ def main():
                     choices=("none",) + tuple(syn.WORKLOADS),
                     help="after the main workload, time one FP64 and one split "
                          "pass of this workload (none: skip)")
+    ap.add_argument("--ladder-compare", default="auto", choices=("auto", "on", "off"),
+                    help="time FP64 passes with the fused ladder next to the slab "
+                         "(auto: on for the No=32 Nv=256 workload)")
+    ap.add_argument("--large-fused-ladder", action="store_true",
+                    help="also time an FP64 pass of the large workload with the fused ladder")
     ap.add_argument("--triples", action="store_true",
                     help="time the (T) correction at the FP64 fixed point")
     ap.add_argument("--adjoint", default="auto", choices=("auto", "on", "off"),
@@ -492,6 +545,7 @@ def main():
         stats = {v: cc.kernel_stats(v) for v in VARIANTS}
         lad = cc.ladder_stats()
         cc.set_profiling(False)
+        main_passes = list(passes)
         if args.converge in ("both", "fp64"):
             passes = converge(cc, "fp64", passes)
             out_conv["fp64"] = time_to_convergence(passes)
@@ -578,9 +632,15 @@ def main():
             out_conv[key]["abs_energy_difference"] = abs(
                 out_conv["fp64"]["energy"] - out_conv[key]["energy"])
 
+    ladders = None
+    if value is not None and cfg.field == "complex" and (
+            args.ladder_compare == "on" or (args.ladder_compare == "auto"
+                                            and wname == "complex_no32_nv256_ng1500")):
+        ladders = ladder_comparison(cfg, inp, main_passes, lad)
+
     large = None
     if args.large != "none" and args.large != wname:
-        large = large_workload_report(args.large, args.scale)
+        large = large_workload_report(args.large, args.scale, args.large_fused_ladder)
 
     out = {
         "metric": METRIC,
@@ -613,6 +673,8 @@ def main():
         out["config"]["triples"] = triples
     if large is not None:
         out["config"]["large_workload"] = large
+    if ladders is not None:
+        out["config"]["ladder_comparison"] = ladders
     if value is not None:
         dk = dominant_kernel(stats)
         lad_ms, lad_flops, _ = lad

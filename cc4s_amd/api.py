@@ -61,6 +61,7 @@ class Params(C.Structure):
         ("mixed_refine", C.c_int),
         ("mixed_split", C.c_int),
         ("mixed_max_k", C.c_int64),
+        ("ladder", C.c_int),
     ]
 
 
@@ -170,6 +171,9 @@ def _fortran(a, dtype):
 
 
 MIXED_SPLITS = {"bf16": 0, "half": 1}
+# particle-particle ladder of FP64 complex passes: V slab through HBM, or one
+# fused kernel that forms V in LDS
+LADDERS = {"slab": 0, "fused": 1}
 
 
 def finite_size_correction(grid, B, S, N=20, vertex_unit=1.0, grid_unit=1.0,
@@ -202,7 +206,7 @@ class CoupledCluster:
                  integralsSliceSize=0, device=0, precision=PRECISION_FP64,
                  mixedSwitch=1e-5, method="Ccsd", linearized=False,
                  adjacentPairsExchange=False, mixedRestart=None, mixedRestartKeep=None,
-                 mixedRefine=None, mixedSplit=None, mixedMaxK=None):
+                 mixedRefine=None, mixedSplit=None, mixedMaxK=None, ladder="slab"):
         self.lib = load_library()
         p = default_params()
         p.field = FIELD_COMPLEX if field == "complex" else FIELD_REAL
@@ -234,6 +238,9 @@ class CoupledCluster:
             p.mixed_split = MIXED_SPLITS[mixedSplit]
         if mixedMaxK is not None:
             p.mixed_max_k = int(mixedMaxK)
+        if ladder not in LADDERS:
+            raise CcsdError(f"unknown ladder {ladder!r}")
+        p.ladder = LADDERS[ladder]
         self.params = p
         self.field = field
         self.No, self.Nv, self.NG = No, Nv, NG

@@ -76,6 +76,7 @@ void cc4s_ccsd_default_params(cc4s_ccsd_params *p) {
   p->mixed_refine = 1;
   p->mixed_split = 1;
   p->mixed_max_k = 2048;
+  p->ladder = CC4S_LADDER_SLAB;
 }
 
 int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
@@ -123,6 +124,9 @@ int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
     p.max_residua = params->max_residua;
     p.ccsd.ppl = params->ppl != 0;
     p.ccsd.integrals_slice_size = params->integrals_slice_size;
+    CC4S_ASSERT(params->ladder == CC4S_LADDER_SLAB || params->ladder == CC4S_LADDER_FUSED,
+                "ladder must be CC4S_LADDER_SLAB or CC4S_LADDER_FUSED");
+    p.ccsd.ladder_fused = params->ladder == CC4S_LADDER_FUSED;
     auto ctx = std::make_unique<cc4s_ccsd_context>();
     ctx->params = *params;
     ctx->cc.reset(new CoupledCluster(p, params->device));

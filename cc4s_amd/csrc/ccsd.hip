@@ -5,7 +5,10 @@
 // ladder forms V_xycd slabs from the dressed vertices and never holds the
 // full v^4 tensor.
 #include "ccsd.hpp"
+#include "ladder.hpp"
 #include "triples.hpp"
+
+#include <cstdlib>
 
 #include "elementwise.hpp"
 
@@ -35,6 +38,7 @@ struct Ccsd::Work {
   Tensor DGph, DGpp, DGhh, Dpp;  // dressed vertices
   Tensor wG, Zgvo, Wgvo, Wvooo, Woo, Zgoo;
   Tensor slab, Rxy;
+  FusedLadderTable fused;  // pair blocks of the fused ladder
   int64_t NGe = 0;  // length of the G index of the real-path vertex views
   bool hermitian = false;
 };
@@ -179,6 +183,8 @@ void Ccsd::get_residuum(const void *T, void *R) {
 // (x in slice n, y in slice m), R_xyij = V_xycd tau_cdij added to R2 at
 // (a,b) and, for a > b, transposed at (b,a,j,i).  The x range of a slice is
 // processed in chunks so that a V slab stays within a fixed HBM budget.
+// With ladder_fused (FP64 complex passes) one kernel forms V in LDS and
+// contracts it, and no slab exists.
 void Ccsd::ladder(const View &left, const View &right, const View &tauv,
                   const View &R2) {
   auto &w = *w_;
@@ -186,10 +192,11 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
   const int64_t v = s_.Nv, o = s_.No, s = opt_.integrals_slice_size;
   const int64_t ns = (v + s - 1) / s;
   const size_t es = scalar_bytes(f);
+  const bool fused = opt_.ladder_fused && f == Field::Complex && !E_.mixed();
   const size_t slab_budget = size_t(2) << 30;
   const int64_t per_x = v * v * std::min(s, v) * static_cast<int64_t>(es);
   const int64_t cx_max = std::max<int64_t>(1, std::min<int64_t>(s, slab_budget / per_x));
-  if (w.slab.size() < v * v * cx_max * std::min(s, v)) {
+  if (!fused && w.slab.size() < v * v * cx_max * std::min(s, v)) {
     w.slab.reset(f, "cxdy", {v, cx_max, v, std::min(s, v)});
     w.Rxy.reset(f, "xyij", {cx_max, std::min(s, v), o, o});
   }
@@ -200,10 +207,15 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
     CC4S_HIP(hipEventRecord(e0, E_.stream()));
   }
   const double flops0 = E_.stats.flops;
+  if (fused) {
+    E_.stats.flops += ladder_fused(left.as("Gxc"), right.as("Gyd"), tauv.as("cdij"),
+                                   R2.as("xyij"), s, w.fused, E_.stream());
+    E_.stats.launches += 1;
+  }
   // tau is read unchanged by every slab contraction: in split-bf16 mode it
   // is packed once for all of them
   if (E_.mixed()) E_.set_operand_cache(tauv.ptr);
-  for (int64_t m = 0; m < ns; ++m) {
+  for (int64_t m = 0; m < ns && !fused; ++m) {
     for (int64_t n = m; n < ns; ++n) {
       const int64_t a = n * s, b = m * s;
       const int64_t Nx = std::min(s, v - a), Ny = std::min(s, v - b);
@@ -539,7 +551,9 @@ double Ccsd::triples(const void *Tp, const double *eps_h, const double *eps_p) {
   const View Vh0 = cplx ? w.Vhhhp.v.conjugated() : w.Vhhhp.v;   // conj V_hhhp
   auto vh = [&](const char *names) { return Vh0.as(names); };
 
-  const size_t budget = size_t(4) << 30;
+  // bytes of W per hole-triple chunk (CC4S_TRIPLES_W_BYTES overrides)
+  size_t budget = size_t(4) << 30;
+  if (const char *e = std::getenv("CC4S_TRIPLES_W_BYTES")) budget = std::strtoull(e, nullptr, 10);
   const int64_t per_k = v * v * v * static_cast<int64_t>(es);
   const int64_t kc = std::max<int64_t>(1, std::min<int64_t>(o, static_cast<int64_t>(budget) / per_k));
   Tensor Wt(f, "abck", {v, v, v, kc});

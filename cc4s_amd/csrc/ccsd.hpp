@@ -28,6 +28,9 @@ struct CcsdOptions {
   int64_t integrals_slice_size = 0;  // 0: No (Ccsd.cxx:287-289)
   bool linearized = false;           // Drccd.cxx:53-55
   bool adjacent_pairs_exchange = false;  // Drccd.cxx:56-58
+  // FP64 complex ladder as one fused kernel (V formed in LDS, never in HBM)
+  // instead of the slab GETTs; split passes keep the slab
+  bool ladder_fused = false;
 };
 
 // time per residuum stage, filled when profiling is on

@@ -706,6 +706,11 @@ class Runner {
     else if (precision != "fp64")
       throw std::runtime_error("unknown precision '" + precision + "' (fp64 | mixed)");
     p.mixed_switch = yaml::get_real(args, "mixedSwitch", p.mixed_switch);
+    const std::string ladder = yaml::get_text(args, "ladder", "slab");
+    if (ladder == "fused")
+      p.ladder = CC4S_LADDER_FUSED;
+    else if (ladder != "slab")
+      throw std::runtime_error("unknown ladder '" + ladder + "' (slab | fused)");
     p.device = static_cast<int>(yaml::get_integer(args, "device", 0));
     p.method = method == "Drccd" ? CC4S_METHOD_DRCCD : CC4S_METHOD_CCSD;
     p.linearized = yaml::get_boolean(args, "linearized", false) ? 1 : 0;

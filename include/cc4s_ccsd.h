@@ -97,7 +97,14 @@ typedef struct {
                                      power-of-two scale per operand (1) */
   int64_t mixed_max_k;            /* MIXED: longest FP32 sum of a split contraction; longer
                                      K is cut into chunks added in FP64 (2048) */
+  int ladder;                     /* particle-particle ladder of FP64 complex passes
+                                     (Ccsd.cxx:579-600): CC4S_LADDER_SLAB (V slab through
+                                     HBM, 0) or CC4S_LADDER_FUSED (V formed in LDS inside the
+                                     contraction kernel); same results to rounding */
 } cc4s_ccsd_params;
+
+/* Values of cc4s_ccsd_params.ladder. */
+enum { CC4S_LADDER_SLAB = 0, CC4S_LADDER_FUSED = 1 };
 
 /* One pass of the iteration loop (CoupledCluster.cxx:129-175). */
 typedef struct {

@@ -7,7 +7,8 @@ small_complex.json  a small complex workload (No=6, Nv=24, NG=60) run the
               same way, plus the residuum at seeded amplitudes.
 complex_no32_nv256_ng1500.json  the complex No=32, Nv=256, NG=1500 workload
               with the vertex scaled by the "mp2_per_occupied" rule (the scale
-              at which the iteration converges), run the same way (about 1.5
+              at which the iteration converges), run the same way, plus the
+              converged T1 and T2 at seeded sample positions (about 1.5
               hours of host time on 8 cores; about 35 GB of memory).
 complex_no32_nv256_ng1500_residuum.json  one oracle residuum of that workload
               at seeded amplitudes: the scale, input SHA-256 values,
@@ -59,7 +60,21 @@ def _progress(i, e, dR):
           f"({time.strftime('%H:%M:%S')})", flush=True)
 
 
-def run_config(cfg, extra=None, scale_rule="max_v"):
+def amplitude_samples(T, seed):
+    """T1 and T2 of an amplitude set at seeded positions (cc4s-layout flat
+    indices), with the largest magnitude of each part."""
+    out = {}
+    for k, (name, t) in enumerate((("T1", T[0]), ("T2", T[1]))):
+        f = np.asfortranarray(t).ravel(order="F")
+        count = RESIDUUM_SAMPLES_R1 if name == "T1" else RESIDUUM_SAMPLES_R2
+        pos = sample_positions(f.size, count, seed + k)
+        out[name + "_samples"] = {"seed": seed + k, "positions": pos.tolist(),
+                                  "re": f[pos].real.tolist(), "im": f[pos].imag.tolist()}
+        out[name + "_max_abs"] = float(np.max(np.abs(f)))
+    return out
+
+
+def run_config(cfg, extra=None, scale_rule="max_v", samples=False):
     inp = syn.generate_config(cfg, scale_rule=scale_rule)
     P = O.CcsdProblem(inp.gamma, inp.eps, inp.No, cfg.field)
     t = time.time()
@@ -83,6 +98,8 @@ def run_config(cfg, extra=None, scale_rule="max_v"):
         "amplitudes": checksums(r["amplitudes"]),
         "oracle_seconds": dt,
     }
+    if samples:
+        out["amplitude_samples"] = amplitude_samples(r["amplitudes"], AMPLITUDE_SAMPLE_SEED)
     if extra:
         out.update(extra(P, inp))
     return out
@@ -102,6 +119,7 @@ def residuum_fixture(P, inp):
 RESIDUUM_AMPLITUDE_SEED = 77
 RESIDUUM_SAMPLE_SEED = 2026
 RESIDUUM_SAMPLES_R1, RESIDUUM_SAMPLES_R2 = 256, 2048
+AMPLITUDE_SAMPLE_SEED = 3030
 
 
 def sample_positions(n, count, seed):
@@ -162,7 +180,7 @@ def main(which):
               data["oracle_seconds"])
     if "complex_no32_nv256_ng1500" in which:
         data = run_config(syn.WORKLOADS["complex_no32_nv256_ng1500"],
-                          scale_rule="mp2_per_occupied")
+                          scale_rule="mp2_per_occupied", samples=True)
         (HERE / "complex_no32_nv256_ng1500.json").write_text(json.dumps(data, indent=1))
         print("complex_no32_nv256_ng1500", data["energy"], data["iterations"],
               data["oracle_seconds"])

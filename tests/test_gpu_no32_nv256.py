@@ -3,9 +3,10 @@ mp2_per_occupied rule) on the GPU against the oracle's committed fixtures
 (tests/golden/make_golden.py; the oracle needs about 1.5 hours of host time
 for the converged run, so the fixtures are made once, not in the test):
 
-* one residuum at seeded amplitudes (Ccsd.cxx:381-633): 256 sampled R1 and
-  2048 sampled R2 elements to 1e-12 of max |R|, checksums of R1 and R2, and
-  the energy of the amplitudes (CoupledCluster.cxx:194-253) to 1e-12;
+* one residuum at seeded amplitudes (Ccsd.cxx:381-633), with the slab and
+  with the fused ladder: 256 sampled R1 and 2048 sampled R2 elements to
+  1e-12 of max |R|, checksums of R1 and R2, and the energy of the amplitudes
+  (CoupledCluster.cxx:194-253) to 1e-12;
 * the converged FP64 run (CoupledCluster.cxx:129-175, DIIS 8): the energy of
   every iteration to 1e-9 Eh, the same iteration count and
   convergenceReached, and the checksums of the converged amplitudes to 1e-8.
@@ -64,14 +65,17 @@ def _compare_checksums(got, ref, n, tol):
         assert abs(got[key] - ref[key]) <= tol * scale, (key, got[key], ref[key])
 
 
-def test_residuum_at_seeded_amplitudes_against_the_oracle():
+@pytest.mark.parametrize("ladder", ["slab", "fused"])
+def test_residuum_at_seeded_amplitudes_against_the_oracle(ladder):
     fx = _fixture("complex_no32_nv256_ng1500_residuum.json")
-    print(f"No=32 Nv=256 NG=1500 residuum, scale {fx['scale']:.9e}", flush=True)
+    print(f"No=32 Nv=256 NG=1500 residuum ({ladder} ladder), scale {fx['scale']:.9e}",
+          flush=True)
     gamma, eps = _inputs(fx)
     a = fx["amplitudes"]
     T1, T2 = syn.random_amplitudes(CFG.No, CFG.Nv, CFG.field, seed=a["seed"],
                                    t1_scale=a["t1_scale"], t2_scale=a["t2_scale"])
-    with api.CoupledCluster(CFG.No, CFG.Nv, CFG.NG, maxResidua=CFG.max_residua) as cc:
+    with api.CoupledCluster(CFG.No, CFG.Nv, CFG.NG, maxResidua=CFG.max_residua,
+                            ladder=ladder) as cc:
         cc.set_inputs(gamma, eps)
         E, D, X = cc.energy((T1, T2))
         R1, R2 = cc.residuum((T1, T2))
