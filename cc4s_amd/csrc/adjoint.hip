@@ -1,9 +1,11 @@
-// Energy adjoint kernels: energy weights from the amplitudes, FP32 operand
-// copies, the FP32 complex product of the FP32 gradient, and the per-G block
-// dots of the structure factor.  The FP64 gradient products run on the FP64
-// MFMA contraction engine; everything here is bandwidth-bound except
-// gemm_fp32 (G*No^2*Nv^2 complex MACs, small next to one CCSD iteration).
+// Energy adjoint kernels: the gradient products with the energy weights
+// recomputed from the amplitudes inside the product (FP64 or FP32 MFMA,
+// G*No^2*Nv^2 complex MACs per block, small next to one CCSD iteration), the
+// energy weights as a tensor (for the structure factor of the finite-size
+// correction), and the per-G block dots of the structure factor.
 #include "adjoint.hpp"
+
+#include <type_traits>
 
 namespace cc4s_amd {
 
@@ -72,114 +74,271 @@ __global__ __launch_bounds__(kBlock) void weights_kernel(Amp<CPLX> t, void *Zout
   }
 }
 
-// M[k + K*n], K = No*Nv; complex k = i + No*a, real k = a + Nv*i; n = b + Nv*j
-template <bool CPLX>
-__global__ __launch_bounds__(kBlock) void weights_fp32_kernel(Amp<CPLX> t, float2 *M) {
-  const int64_t No = t.No, Nv = t.Nv, K = No * Nv;
-  const int64_t n = K * K;
-  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
-       e += int64_t(gridDim.x) * kBlock) {
-    const int64_t k = e % K, col = e / K;
-    const int64_t b = col % Nv, j = col / Nv;
-    if constexpr (CPLX) {
-      const int64_t i = k % No, a = k / No;
-      const double2 z = t.z(a, b, i, j);
-      M[e] = make_float2(static_cast<float>(z.x), static_cast<float>(z.y));
-    } else {
-      const int64_t a = k % Nv, i = k / Nv;
-      M[e] = make_float2(static_cast<float>(t.zs(a, b, i, j)), 0.0f);
-    }
+// ---------------------------------------------------------------------------
+// Fused gradient products: C[G, n] = sum_k Gamma_blk[G, k] W(k, n), with the
+// energy weights W recomputed from T1 and T2 for every tile as it is staged
+// into LDS (no Z tensor exists).  Tile 128 (G) x 64 (n), 16 k per stage,
+// double-buffered LDS, 8 waves of 32 x 32 on 16x16x4 MFMA; FP64 uses the
+// three-product complex multiply, FP32 the four-product one (FP32 sums
+// cancel too much in P3 - P1 - P2).  C is complex FP64, G fastest.
+//   kHP (complex): k = (b,j), n = (a,i), A = Gamma[G, No+b, j],
+//                  W = conj Z_abij, C at G + NG (i + No a)
+//   kPH (complex): k = (a,i), n = (b,j), A = Gamma[G, i, No+a],
+//                  W = Z_abij, C at G + NG (b + Nv j)
+//   kRE (real):    k = (b,j), n = (a,i), A = Gamma[G, No+b, j] (Re and Im are
+//                  the two real vertex components), W = Z_abij + Z_baji (real),
+//                  C at G + NG (a + Nv i)
+enum { kHP = 0, kPH = 1, kRE = 2 };
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc = dbl4;
+  using pair = double2;
+  __device__ static acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
-}
-
-__global__ __launch_bounds__(kBlock) void vertex_block_fp32_kernel(
-    const double2 *g, int64_t NG, int64_t Np, int64_t row0, int64_t rows,
-    int64_t col0, int64_t cols, float2 *out) {
-  const int64_t n = NG * rows * cols;
-  for (int64_t e = blockIdx.x * int64_t(kBlock) + threadIdx.x; e < n;
-       e += int64_t(gridDim.x) * kBlock) {
-    const int64_t G = e % NG, xy = e / NG;
-    const int64_t x = xy % rows, y = xy / rows;
-    const double2 v = g[G + NG * ((row0 + x) + Np * (col0 + y))];
-    out[e] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+  // row of register r of the 16x16 block held by `lane`
+  __device__ static int row(int r, int lane) { return 4 * r + (lane >> 4); }
+  __device__ static pair make(double x, double y) { return make_double2(x, y); }
+};
+template <>
+struct Mfma<float> {
+  using acc = flt4;
+  using pair = float2;
+  __device__ static acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
-}
+  __device__ static int row(int r, int lane) { return 4 * (lane >> 4) + r; }
+  __device__ static pair make(double x, double y) {
+    return make_float2(static_cast<float>(x), static_cast<float>(y));
+  }
+};
 
-// 64 (G) x 64 (n) tile, K steps of 16, 256 threads with 4 x 4 complex
-// accumulators each; FP32 FMAs on the vector ALUs.
-constexpr int kGM = 64, kGN = 64, kGK = 16;
+struct GradArgs {
+  const double2 *gamma;  // [NG, Np, Np]
+  const double2 *T1c, *T2c;  // complex amplitudes
+  const double *T1r, *T2r;   // real amplitudes
+  double2 *C;
+  int64_t NG, Np, No, Nv;
+};
 
-__global__ __launch_bounds__(256) void gemm_fp32_kernel(
-    const float2 *__restrict__ A, const float2 *__restrict__ B, int64_t sbk,
-    int64_t sbn, int conjB, double2 *__restrict__ C, int64_t NG, int64_t N,
-    int64_t K) {
-  __shared__ float2 As[kGK][kGM];
-  __shared__ float2 Bs[kGK][kGN];
-  const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
-  const int64_t g0 = int64_t(blockIdx.x) * kGM;
-  const int64_t n0 = int64_t(blockIdx.y) * kGN;
-  const float bsign = conjB ? -1.0f : 1.0f;
-  float2 acc[4][4];
+constexpr int kAT = 512;                  // 8 waves
+constexpr int kABM = 128, kABN = 64, kABK = 16;
+constexpr int kALA = kABM + 1, kALW = kABN + 1;
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kAT, 1) void grad_kernel(GradArgs p) {
+  using M = Mfma<T>;
+  using P = typename M::pair;
+  using Acc = typename M::acc;
+  constexpr bool M3 = std::is_same<T, double>::value && MODE != kRE;
+  __shared__ P As[2][kABK][kALA];
+  __shared__ P Ws[2][kABK][kALW];
+
+  const int64_t NG = p.NG, No = p.No, Nv = p.Nv, Np = p.Np;
+  const int64_t K = No * Nv, N = No * Nv;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t g0 = static_cast<int64_t>(blockIdx.x) * kABM;
+  const int64_t n0 = static_cast<int64_t>(blockIdx.y) * kABN;
+  const int nst = static_cast<int>((K + kABK - 1) / kABK);
+
+  // A elements of this thread: G = g0 + t % 128, k = 4e + t / 128 of a stage
+  const int ag = t % kABM, ak = t / kABM;
+  const bool gOk = g0 + ag < NG;
+  const int64_t gIdx = gOk ? g0 + ag : 0;
+  // W elements: n = n0 + t % 64, k = 8e + t / 64 of a stage
+  const int wn = t % kABN, wk = t / kABN;
+  const int64_t nn = n0 + wn;
+  const bool nOk = nn < N;
+  // the n index pair: (a, i) for kHP and kRE, (b, j) for kPH; fastest first
+  const int64_t nf = nOk ? nn % Nv : 0, ns = nOk ? nn / Nv : 0;
+
+  double2 ra[4];
+  bool kaOk[4];
+  // amplitude values of the two W elements, combined at the LDS store
+  double2 t2a[2], t2b[2], t1a[2], t1b[2], t1c[2], t1d[2];
+  double r2a[2], r2b[2], r2c[2], r2d[2], r1a[2], r1b[2], r1c[2], r1d[2];
+  bool kwOk[2];
+
+  auto t2c = [&](int64_t a, int64_t b, int64_t i, int64_t j) {
+    return p.T2c[a + Nv * (b + Nv * (i + No * j))];
+  };
+  auto t2r = [&](int64_t a, int64_t b, int64_t i, int64_t j) {
+    return p.T2r[a + Nv * (b + Nv * (i + No * j))];
+  };
+  auto load = [&](int s) {
+    const int64_t kb = static_cast<int64_t>(s) * kABK;
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[r][c] = make_float2(0.0f, 0.0f);
-  for (int64_t k0 = 0; k0 < K; k0 += kGK) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int q = tid + 256 * r;
-      const int m = q % kGM, k = q / kGM;
-      const int64_t G = g0 + m, kk = k0 + k;
-      As[k][m] = (G < NG && kk < K) ? A[G + NG * kk] : make_float2(0.0f, 0.0f);
+    for (int e = 0; e < 4; ++e) {
+      const int64_t k = kb + 4 * e + ak;
+      kaOk[e] = k < K;
+      const int64_t kk = kaOk[e] ? k : 0;
+      const int64_t kf = kk % Nv, ks = kk / Nv;  // (b, j) or (a, i)
+      // Gamma[G, P, Q] at G + NG (P + Np Q)
+      const int64_t P = MODE == kPH ? ks : No + kf, Q = MODE == kPH ? No + kf : ks;
+      ra[e] = p.gamma[gIdx + NG * (P + Np * Q)];
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int q = tid + 256 * r;
-      int k, nn;
-      if (sbk == 1) {  // read along k
-        k = q % kGK;
-        nn = q / kGK;
-      } else {         // read along n
-        nn = q % kGN;
-        k = q / kGN;
+    for (int e = 0; e < 2; ++e) {
+      const int64_t k = kb + 8 * e + wk;
+      kwOk[e] = k < K && nOk;
+      const int64_t kk = k < K ? k : 0;
+      const int64_t kf = kk % Nv, ks = kk / Nv;
+      int64_t a, b, i, j;
+      if (MODE == kPH) {
+        a = kf; i = ks; b = nf; j = ns;
+      } else {
+        b = kf; j = ks; a = nf; i = ns;
       }
-      const int64_t kk = k0 + k, col = n0 + nn;
-      float2 b = make_float2(0.0f, 0.0f);
-      if (kk < K && col < N) {
-        b = B[kk * sbk + col * sbn];
-        b.y *= bsign;
+      if constexpr (MODE != kRE) {
+        t2a[e] = t2c(a, b, i, j);
+        t2b[e] = t2c(b, a, i, j);
+        t1a[e] = p.T1c[a + Nv * i];
+        t1b[e] = p.T1c[b + Nv * j];
+        t1c[e] = p.T1c[b + Nv * i];
+        t1d[e] = p.T1c[a + Nv * j];
+      } else {
+        r2a[e] = t2r(a, b, i, j);
+        r2b[e] = t2r(b, a, j, i);
+        r2c[e] = t2r(b, a, i, j);
+        r2d[e] = t2r(a, b, j, i);
+        r1a[e] = p.T1r[a + Nv * i];
+        r1b[e] = p.T1r[b + Nv * j];
+        r1c[e] = p.T1r[b + Nv * i];
+        r1d[e] = p.T1r[a + Nv * j];
       }
-      Bs[k][nn] = b;
     }
+  };
+  auto cmul = [](double2 x, double2 y) {
+    return make_double2(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x);
+  };
+  // out-of-range elements are zeroed here, component by component (a select
+  // of whole pairs would go through scratch memory)
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const bool ok = gOk && kaOk[e];
+      As[buf][4 * e + ak][ag] = M::make(ok ? ra[e].x : 0.0, ok ? ra[e].y : 0.0);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      double zr, zi;
+      if constexpr (MODE != kRE) {
+        // Z_abij = 2 tau_abij - tau_baij, tau = T2 + T1 T1 (CoupledCluster.cxx:223-231)
+        const double2 p1 = cmul(t1a[e], t1b[e]), p2 = cmul(t1c[e], t1d[e]);
+        zr = 2.0 * (t2a[e].x + p1.x) - (t2b[e].x + p2.x);
+        zi = 2.0 * (t2a[e].y + p1.y) - (t2b[e].y + p2.y);
+        if (MODE == kHP) zi = -zi;
+      } else {
+        // Z_abij + Z_baji
+        const double p1 = r1a[e] * r1b[e], p2 = r1c[e] * r1d[e];
+        zr = 2.0 * (r2a[e] + p1) - (r2c[e] + p2) + 2.0 * (r2b[e] + p1) - (r2d[e] + p2);
+        zi = 0.0;
+      }
+      Ws[buf][8 * e + wk][wn] = M::make(kwOk[e] ? zr : 0.0, kwOk[e] ? zi : 0.0);
+    }
+  };
+
+  const int wr = w / 2, wc = w % 2;  // the wave's 32 x 32 part of the tile
+  Acc a1[2][2], a2[2][2], a3[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      a1[i][j] = Acc{0, 0, 0, 0};
+      a2[i][j] = Acc{0, 0, 0, 0};
+      a3[i][j] = Acc{0, 0, 0, 0};
+    }
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int s = 0; s < nst; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nst) load(s + 1);
+#pragma unroll
+    for (int kq = 0; kq < kABK / 4; ++kq) {
+      const int kr = kq * 4 + (lane >> 4);
+      P x[2], y[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) x[i] = As[buf][kr][wr * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) y[j] = Ws[buf][kr][wc * 32 + j * 16 + (lane & 15)];
+      if constexpr (MODE == kRE) {
+        // real W: re and im of Gamma times W
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            a1[i][j] = M::mma(x[i].x, y[j].x, a1[i][j]);
+            a2[i][j] = M::mma(x[i].y, y[j].x, a2[i][j]);
+          }
+      } else if constexpr (M3) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) a1[i][j] = M::mma(x[i].x, y[j].x, a1[i][j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) a2[i][j] = M::mma(x[i].y, y[j].y, a2[i][j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            a3[i][j] = M::mma(x[i].x + x[i].y, y[j].x + y[j].y, a3[i][j]);
+      } else {
+        // four products: a1 = re, a2 = im
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            a1[i][j] = M::mma(x[i].x, y[j].x, a1[i][j]);
+            a2[i][j] = M::mma(x[i].x, y[j].y, a2[i][j]);
+          }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            a1[i][j] = M::mma(-x[i].y, y[j].y, a1[i][j]);
+            a2[i][j] = M::mma(x[i].y, y[j].x, a2[i][j]);
+          }
+      }
+    }
+    // the other buffer was read in stage s-1, released by its barrier
+    if (s + 1 < nst) store(buf ^ 1);
     __syncthreads();
+  }
+
 #pragma unroll
-    for (int kk = 0; kk < kGK; ++kk) {
-      float2 a[4], b[4];
+  for (int j = 0; j < 2; ++j) {
+    const int64_t n = n0 + wc * 32 + j * 16 + (lane & 15);
+    if (n >= N) continue;
+    const int64_t f = n % Nv, sl = n / Nv;
+    // output column of n: (i + No a) for kHP, (b + Nv j) for kPH, (a + Nv i) for kRE
+    const int64_t col = MODE == kHP ? sl + No * f : n;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = As[kk][tx + 16 * r];
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) b[c] = Bs[kk][ty + 16 * c];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          acc[r][c].x = fmaf(a[r].x, b[c].x, acc[r][c].x);
-          acc[r][c].x = fmaf(-a[r].y, b[c].y, acc[r][c].x);
-          acc[r][c].y = fmaf(a[r].x, b[c].y, acc[r][c].y);
-          acc[r][c].y = fmaf(a[r].y, b[c].x, acc[r][c].y);
+      for (int r = 0; r < 4; ++r) {
+        const int64_t G = g0 + wr * 32 + i * 16 + M::row(r, lane);
+        if (G >= NG) continue;
+        double re, im;
+        if constexpr (M3) {
+          const double p1 = a1[i][j][r], p2 = a2[i][j][r];
+          re = p1 - p2;
+          im = a3[i][j][r] - (p1 + p2);
+        } else {
+          re = a1[i][j][r];
+          im = a2[i][j][r];
         }
-    }
-    __syncthreads();
+        p.C[G + NG * col] = make_double2(re, im);
+      }
   }
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int64_t G = g0 + tx + 16 * r, col = n0 + ty + 16 * c;
-      if (G < NG && col < N)
-        C[G + NG * col] = make_double2(acc[r][c].x, acc[r][c].y);
-    }
 }
 
 // partials[c * NG + G]: chunk c of the flattened (x, y) range of one block
@@ -227,43 +386,41 @@ void energy_weights(hipStream_t s, Field f, const void *T, int64_t No,
   CC4S_HIP(hipGetLastError());
 }
 
-void energy_weights_fp32(hipStream_t s, Field f, const void *T, int64_t No,
-                         int64_t Nv, void *M) {
-  const int64_t n = Nv * Nv * No * No;
+void energy_gradient_product(hipStream_t s, Field f, bool fp32, GradientBlock blk,
+                             const void *gamma, const void *T, int64_t NG, int64_t No,
+                             int64_t Nv, void *C) {
+  CC4S_ASSERT(NG > 0 && No > 0 && Nv > 0, "empty gradient product");
+  CC4S_ASSERT(f == Field::Complex || blk == GradientBlock::PH,
+              "the real field has the ph gradient block only");
+  GradArgs a{};
+  a.gamma = static_cast<const double2 *>(gamma);
+  a.C = static_cast<double2 *>(C);
+  a.NG = NG;
+  a.No = No;
+  a.Nv = Nv;
+  a.Np = No + Nv;
   if (f == Field::Complex) {
-    Amp<true> t{static_cast<const double2 *>(T), static_cast<const double2 *>(T) + Nv * No,
-                No, Nv};
-    weights_fp32_kernel<true><<<grid_for(n), kBlock, 0, s>>>(t, static_cast<float2 *>(M));
+    a.T1c = static_cast<const double2 *>(T);
+    a.T2c = a.T1c + Nv * No;
   } else {
-    Amp<false> t{static_cast<const double *>(T), static_cast<const double *>(T) + Nv * No,
-                 No, Nv};
-    weights_fp32_kernel<false><<<grid_for(n), kBlock, 0, s>>>(t, static_cast<float2 *>(M));
+    a.T1r = static_cast<const double *>(T);
+    a.T2r = a.T1r + Nv * No;
   }
-  CC4S_HIP(hipGetLastError());
-}
-
-void vertex_block_fp32(hipStream_t s, const void *gamma, int64_t NG, int64_t Np,
-                       int64_t row0, int64_t rows, int64_t col0, int64_t cols,
-                       void *out) {
-  CC4S_ASSERT(row0 >= 0 && col0 >= 0 && row0 + rows <= Np && col0 + cols <= Np,
-              "vertex block out of range");
-  vertex_block_fp32_kernel<<<grid_for(NG * rows * cols), kBlock, 0, s>>>(
-      static_cast<const double2 *>(gamma), NG, Np, row0, rows, col0, cols,
-      static_cast<float2 *>(out));
-  CC4S_HIP(hipGetLastError());
-}
-
-void gemm_fp32(hipStream_t s, const void *A, const void *B, int64_t sbk,
-               int64_t sbn, bool conjB, void *C, int64_t NG, int64_t N,
-               int64_t K) {
-  CC4S_ASSERT(NG > 0 && N > 0 && K > 0, "empty FP32 product");
-  const int64_t gy = (N + kGN - 1) / kGN;
-  CC4S_ASSERT(gy <= 65535, "FP32 product too wide");
-  dim3 grid(static_cast<unsigned>((NG + kGM - 1) / kGM), static_cast<unsigned>(gy));
-  gemm_fp32_kernel<<<grid, 256, 0, s>>>(static_cast<const float2 *>(A),
-                                         static_cast<const float2 *>(B), sbk, sbn,
-                                         conjB ? 1 : 0, static_cast<double2 *>(C), NG,
-                                         N, K);
+  const int64_t gy = (No * Nv + kABN - 1) / kABN;
+  CC4S_ASSERT(gy <= 65535, "gradient product too wide");
+  const dim3 grid(static_cast<unsigned>((NG + kABM - 1) / kABM), static_cast<unsigned>(gy));
+  const int mode = f == Field::Real ? kRE : (blk == GradientBlock::HP ? kHP : kPH);
+#define CC4S_GRAD(T_, M_) grad_kernel<T_, M_><<<grid, dim3(kAT), 0, s>>>(a)
+  if (fp32) {
+    if (mode == kHP) CC4S_GRAD(float, kHP);
+    else if (mode == kPH) CC4S_GRAD(float, kPH);
+    else CC4S_GRAD(float, kRE);
+  } else {
+    if (mode == kHP) CC4S_GRAD(double, kHP);
+    else if (mode == kPH) CC4S_GRAD(double, kPH);
+    else CC4S_GRAD(double, kRE);
+  }
+#undef CC4S_GRAD
   CC4S_HIP(hipGetLastError());
 }
 

@@ -23,24 +23,18 @@ namespace cc4s_amd {
 void energy_weights(hipStream_t s, Field f, const void *T, int64_t No,
                     int64_t Nv, void *Z);
 
-// Z in FP32 as the matrix M[k][n] = Z of the FP32 gradient products:
-// complex field k = i + No*a, n = b + Nv*j, M = Z_abij;
-// real field    k = a + Nv*i, n = b + Nv*j, M = Z_abij + Z_baji.
-void energy_weights_fp32(hipStream_t s, Field f, const void *T, int64_t No,
-                         int64_t Nv, void *M);
-
-// FP32 copy of one block of the vertex: out[G + NG*(x + rows*y)] =
-// Gamma[G, row0 + x, col0 + y] (complex, Gamma is [NG, Np, Np])
-void vertex_block_fp32(hipStream_t s, const void *gamma, int64_t NG, int64_t Np,
-                       int64_t row0, int64_t rows, int64_t col0, int64_t cols,
-                       void *out);
-
-// FP32 complex product C[G + NG*n] = sum_k A[G + NG*k] op(B)[k, n] with
-// op(B)[k, n] = B[k*sbk + n*sbn] (conjugated if conjB); A, B complex FP32,
-// C complex FP64; FP32 accumulation.
-void gemm_fp32(hipStream_t s, const void *A, const void *B, int64_t sbk,
-               int64_t sbn, bool conjB, void *C, int64_t NG, int64_t N,
-               int64_t K);
+// One block of the energy gradient, with the weights recomputed from the
+// amplitude set T = [T1; T2] tile by tile inside the product (no Z tensor):
+//   complex HP: C[G, i, a] = sum_bj conj(Z_abij) Gamma^G_bj
+//   complex PH: C[G, b, j] = sum_ai Z_abij Gamma^G_ia
+//   real    PH: C[G, a, i] = sum_bj (Z_abij + Z_baji) Gamma^G_bj
+// (complex FP64 output, G fastest; for the real field Re and Im of C are the
+// gradient of the two real vertex components).  FP64 (fp32 false) or FP32
+// MFMA arithmetic with FP32 accumulation.
+enum class GradientBlock { HP, PH };
+void energy_gradient_product(hipStream_t s, Field f, bool fp32, GradientBlock blk,
+                             const void *gamma, const void *T, int64_t NG, int64_t No,
+                             int64_t Nv, void *C);
 
 // Per-G dots over one block: out[G] (=, or += when accumulate) factor *
 // Re sum_{x<rows, y<cols} conj(Gamma[G, row0+x, col0+y]) g[G + NG*(x + rows*y)]

@@ -677,7 +677,8 @@ void CoupledCluster::transition_structure_factor(const double *U, int64_t NGg,
 // E = sum_G Re sum_ia conj(Gamma^G_ia) g^G_ia with g^G_ia = sum_bj conj(Z_abij)
 // Gamma^G_bj (complex), or 1/2 sum_G sum_ai Re(conj Gamma^G_ai g^G_ai) with
 // g^G_ai = sum_bj Zs_abij Gamma^G_bj (real): the pphh integrals are never
-// formed, Z is rebuilt from the amplitudes in the residuum buffer.
+// formed, and Z exists only tile by tile inside the product kernel, rebuilt
+// from the amplitudes.
 double CoupledCluster::energy_forward(const double *gamma, bool gamma_on_device) {
   CC4S_ASSERT(inputs_set_, "inputs not set");
   CC4S_ASSERT(have_amplitudes_, "no amplitudes yet");
@@ -685,21 +686,19 @@ double CoupledCluster::energy_forward(const double *gamma, bool gamma_on_device)
   const auto &s = p_.shape;
   const int64_t No = s.No, Nv = s.Nv, NG = s.NG;
   const void *g = adjoint_vertex(gamma, gamma_on_device);
-  energy_weights(stream_, s.field, T_.get(), No, Nv, R_.get());
-  const View Z = dense_view(R_.get(), s.field, "abij", {Nv, Nv, No, No});
   const size_t block_bytes = static_cast<size_t>(NG) * No * Nv * 16;
   adj_partials_.reserve(block_dots_partials_size(NG));
   adj_S_.reserve(NG * sizeof(double));
   if (s.field == Field::Complex) {
     adj_hp_.reserve(block_bytes);
-    engine_->contract(1.0, Z.conjugated(), vertex_block(g, false, "Gbj"), 0.0,
-                      dense_view(adj_hp_.get(), Field::Complex, "Gia", {NG, No, Nv}));
+    energy_gradient_product(stream_, s.field, false, GradientBlock::HP, g, T_.get(), NG,
+                            No, Nv, adj_hp_.get());
     block_dots(stream_, g, NG, s.Np(), 0, No, No, Nv, adj_hp_.get(), 1.0, false,
                adj_partials_.get(), adj_S_.as<double>());
   } else {
     adj_ph_.reserve(block_bytes);
-    engine_->contract(1.0, Z, vertex_block(g, false, "Gbj"), 0.0,
-                      dense_view(adj_ph_.get(), Field::Real, "Gai", {2 * NG, Nv, No}));
+    energy_gradient_product(stream_, s.field, false, GradientBlock::PH, g, T_.get(), NG,
+                            No, Nv, adj_ph_.get());
     block_dots(stream_, g, NG, s.Np(), No, Nv, 0, No, adj_ph_.get(), 0.5, false,
                adj_partials_.get(), adj_S_.as<double>());
   }
@@ -722,42 +721,19 @@ void CoupledCluster::energy_backward(const double *gamma, bool gamma_on_device,
   CC4S_ASSERT(have_amplitudes_, "no amplitudes yet");
   CC4S_HIP(hipSetDevice(device_));
   const auto &s = p_.shape;
-  const int64_t No = s.No, Nv = s.Nv, NG = s.NG, Np = s.Np(), K = No * Nv;
+  const int64_t No = s.No, Nv = s.Nv, NG = s.NG, Np = s.Np();
   const bool cplx_field = s.field == Field::Complex;
   const void *g = adjoint_vertex(gamma, gamma_on_device);
   const size_t block_bytes = static_cast<size_t>(NG) * No * Nv * 16;
   if (cplx_field) adj_hp_.reserve(block_bytes);
   adj_ph_.reserve(block_bytes);
-  if (!fp32) {
-    energy_weights(stream_, s.field, T_.get(), No, Nv, R_.get());
-    const View Z = dense_view(R_.get(), s.field, "abij", {Nv, Nv, No, No});
-    if (cplx_field) {
-      engine_->contract(1.0, Z.conjugated(), vertex_block(g, false, "Gbj"), 0.0,
-                        dense_view(adj_hp_.get(), Field::Complex, "Gia", {NG, No, Nv}));
-      engine_->contract(1.0, Z, vertex_block(g, true, "Gia"), 0.0,
-                        dense_view(adj_ph_.get(), Field::Complex, "Gbj", {NG, Nv, No}));
-    } else {
-      engine_->contract(1.0, Z, vertex_block(g, false, "Gbj"), 0.0,
-                        dense_view(adj_ph_.get(), Field::Real, "Gai", {2 * NG, Nv, No}));
-    }
-  } else {
-    // M (K x K complex FP32) fits in the residuum buffer (nset x 8+ bytes)
-    energy_weights_fp32(stream_, s.field, T_.get(), No, Nv, R_.get());
-    const size_t half = static_cast<size_t>(NG) * K * 8;
-    adj_b_.reserve(half);
-    vertex_block_fp32(stream_, g, NG, Np, No, Nv, 0, No, adj_b_.get());  // [G,(b,j)]
-    if (cplx_field) {
-      adj_a_.reserve(half);
-      vertex_block_fp32(stream_, g, NG, Np, 0, No, No, Nv, adj_a_.get());  // [G,(i,a)]
-      // g_hp[G,(i,a)] = sum_(b,j) Gamma_ph[G,(b,j)] conj M[(i,a) + K (b,j)]
-      gemm_fp32(stream_, adj_b_.get(), R_.get(), K, 1, true, adj_hp_.get(), NG, K, K);
-      // g_ph[G,(b,j)] = sum_(i,a) Gamma_hp[G,(i,a)] M[(i,a) + K (b,j)]
-      gemm_fp32(stream_, adj_a_.get(), R_.get(), 1, K, false, adj_ph_.get(), NG, K, K);
-    } else {
-      // g_ph[G,(a,i)] = sum_(b,j) Gamma_ph[G,(b,j)] M[(a,i) + K (b,j)]
-      gemm_fp32(stream_, adj_b_.get(), R_.get(), K, 1, false, adj_ph_.get(), NG, K, K);
-    }
-  }
+  // the energy weights are rebuilt from the amplitudes tile by tile inside
+  // the products (FP64 or FP32 MFMA)
+  if (cplx_field)
+    energy_gradient_product(stream_, s.field, fp32, GradientBlock::HP, g, T_.get(), NG, No,
+                            Nv, adj_hp_.get());
+  energy_gradient_product(stream_, s.field, fp32, GradientBlock::PH, g, T_.get(), NG, No, Nv,
+                          adj_ph_.get());
   if (S) {
     adj_partials_.reserve(block_dots_partials_size(NG));
     adj_S_.reserve(NG * sizeof(double));

@@ -171,7 +171,7 @@ class CoupledCluster {
   std::unique_ptr<DiisMixer> mixer_;
   DeviceBuffer T_, R_, scratch_, energy_partials_;
   // energy adjoint: uploaded vertex, gradient blocks, FP32 operands, dots
-  DeviceBuffer adj_gamma_, adj_hp_, adj_ph_, adj_a_, adj_b_, adj_partials_, adj_S_;
+  DeviceBuffer adj_gamma_, adj_hp_, adj_ph_, adj_partials_, adj_S_;
   // transition structure factor: singular vectors, grid vertex blocks, Y
   DeviceBuffer fsc_u_, fsc_ph_, fsc_hp_, fsc_y_;
   bool have_amplitudes_ = false;

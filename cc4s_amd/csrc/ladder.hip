@@ -113,12 +113,14 @@ __global__ __launch_bounds__(kNT, 1) void ladder_fused_kernel(LadderArgs p) {
       rb[e] = p.R[static_cast<int64_t>(boff[e]) + go * p.sRG];
     }
   };
+  // zeroed component by component (a select of whole pairs would go through
+  // scratch memory)
   auto store_stage = [&](int buf) {
-    const double2 zero = make_double2(0.0, 0.0);
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-      Fa[buf][fk][fr[e]] = (oka[e] && gok) ? make_double2(ra[e].x, ra[e].y * p.signL) : zero;
-      Fb[buf][fk][fr[e]] = (okb[e] && gok) ? make_double2(rb[e].x, rb[e].y * p.signR) : zero;
+      const bool ka = oka[e] && gok, kb = okb[e] && gok;
+      Fa[buf][fk][fr[e]] = make_double2(ka ? ra[e].x : 0.0, ka ? ra[e].y * p.signL : 0.0);
+      Fb[buf][fk][fr[e]] = make_double2(kb ? rb[e].x : 0.0, kb ? rb[e].y * p.signR : 0.0);
     }
   };
 
