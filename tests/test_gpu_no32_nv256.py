@@ -9,7 +9,8 @@ for the converged run, so the fixtures are made once, not in the test):
   (CoupledCluster.cxx:194-253) to 1e-12;
 * the converged FP64 run (CoupledCluster.cxx:129-175, DIIS 8): the energy of
   every iteration to 1e-9 Eh, the same iteration count and
-  convergenceReached, and the checksums of the converged amplitudes to 1e-8.
+  convergenceReached, the checksums of the converged amplitudes to 1e-8, and
+  256 sampled T1 and 2048 sampled T2 elements to 1e-8 of max |T|.
 
 The inputs are regenerated from the seeds with the fixture's recorded scale
 and checked against its SHA-256 values before anything runs.
@@ -118,3 +119,12 @@ def test_converged_run_against_the_oracle_trace():
     for name, t in (("T1", T1), ("T2", T2)):
         sums, f = _checksums(t)
         _compare_checksums(sums, fx["amplitudes"][name], f.size, T_TOL)
+        # converged amplitudes element by element at seeded positions
+        if "amplitude_samples" not in fx:
+            continue
+        s = fx["amplitude_samples"][name + "_samples"]
+        pos = np.asarray(s["positions"], dtype=np.int64)
+        ref = np.asarray(s["re"]) + 1j * np.asarray(s["im"])
+        err = float(np.max(np.abs(f[pos] - ref))) / fx["amplitude_samples"][name + "_max_abs"]
+        print(f"  {name}: sampled max error {err:.2e} of max |{name}|", flush=True)
+        assert err < T_TOL
