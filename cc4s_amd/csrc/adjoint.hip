@@ -142,6 +142,7 @@ __global__ __launch_bounds__(kAT, 1) void grad_kernel(GradArgs p) {
 
   const int64_t NG = p.NG, No = p.No, Nv = p.Nv, Np = p.Np;
   const int64_t K = No * Nv, N = No * Nv;
+  const uint32_t nv32 = static_cast<uint32_t>(Nv);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t g0 = static_cast<int64_t>(blockIdx.x) * kABM;
   const int64_t n0 = static_cast<int64_t>(blockIdx.y) * kABN;
@@ -178,7 +179,8 @@ __global__ __launch_bounds__(kAT, 1) void grad_kernel(GradArgs p) {
       const int64_t k = kb + 4 * e + ak;
       kaOk[e] = k < K;
       const int64_t kk = kaOk[e] ? k : 0;
-      const int64_t kf = kk % Nv, ks = kk / Nv;  // (b, j) or (a, i)
+      // (b, j) or (a, i); 32-bit division (No Nv < 2^31)
+      const int64_t kf = static_cast<uint32_t>(kk) % nv32, ks = static_cast<uint32_t>(kk) / nv32;
       // Gamma[G, P, Q] at G + NG (P + Np Q)
       const int64_t P = MODE == kPH ? ks : No + kf, Q = MODE == kPH ? No + kf : ks;
       ra[e] = p.gamma[gIdx + NG * (P + Np * Q)];
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(kAT, 1) void grad_kernel(GradArgs p) {
       const int64_t k = kb + 8 * e + wk;
       kwOk[e] = k < K && nOk;
       const int64_t kk = k < K ? k : 0;
-      const int64_t kf = kk % Nv, ks = kk / Nv;
+      const int64_t kf = static_cast<uint32_t>(kk) % nv32, ks = static_cast<uint32_t>(kk) / nv32;
       int64_t a, b, i, j;
       if (MODE == kPH) {
         a = kf; i = ks; b = nf; j = ns;
@@ -407,7 +409,7 @@ void energy_gradient_product(hipStream_t s, Field f, bool fp32, GradientBlock bl
     a.T2r = a.T1r + Nv * No;
   }
   const int64_t gy = (No * Nv + kABN - 1) / kABN;
-  CC4S_ASSERT(gy <= 65535, "gradient product too wide");
+  CC4S_ASSERT(gy <= 65535 && No * Nv < (int64_t(1) << 31), "gradient product too wide");
   const dim3 grid(static_cast<unsigned>((NG + kABM - 1) / kABM), static_cast<unsigned>(gy));
   const int mode = f == Field::Real ? kRE : (blk == GradientBlock::HP ? kHP : kPH);
 #define CC4S_GRAD(T_, M_) grad_kernel<T_, M_><<<grid, dim3(kAT), 0, s>>>(a)

@@ -212,10 +212,16 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
                                    R2.as("xyij"), s, w.fused, E_.stream());
     E_.stats.launches += 1;
   }
-  // tau is read unchanged by every slab contraction: in split-bf16 mode it
-  // is packed once for all of them
+  // tau is read unchanged by every slab contraction: in split mode it is
+  // packed once for all of them
   if (E_.mixed()) E_.set_operand_cache(tauv.ptr);
   for (int64_t m = 0; m < ns && !fused; ++m) {
+    // the right-hand slice is the same for the whole row m: packed once (its
+    // index names tell it from the left-hand slice of a diagonal pair, which
+    // can start at the same address)
+    if (E_.mixed())
+      E_.set_operand_cache(right.sliced(1, m * s, m * s + std::min(s, v - m * s)).ptr, 1,
+                           "Gyd");
     for (int64_t n = m; n < ns; ++n) {
       const int64_t a = n * s, b = m * s;
       const int64_t Nx = std::min(s, v - a), Ny = std::min(s, v - b);

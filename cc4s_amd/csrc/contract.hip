@@ -1068,37 +1068,47 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
   const int64_t Kfull = (P.K + T.BK - 1) / T.BK * T.BK;
   const View &Aop = P.swap ? B : A;
   const View &Bop = P.swap ? A : B;
-  // an operand marked constant is packed once over the full K range
-  const bool cacheA = cache_ptr_ && Aop.ptr == cache_ptr_;
-  const bool cacheB = !cacheA && cache_ptr_ && Bop.ptr == cache_ptr_;
-  // half split: per-operand scales (A, B, cached operand) and reduction room
+  // an operand marked constant (a cache slot holds its pointer) is packed
+  // once over the full K range
+  auto slot_of = [&](const View &v) {
+    for (int s = 0; s < kCacheSlots; ++s)
+      if (cache_ptr_[s] && v.ptr == cache_ptr_[s] &&
+          (cache_idx_[s].empty() || v.idx == cache_idx_[s]))
+        return s;
+    return -1;
+  };
+  const int slotA = slot_of(Aop);
+  const int slotB = slotA < 0 ? slot_of(Bop) : -1;
+  const bool cacheA = slotA >= 0, cacheB = slotB >= 0;
+  const int slot = cacheA ? slotA : slotB;
+  // half split: per-operand scales (A, B, cache slots 0 and 1) and reduction room
   double *scA = nullptr, *scB = nullptr, *scC = nullptr, *scPart = nullptr;
   if (mixed_half) {
     scales_.reserve(4 * sizeof(double) + split_scale_partials_bytes());
     scA = scales_.as<double>();
     scB = scA + 1;
-    scC = scA + 2;
+    scC = scA + 2 + (slot > 0 ? slot : 0);
     scPart = scA + 4;
   }
   if (cacheA || cacheB) {
     const int64_t rows = cacheA ? Mp : Np;
-    if (cache_plan_ != &P || cache_is_a_ != cacheA) {
-      cache_.reserve(static_cast<size_t>(rows) * Kfull * 8);
+    if (cache_plan_[slot] != &P || cache_is_a_[slot] != cacheA) {
+      cache_[slot].reserve(static_cast<size_t>(rows) * Kfull * 8);
       if (cacheA) {
         if (mixed_half)
           launch_split_scale(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M,
                              P.K, P.a_kc, scPart, scC, stream_);
         launch_pack(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K, 0,
-                    Kfull, Mp, Aop.conj, P.a_kc, cache_.get(), scC, stream_);
+                    Kfull, Mp, Aop.conj, P.a_kc, cache_[slot].get(), scC, stream_);
       } else {
         if (mixed_half)
           launch_split_scale(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N,
                              P.K, P.b_kc, scPart, scC, stream_);
         launch_pack(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K, 0,
-                    Kfull, Np, Bop.conj, P.b_kc, cache_.get(), scC, stream_);
+                    Kfull, Np, Bop.conj, P.b_kc, cache_[slot].get(), scC, stream_);
       }
-      cache_plan_ = &P;
-      cache_is_a_ = cacheA;
+      cache_plan_[slot] = &P;
+      cache_is_a_[slot] = cacheA;
     }
   }
   if (mixed_half && !cacheA)
@@ -1143,10 +1153,10 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
   for (int64_t c = 0; c < chunks; ++c) {
     const int64_t k0 = c * Kc;
     const int64_t kc = std::min(Kc, (P.K - k0 + T.BK - 1) / T.BK * T.BK);
-    const void *Ause = cacheA ? static_cast<const void *>(cache_.as<char>() +
+    const void *Ause = cacheA ? static_cast<const void *>(cache_[slot].as<char>() +
                                                           static_cast<size_t>(k0) * Mp * 8)
                               : Apk;
-    const void *Buse = cacheB ? static_cast<const void *>(cache_.as<char>() +
+    const void *Buse = cacheB ? static_cast<const void *>(cache_[slot].as<char>() +
                                                           static_cast<size_t>(k0) * Np * 8)
                               : Bpk;
     if (!cacheA)

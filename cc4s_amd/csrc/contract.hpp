@@ -83,14 +83,23 @@ class ContractionEngine {
   // While set, the split-bf16 packing of an operand whose memory starts at
   // `ptr` is kept for the whole K range and reused by later contractions of
   // the same plan; the caller guarantees that the memory does not change
-  // until clear_operand_cache() (the ladder's tau across its slab pairs).
-  void set_operand_cache(const void *ptr) {
-    cache_ptr_ = ptr;
-    cache_plan_ = nullptr;
+  // until the slot is set again or cleared.  A non-empty `idx` also requires
+  // the operand view's index names (two views of the same memory in one
+  // contraction).  Two slots: the ladder keeps tau (its contractions'
+  // constant operand) in slot 0 and the right-hand dressed-vertex slice of a
+  // slice pair row (its formations' constant operand) in slot 1.
+  static constexpr int kCacheSlots = 2;
+  void set_operand_cache(const void *ptr, int slot = 0, const std::string &idx = "") {
+    cache_ptr_[slot] = ptr;
+    cache_idx_[slot] = idx;
+    cache_plan_[slot] = nullptr;
   }
   void clear_operand_cache() {
-    cache_ptr_ = nullptr;
-    cache_plan_ = nullptr;
+    for (int s = 0; s < kCacheSlots; ++s) {
+      cache_ptr_[s] = nullptr;
+      cache_idx_[s].clear();
+      cache_plan_[s] = nullptr;
+    }
   }
 
  private:
@@ -99,11 +108,12 @@ class ContractionEngine {
                       double beta, const View &C, double flops);
   bool mixed_ = false;
   DeviceBuffer pack_;
-  DeviceBuffer cache_;
-  DeviceBuffer scales_;  // half split: [A, B, cached operand] scales, partials
-  const void *cache_ptr_ = nullptr;
-  const GettPlan *cache_plan_ = nullptr;
-  bool cache_is_a_ = true;
+  DeviceBuffer cache_[kCacheSlots];
+  DeviceBuffer scales_;  // half split: [A, B, cache slot 0, cache slot 1] scales, partials
+  const void *cache_ptr_[kCacheSlots] = {nullptr, nullptr};
+  std::string cache_idx_[kCacheSlots];
+  const GettPlan *cache_plan_[kCacheSlots] = {nullptr, nullptr};
+  bool cache_is_a_[kCacheSlots] = {true, true};
   struct Pending {
     hipEvent_t start, stop;
     std::string name;
