@@ -53,8 +53,8 @@ from cc4s_amd import synthetic as syn  # noqa: E402
 FP64_MFMA_PEAK_TFLOPS = 78.6      # MI355X dense FP64 matrix
 METRIC = ("s per CCSD iteration + % MFMA peak, mixed vs FP64 GPU at "
           "|ΔE|≤1 µEh")
-VARIANTS = [f"gett_{t}_{a}_{b}" for t in ("c64", "f64") for a in ("ak", "am")
-            for b in ("bk", "bn")]
+VARIANTS = [f"gett_{t}_{a}_{b}{k}" for t in ("c64", "f64") for a in ("ak", "am")
+            for b in ("bk", "bn") for k in ("", "_shortk")]
 
 
 def measured_traffic(kernel, cfg):

@@ -303,10 +303,13 @@ __global__ __launch_bounds__(kThreads, 2) void gett_kernel(KArgs p) {
 constexpr int kZThreads = 512;
 
 // DEEP keeps two stages of global loads in flight (two register sets): the
-// loads for stage s+3 are issued while stage s is multiplied.
+// loads for stage s+3 are issued while stage s is multiplied.  MINW is the
+// launch bound's minimum of waves per SIMD (2 with 256 threads: two
+// workgroups share a CU, so one's prologue and epilogue overlap the other's
+// multiplies).
 template <int NT, int BM, int BN, int BK, int WM, int WN, bool AKC, bool BKC, bool M3,
-          bool DEEP>
-__global__ __launch_bounds__(NT, 1) void zgett_kernel(KArgs p) {
+          bool DEEP, int MINW = 1>
+__global__ __launch_bounds__(NT, MINW) void zgett_kernel(KArgs p) {
   constexpr int MB = WM / 16, NB = WN / 16;
   constexpr int WAVES_N = BN / WN;
   static_assert((BM / WM) * (BN / WN) * 64 == NT, "one wave per WM x WN block");
@@ -687,16 +690,18 @@ void dispatch_gett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_
     gett_kernel<CPLX, BM, BN, BK, WM, WN, false, false><<<grid, kThreads, 0, s>>>(args);
 }
 
-template <int NT, int BM, int BN, int BK, int WM, int WN, bool M3, bool DEEP = false>
+template <int NT, int BM, int BN, int BK, int WM, int WN, bool M3, bool DEEP = false,
+          int MINW = 1>
 void dispatch_zgett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_t s) {
   if (L.a_kcontig && L.b_kcontig)
-    zgett_kernel<NT, BM, BN, BK, WM, WN, true, true, M3, DEEP><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, true, true, M3, DEEP, MINW><<<grid, NT, 0, s>>>(args);
   else if (L.a_kcontig)
-    zgett_kernel<NT, BM, BN, BK, WM, WN, true, false, M3, DEEP><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, true, false, M3, DEEP, MINW><<<grid, NT, 0, s>>>(args);
   else if (L.b_kcontig)
-    zgett_kernel<NT, BM, BN, BK, WM, WN, false, true, M3, DEEP><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, false, true, M3, DEEP, MINW><<<grid, NT, 0, s>>>(args);
   else
-    zgett_kernel<NT, BM, BN, BK, WM, WN, false, false, M3, DEEP><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, false, false, M3, DEEP, MINW><<<grid, NT, 0, s>>>(
+        args);
 }
 
 // tile shapes (complex elements): first generation 128x64; second 128x128;
@@ -705,15 +710,24 @@ void dispatch_zgett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream
 constexpr int kCBM = 128, kCBN = 64, kCBK = 16, kCWM = 64, kCWN = 32;
 constexpr int kZBM = 128, kZBN = 128, kZBK = 16, kZWM = 64, kZWN = 32;
 constexpr int kZ3BM = 128, kZ3BN = 64, kZ3BK = 16;
+// short K (at most kShortK): generation 3's 32x32 wave tiles, 4 waves on a
+// 64x64 tile, two workgroups per CU (66.5 KB of LDS each)
+constexpr int kZSBM = 64, kZSBN = 64, kShortK = 256;
 constexpr int kRBM = 128, kRBN = 128, kRBK = 16, kRWM = 64, kRWN = 64;
 
 int g_complex_generation = 3;
 
-void tile_of(Field f, int &BM, int &BN, int &BK) {
+bool short_k(Field f, int64_t K) {
+  return f == Field::Complex && g_complex_generation == 3 && K <= kShortK;
+}
+
+void tile_of(Field f, int64_t K, int &BM, int &BN, int &BK) {
   if (f == Field::Real) {
     BM = kRBM; BN = kRBN; BK = kRBK;
   } else if (g_complex_generation == 1) {
     BM = kCBM; BN = kCBN; BK = kCBK;
+  } else if (short_k(f, K)) {
+    BM = kZSBM; BN = kZSBN; BK = kZ3BK;
   } else if (g_complex_generation >= 3) {
     BM = kZ3BM; BN = kZ3BN; BK = kZ3BK;
   } else {
@@ -735,6 +749,7 @@ std::string gett_variant(const GettLaunch &L) {
   std::string n = L.field == Field::Complex ? "gett_c64" : "gett_f64";
   n += L.a_kcontig ? "_ak" : "_am";
   n += L.b_kcontig ? "_bk" : "_bn";
+  if (short_k(L.field, L.K)) n += "_shortk";
   return n;
 }
 
@@ -742,7 +757,7 @@ void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start,
                  hipEvent_t stop) {
   const bool cplx = L.field == Field::Complex;
   int BM, BN, BK;
-  tile_of(L.field, BM, BN, BK);
+  tile_of(L.field, L.K, BM, BN, BK);
   KArgs a;
   a.A = static_cast<const double *>(L.A);
   a.B = static_cast<const double *>(L.B);
@@ -764,6 +779,8 @@ void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start,
   if (start) CC4S_HIP(hipEventRecord(start, s));
   if (cplx && g_complex_generation == 2)
     dispatch_zgett<kZThreads, kZBM, kZBN, kZBK, kZWM, kZWN, false>(L, a, grid, s);
+  else if (short_k(L.field, L.K))
+    dispatch_zgett<256, kZSBM, kZSBN, kZ3BK, 32, 32, true, false, 2>(L, a, grid, s);
   else if (cplx && g_complex_generation == 3)
     dispatch_zgett<512, kZ3BM, kZ3BN, kZ3BK, 32, 32, true>(L, a, grid, s);
   else if (cplx && g_complex_generation == 4)
@@ -994,8 +1011,10 @@ GettPlan &ContractionEngine::plan_for(const View &A0, const View &B0,
 
   // split K when the tile grid cannot fill the 256 CUs twice over
   int BM, BN, BK;
-  tile_of(C.field, BM, BN, BK);
-  const int slots = 256 * ((C.field == Field::Complex && g_complex_generation >= 2) ? 1 : 2);
+  tile_of(C.field, plan->K, BM, BN, BK);
+  // workgroups per CU: one for the 512-thread complex kernels, two otherwise
+  const int slots = 256 * ((C.field == Field::Complex && g_complex_generation >= 2 &&
+                            !short_k(C.field, plan->K)) ? 1 : 2);
   const int64_t tiles = ((plan->M + BM - 1) / BM) * ((plan->N + BN - 1) / BN);
   const int64_t kstages = (plan->K + BK - 1) / BK;
   int64_t splits = 1;

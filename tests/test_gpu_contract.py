@@ -34,8 +34,23 @@ LENS = {"a": 13, "b": 13, "c": 13, "d": 13, "i": 5, "j": 5, "k": 5, "l": 5,
         "G": 37, "x": 7, "y": 6}
 
 
-def _rand(idx, cplx, rng):
-    shape = tuple(LENS[c] for c in idx)
+# longer contracted lengths: every case below has K > 256 (the complex kernel
+# for K <= 256 is a 64x64-tile variant; longer K takes the 128x64 tile)
+LENS_LONG = {"a": 21, "b": 19, "c": 23, "d": 17, "i": 6, "j": 5, "k": 12, "l": 16,
+             "G": 300, "x": 9, "y": 6}
+CASES_LONG = [
+    ("Gai", "Gbj", "abij"),
+    ("akic", "cbkj", "abij"),
+    ("klcd", "adkl", "ac"),
+    ("kldc", "cdil", "ki"),
+    ("cxdy", "cdij", "xyij"),
+    ("Gxc", "Gyd", "cxdy"),
+    ("dclk", "adil", "akic"),
+]
+
+
+def _rand(idx, cplx, rng, lens=None):
+    shape = tuple((lens or LENS)[c] for c in idx)
     x = rng.standard_normal(shape)
     if cplx:
         x = x + 1j * rng.standard_normal(shape)
@@ -49,6 +64,20 @@ def test_contract_matches_einsum(case, cplx):
     # seeded from a stable digest of the case (str hashing is salted per process)
     rng = np.random.default_rng(zlib.crc32(",".join(case).encode()))
     A, B, C = (_rand(i, cplx, rng) for i in (ia, ib, ic))
+    for alpha, beta in ((1.0, 0.0), (-0.5, 1.0)):
+        ref = alpha * np.einsum(f"{ia},{ib}->{ic}", A, B) + beta * C
+        out = api.contract(alpha, A, ia, B, ib, beta, C, ic)
+        assert rel_err(out, ref) < 1e-13, (case, alpha, beta)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("case", CASES_LONG, ids=["%s,%s->%s" % c for c in CASES_LONG])
+def test_contract_long_k_matches_einsum(case, cplx):
+    ia, ib, ic = case
+    rng = np.random.default_rng(zlib.crc32(("long," + ",".join(case)).encode()))
+    A, B, C = (_rand(i, cplx, rng, LENS_LONG) for i in (ia, ib, ic))
+    K = int(np.prod([LENS_LONG[c] for c in ia if c not in ic]))
+    assert K > 256
     for alpha, beta in ((1.0, 0.0), (-0.5, 1.0)):
         ref = alpha * np.einsum(f"{ia},{ib}->{ic}", A, B) + beta * C
         out = api.contract(alpha, A, ia, B, ib, beta, C, ic)
