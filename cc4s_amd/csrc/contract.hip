@@ -848,6 +848,7 @@ void ContractionEngine::apply_env_overrides() {
     mixed_max_k = v;
   }
   // CC4S_MIXED_MIN_FLOPS: smaller complex contractions stay FP64
+  if (const char *e = std::getenv("CC4S_MIXED_TILE")) set_mixed_tile(std::atoi(e));
   if (const char *e = std::getenv("CC4S_MIXED_MIN_FLOPS")) {
     const double v = std::atof(e);
     CC4S_ASSERT(v >= 0.0, "CC4S_MIXED_MIN_FLOPS must not be negative");

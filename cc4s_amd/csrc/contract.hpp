@@ -166,6 +166,10 @@ struct MixedTile {
   int BM, BN, BK;
 };
 MixedTile mixed_tile();
+// multiply tile of the split kernel: 2 (default) 128 x 128 with 4 waves, two
+// workgroups per CU; 1: 128 x 256 with 8 waves, one workgroup per CU
+// (CC4S_MIXED_TILE overrides)
+void set_mixed_tile(int tile);
 // packs k in [k0, k0+Kc) of an operand (FP64 complex, addressed by
 // offMN[mn] + offK[k]) into split words, zero-padded: bf16 parts, or
 // (scale != null) half parts of the operand times scale[0].  Layout
