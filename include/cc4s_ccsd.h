@@ -230,7 +230,9 @@ int cc4s_ccsd_energy_grad_forward(cc4s_ccsd_context *ctx, const double *gamma,
  * skip) and structure_factor[G] = 1/2 sum_pq Re(conj(Gamma^G_pq) grad^G_pq)
  * (host, NG values; NULL to skip), whose sum over G is the energy.
  * precision: CC4S_PRECISION_FP64 (FP64 MFMA) or CC4S_PRECISION_FP32 (FP32
- * operands and arithmetic). */
+ * MFMA operands and sums).  Both directions rebuild the energy weights
+ * 2 tau_abij - tau_baij from the amplitudes tile by tile inside the gradient
+ * products; no amplitude-sized intermediate is stored. */
 int cc4s_ccsd_energy_grad_backward(cc4s_ccsd_context *ctx, const double *gamma,
                                    int gamma_on_device, double *grad,
                                    int grad_on_device, double *structure_factor,
