@@ -182,6 +182,7 @@ def test_flow_value_on_the_next_line(built, tmp_path):
     ("version", "serialization versions mismatch"),
     ("method", "Unknown method: 'Ccsdt'"),
     ("mixer", "Unknown mixer type: 'LinearMixer'"),
+    ("ladder", "unknown ladder 'tiled' (slab | fused)"),
 ])
 def test_reference_errors(built, tmp_path, case, message):
     inp = syn.generate(2, 5, 3, "complex", 1)
@@ -201,6 +202,8 @@ def test_reference_errors(built, tmp_path, case, message):
         steps = steps.replace("method: Ccsd", "method: Ccsdt")
     if case == "mixer":
         steps = steps.replace("type: DiisMixer", "type: LinearMixer")
+    if case == "ladder":
+        steps = steps.replace("    maxIterations: 20\n", "    maxIterations: 20\n    ladder: tiled\n")
     r = run_driver(tmp_path, steps, "--dry-run")
     assert r.returncode == 1
     assert message in r.stderr
