@@ -2,18 +2,17 @@
 // Ccsd.cxx (cited by line); three-factor products are split into two
 // pairwise contractions in the cheaper order.  Conjugate transposes and
 // slices of Gamma are strided views (no copies); the particle-particle
-// ladder forms V_xycd slabs from the dressed vertices and never holds the
-// full v^4 tensor.
+// ladder forms V_xycd slabs from the dressed vertices (or, as an option,
+// forms V tiles in LDS inside one fused kernel) and never holds the full v^4
+// tensor.
 #include "ccsd.hpp"
+#include "elementwise.hpp"
 #include "ladder.hpp"
 #include "triples.hpp"
 
-#include <cstdlib>
-
-#include "elementwise.hpp"
-
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 
 namespace cc4s_amd {
 
