@@ -570,21 +570,24 @@ double Ccsd::triples(const void *Tp, const double *eps_h, const double *eps_p) {
         const int64_t kn = std::min(kc, o - k0);
         const View W = Wt.v.sliced(3, 0, kn);
         auto K = [&](const View &x) { return x.sliced(x.find('k'), k0, k0 + kn); };
-        // particle terms, T'_{abc,ijk}, T'_{bac,jik}, T'_{acb,ikj}, T'_{cba,kji},
-        // T'_{cab,kij}, T'_{bca,jki}
-        E.contract(1.0, fixed(fixed(t2("adij"), 'i', i), 'j', j), K(vp("bcdk")), 0.0, W);
-        E.contract(1.0, fixed(fixed(t2("bdji"), 'j', j), 'i', i), K(vp("acdk")), 1.0, W);
-        E.contract(1.0, K(fixed(t2("adik"), 'i', i)), fixed(vp("cbdj"), 'j', j), 1.0, W);
-        E.contract(1.0, K(fixed(t2("cdkj"), 'j', j)), fixed(vp("badi"), 'i', i), 1.0, W);
-        E.contract(1.0, K(fixed(t2("cdki"), 'i', i)), fixed(vp("abdj"), 'j', j), 1.0, W);
-        E.contract(1.0, K(fixed(t2("bdjk"), 'j', j)), fixed(vp("cadi"), 'i', i), 1.0, W);
-        // hole terms in the same order
-        E.contract(-1.0, fixed(t2("abil"), 'i', i), K(fixed(vh("jklc"), 'j', j)), 1.0, W);
-        E.contract(-1.0, fixed(t2("bajl"), 'j', j), K(fixed(vh("iklc"), 'i', i)), 1.0, W);
-        E.contract(-1.0, fixed(t2("acil"), 'i', i), K(fixed(vh("kjlb"), 'j', j)), 1.0, W);
-        E.contract(-1.0, K(t2("cbkl")), fixed(fixed(vh("jila"), 'j', j), 'i', i), 1.0, W);
-        E.contract(-1.0, K(t2("cakl")), fixed(fixed(vh("ijlb"), 'i', i), 'j', j), 1.0, W);
-        E.contract(-1.0, fixed(t2("bcjl"), 'j', j), K(fixed(vh("kila"), 'i', i)), 1.0, W);
+        // the six particle terms T'_{abc,ijk}, T'_{bac,jik}, T'_{acb,ikj},
+        // T'_{cba,kji}, T'_{cab,kij}, T'_{bca,jki} (K = d) and the six hole
+        // terms (K = l), each hole term paired with the particle term that
+        // splits W's indices the same way ({a}|{bck}, {b}|{ack}, {ak}|{bc},
+        // {ck}|{ab} twice, {bk}|{ac}): one launch per pair, W read and
+        // written six times instead of twelve
+        E.contract2(1.0, fixed(fixed(t2("adij"), 'i', i), 'j', j), K(vp("bcdk")),
+                    -1.0, K(t2("cbkl")), fixed(fixed(vh("jila"), 'j', j), 'i', i), 0.0, W);
+        E.contract2(1.0, fixed(fixed(t2("bdji"), 'j', j), 'i', i), K(vp("acdk")),
+                    -1.0, K(t2("cakl")), fixed(fixed(vh("ijlb"), 'i', i), 'j', j), 1.0, W);
+        E.contract2(1.0, K(fixed(t2("adik"), 'i', i)), fixed(vp("cbdj"), 'j', j),
+                    -1.0, fixed(t2("bcjl"), 'j', j), K(fixed(vh("kila"), 'i', i)), 1.0, W);
+        E.contract2(1.0, K(fixed(t2("cdkj"), 'j', j)), fixed(vp("badi"), 'i', i),
+                    -1.0, fixed(t2("abil"), 'i', i), K(fixed(vh("jklc"), 'j', j)), 1.0, W);
+        E.contract2(1.0, K(fixed(t2("cdki"), 'i', i)), fixed(vp("abdj"), 'j', j),
+                    -1.0, fixed(t2("bajl"), 'j', j), K(fixed(vh("iklc"), 'i', i)), 1.0, W);
+        E.contract2(1.0, K(fixed(t2("bdjk"), 'j', j)), fixed(vp("cadi"), 'i', i),
+                    -1.0, fixed(t2("acil"), 'i', i), K(fixed(vh("kjlb"), 'j', j)), 1.0, W);
         triples_energy(E.stream(), f, W.ptr, w.Vpphh.v.ptr, T1.ptr, eps_h, eps_p, o, v, i, j,
                        k0, kn, acc.as<double>());
       }

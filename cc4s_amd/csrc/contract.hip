@@ -27,6 +27,14 @@ struct KArgs {
   double alpha, beta;
   double signA, signB;  // -1 conjugates the operand's imaginary part
   int tilesM, tilesN;
+  // second K segment (SEG2 kernels): k in [K1p, K1p + K2) reads
+  // A2[rowA2[m] + colA2[k - K1p]] and B2[rowB2[k - K1p] + colB2[n]], times
+  // f2, with its own conjugation signs; the first segment is k < K1 (K1p is
+  // K1 rounded up to whole K-stages)
+  const double *A2, *B2;
+  const int64_t *rowA2, *colA2, *rowB2, *colB2;
+  int64_t K1, K1p, K2;
+  double f2, signA2, signB2;
 };
 
 // Workgroup tile BM x BN, K staged BK at a time through LDS; 4 waves in a
@@ -307,8 +315,12 @@ constexpr int kZThreads = 512;
 // launch bound's minimum of waves per SIMD (2 with 256 threads: two
 // workgroups share a CU, so one's prologue and epilogue overlap the other's
 // multiplies).
+//
+// SEG2 adds a second K segment with its own operands (two contractions with
+// the same result index partition, e.g. a particle and a hole term of (T),
+// accumulated in one pass over C); a K-stage lies in one segment.
 template <int NT, int BM, int BN, int BK, int WM, int WN, bool AKC, bool BKC, bool M3,
-          bool DEEP, int MINW = 1>
+          bool DEEP, int MINW = 1, bool SEG2 = false>
 __global__ __launch_bounds__(NT, MINW) void zgett_kernel(KArgs p) {
   constexpr int MB = WM / 16, NB = WN / 16;
   constexpr int WAVES_N = BN / WN;
@@ -353,21 +365,23 @@ __global__ __launch_bounds__(NT, MINW) void zgett_kernel(KArgs p) {
   auto b_n = [&](int i) { return BKC ? (t / BK + i * (NT / BK)) : (t % BN); };
   auto b_k = [&](int i) { return BKC ? (t % BK) : (t / BN + i * (NT / BN)); };
 
-  int64_t aRow[NRA];
+  int64_t aRow[NRA], aRow2[SEG2 ? NRA : 1];
   bool aRowOk[NRA];
 #pragma unroll
   for (int i = 0; i < NRA; ++i) {
     const int64_t m = m0 + a_m(i);
     aRowOk[i] = m < p.M;
     aRow[i] = aRowOk[i] ? p.rowA[m] : 0;
+    if constexpr (SEG2) aRow2[i] = aRowOk[i] ? p.rowA2[m] : 0;
   }
-  int64_t bCol[NRB];
+  int64_t bCol[NRB], bCol2[SEG2 ? NRB : 1];
   bool bColOk[NRB];
 #pragma unroll
   for (int i = 0; i < NRB; ++i) {
     const int64_t n = n0 + b_n(i);
     bColOk[i] = n < p.N;
     bCol[i] = bColOk[i] ? p.colB[n] : 0;
+    if constexpr (SEG2) bCol2[i] = bColOk[i] ? p.colB2[n] : 0;
   }
 
   // K offsets of two stages: stage t's in buffer t & 1, indexed by
@@ -375,6 +389,7 @@ __global__ __launch_bounds__(NT, MINW) void zgett_kernel(KArgs p) {
   // wait for a load
   int64_t aK[2][NKA], bK[2][NKB];
   bool aKok[2][NKA], bKok[2][NKB];
+  bool segK[2] = {false, false};  // SEG2: the stage lies in the second segment
   // every load is unconditional (out-of-range lanes read a valid element,
   // kbeg < kend or offset 0, and are zeroed when stored to LDS): a load
   // under a branch makes the compiler wait for it at the branch's end, which
@@ -384,17 +399,36 @@ __global__ __launch_bounds__(NT, MINW) void zgett_kernel(KArgs p) {
     int64_t *ak = aK[pb], *bk = bK[pb];
     bool *akok = aKok[pb], *bkok = bKok[pb];
     const int64_t kb = kbeg + static_cast<int64_t>(s) * BK;
+    if constexpr (SEG2) {
+      const bool sg = kb >= p.K1p;
+      segK[pb] = sg;
+      const int64_t k0 = sg ? p.K1p : 0, klen = sg ? p.K2 : p.K1;
+      const int64_t *ca = sg ? p.colA2 : p.colA, *rb = sg ? p.rowB2 : p.rowB;
 #pragma unroll
-    for (int i = 0; i < NKA; ++i) {
-      const int64_t k = kb + a_k(i);
-      akok[i] = k < kend;
-      ak[i] = p.colA[akok[i] ? k : kbeg];
-    }
+      for (int i = 0; i < NKA; ++i) {
+        const int64_t k = kb + a_k(i);
+        akok[i] = k < kend && k - k0 < klen;
+        ak[i] = ca[akok[i] ? k - k0 : 0];
+      }
 #pragma unroll
-    for (int i = 0; i < NKB; ++i) {
-      const int64_t k = kb + b_k(i);
-      bkok[i] = k < kend;
-      bk[i] = p.rowB[bkok[i] ? k : kbeg];
+      for (int i = 0; i < NKB; ++i) {
+        const int64_t k = kb + b_k(i);
+        bkok[i] = k < kend && k - k0 < klen;
+        bk[i] = rb[bkok[i] ? k - k0 : 0];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NKA; ++i) {
+        const int64_t k = kb + a_k(i);
+        akok[i] = k < kend;
+        ak[i] = p.colA[akok[i] ? k : kbeg];
+      }
+#pragma unroll
+      for (int i = 0; i < NKB; ++i) {
+        const int64_t k = kb + b_k(i);
+        bkok[i] = k < kend;
+        bk[i] = p.rowB[bkok[i] ? k : kbeg];
+      }
     }
   };
 
@@ -402,37 +436,52 @@ __global__ __launch_bounds__(NT, MINW) void zgett_kernel(KArgs p) {
   constexpr int NSET = DEEP ? 2 : 1;
   double2 ra[NSET][EA], rb[NSET][EB];
   bool oka[NSET][EA], okb[NSET][EB];
+  bool segS[NSET];  // SEG2: the registers hold a second-segment stage
   auto load_data = [&](auto set_c, auto par_c) {
     constexpr int q = decltype(set_c)::value;
     constexpr int pb = decltype(par_c)::value;
-    const double2 *A = reinterpret_cast<const double2 *>(p.A);
-    const double2 *B = reinterpret_cast<const double2 *>(p.B);
+    bool sg = false;
+    if constexpr (SEG2) sg = segK[pb];
+    segS[q] = sg;
+    const double2 *A = reinterpret_cast<const double2 *>(sg ? p.A2 : p.A);
+    const double2 *B = reinterpret_cast<const double2 *>(sg ? p.B2 : p.B);
 #pragma unroll
     for (int i = 0; i < EA; ++i) {
       const int ir = AKC ? i : 0, ik = AKC ? 0 : i;
       oka[q][i] = aRowOk[ir] && aKok[pb][ik];
-      ra[q][i] = A[oka[q][i] ? aRow[ir] + aK[pb][ik] : 0];
+      int64_t row = aRow[ir];
+      if constexpr (SEG2) row = sg ? aRow2[ir] : row;
+      ra[q][i] = A[oka[q][i] ? row + aK[pb][ik] : 0];
     }
 #pragma unroll
     for (int i = 0; i < EB; ++i) {
       const int ir = BKC ? i : 0, ik = BKC ? 0 : i;
       okb[q][i] = bColOk[ir] && bKok[pb][ik];
-      rb[q][i] = B[okb[q][i] ? bCol[ir] + bK[pb][ik] : 0];
+      int64_t col = bCol[ir];
+      if constexpr (SEG2) col = sg ? bCol2[ir] : col;
+      rb[q][i] = B[okb[q][i] ? col + bK[pb][ik] : 0];
     }
   };
-  // out-of-range elements are zeroed and the conjugation signs applied here,
-  // a stage after the loads were issued
+  // out-of-range elements are zeroed and the conjugation signs (and the
+  // second segment's factor) applied here, a stage after the loads were
+  // issued
   auto store_lds = [&](int buf, auto set_c) {
     constexpr int q = decltype(set_c)::value;
     const double2 zero = make_double2(0.0, 0.0);
+    double fa = 1.0, sa = p.signA, sb = p.signB;
+    if constexpr (SEG2) {
+      fa = segS[q] ? p.f2 : 1.0;
+      sa = segS[q] ? p.signA2 : sa;
+      sb = segS[q] ? p.signB2 : sb;
+    }
 #pragma unroll
     for (int i = 0; i < EA; ++i)
       As[buf][a_k(i)][a_m(i)] =
-          oka[q][i] ? make_double2(ra[q][i].x, ra[q][i].y * p.signA) : zero;
+          oka[q][i] ? make_double2(fa * ra[q][i].x, fa * ra[q][i].y * sa) : zero;
 #pragma unroll
     for (int i = 0; i < EB; ++i)
       Bs[buf][b_k(i)][b_n(i)] =
-          okb[q][i] ? make_double2(rb[q][i].x, rb[q][i].y * p.signB) : zero;
+          okb[q][i] ? make_double2(rb[q][i].x, rb[q][i].y * sb) : zero;
   };
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, DEEP ? 1 : 0>;
@@ -691,17 +740,20 @@ void dispatch_gett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_
 }
 
 template <int NT, int BM, int BN, int BK, int WM, int WN, bool M3, bool DEEP = false,
-          int MINW = 1>
+          int MINW = 1, bool SEG2 = false>
 void dispatch_zgett(const GettLaunch &L, const KArgs &args, dim3 grid, hipStream_t s) {
   if (L.a_kcontig && L.b_kcontig)
-    zgett_kernel<NT, BM, BN, BK, WM, WN, true, true, M3, DEEP, MINW><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, true, true, M3, DEEP, MINW, SEG2>
+        <<<grid, NT, 0, s>>>(args);
   else if (L.a_kcontig)
-    zgett_kernel<NT, BM, BN, BK, WM, WN, true, false, M3, DEEP, MINW><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, true, false, M3, DEEP, MINW, SEG2>
+        <<<grid, NT, 0, s>>>(args);
   else if (L.b_kcontig)
-    zgett_kernel<NT, BM, BN, BK, WM, WN, false, true, M3, DEEP, MINW><<<grid, NT, 0, s>>>(args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, false, true, M3, DEEP, MINW, SEG2>
+        <<<grid, NT, 0, s>>>(args);
   else
-    zgett_kernel<NT, BM, BN, BK, WM, WN, false, false, M3, DEEP, MINW><<<grid, NT, 0, s>>>(
-        args);
+    zgett_kernel<NT, BM, BN, BK, WM, WN, false, false, M3, DEEP, MINW, SEG2>
+        <<<grid, NT, 0, s>>>(args);
 }
 
 // tile shapes (complex elements): first generation 128x64; second 128x128;
@@ -717,8 +769,8 @@ constexpr int kRBM = 128, kRBN = 128, kRBK = 16, kRWM = 64, kRWN = 64;
 
 int g_complex_generation = 3;
 
-bool short_k(Field f, int64_t K) {
-  return f == Field::Complex && g_complex_generation == 3 && K <= kShortK;
+bool short_k(Field f, int64_t K, bool two_segments = false) {
+  return f == Field::Complex && g_complex_generation == 3 && K <= kShortK && !two_segments;
 }
 
 void tile_of(Field f, int64_t K, int &BM, int &BN, int &BK) {
@@ -749,15 +801,19 @@ std::string gett_variant(const GettLaunch &L) {
   std::string n = L.field == Field::Complex ? "gett_c64" : "gett_f64";
   n += L.a_kcontig ? "_ak" : "_am";
   n += L.b_kcontig ? "_bk" : "_bn";
-  if (short_k(L.field, L.K)) n += "_shortk";
+  if (short_k(L.field, L.K, L.A2 != nullptr)) n += "_shortk";
+  if (L.A2) n += "_seg2";
   return n;
 }
 
 void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start,
                  hipEvent_t stop) {
   const bool cplx = L.field == Field::Complex;
+  const bool seg2 = L.A2 != nullptr;
+  CC4S_ASSERT(!seg2 || (cplx && g_complex_generation == 3),
+              "two-segment contractions need the complex generation-3 kernel");
   int BM, BN, BK;
-  tile_of(L.field, L.K, BM, BN, BK);
+  tile_of(L.field, seg2 ? kShortK + 1 : L.K, BM, BN, BK);
   KArgs a;
   a.A = static_cast<const double *>(L.A);
   a.B = static_cast<const double *>(L.B);
@@ -775,12 +831,22 @@ void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start,
   a.tilesN = static_cast<int>((L.N + BN - 1) / BN);
   CC4S_ASSERT(static_cast<int64_t>(a.tilesM) * a.tilesN < (int64_t(1) << 31),
               "contraction too large for one launch");
+  a.A2 = static_cast<const double *>(L.A2);
+  a.B2 = static_cast<const double *>(L.B2);
+  a.rowA2 = L.rowA2; a.colA2 = L.colA2; a.rowB2 = L.rowB2; a.colB2 = L.colB2;
+  a.K1 = L.K1; a.K1p = L.K1p; a.K2 = L.K2;
+  a.f2 = L.f2;
+  a.signA2 = L.conjA2 ? -1.0 : 1.0;
+  a.signB2 = L.conjB2 ? -1.0 : 1.0;
   dim3 grid(a.tilesM * a.tilesN, L.splits);
   if (start) CC4S_HIP(hipEventRecord(start, s));
-  if (cplx && g_complex_generation == 2)
+  if (seg2)
+    dispatch_zgett<512, kZ3BM, kZ3BN, kZ3BK, 32, 32, true, false, 1, true>(L, a, grid, s);
+  else if (cplx && g_complex_generation == 2)
     dispatch_zgett<kZThreads, kZBM, kZBN, kZBK, kZWM, kZWN, false>(L, a, grid, s);
   else if (short_k(L.field, L.K))
     dispatch_zgett<256, kZSBM, kZSBN, kZ3BK, 32, 32, true, false, 2>(L, a, grid, s);
+
   else if (cplx && g_complex_generation == 3)
     dispatch_zgett<512, kZ3BM, kZ3BN, kZ3BK, 32, 32, true>(L, a, grid, s);
   else if (cplx && g_complex_generation == 4)
@@ -822,6 +888,11 @@ struct GettPlan {
   int splits = 1;
   DeviceBuffer tables;
   const int64_t *rowA, *colA, *rowB, *colB, *rowC, *colC;
+  // two-segment plans: the second contraction's tables; swap2 tells that
+  // its second operand carries the M indices
+  bool swap2 = false;
+  int64_t K1 = 0, K1p = 0, K2 = 0;
+  const int64_t *rowA2 = nullptr, *colA2 = nullptr, *rowB2 = nullptr, *colB2 = nullptr;
 };
 
 ContractionEngine::ContractionEngine(hipStream_t s) : stream_(s) { apply_env_overrides(); }
@@ -1032,6 +1103,150 @@ GettPlan &ContractionEngine::plan_for(const View &A0, const View &B0,
   auto &ref = *plan;
   plans_.emplace(key, std::move(plan));
   return ref;
+}
+
+GettPlan &ContractionEngine::plan_for2(const View &A1, const View &B1, const View &A2,
+                                       const View &B2, const View &C) {
+  const std::string key = "2|" + plan_key(A1, B1, C) + "|" + plan_key(A2, B2, C);
+  auto it = plans_.find(key);
+  if (it != plans_.end()) return *it->second;
+  // the first contraction's plan fixes the M and N orders and the swap
+  const GettPlan &P1 = plan_for(A1, B1, C);
+  auto plan = std::make_unique<GettPlan>();
+  plan->M = P1.M;
+  plan->N = P1.N;
+  plan->swap = P1.swap;
+  plan->a_kc = P1.a_kc;
+  plan->b_kc = P1.b_kc;
+  // M indices: C's indices carried by the first contraction's A-role operand,
+  // in the order of its strides (as plan_for orders them)
+  const View &Aop = P1.swap ? B1 : A1;
+  std::string Mi, Ni;
+  for (char c : C.idx) (Aop.find(c) >= 0 ? Mi : Ni) += c;
+  auto by = [](const View &v) {
+    return [&v](char x, char y) { return v.stride(x) < v.stride(y); };
+  };
+  std::stable_sort(Mi.begin(), Mi.end(), by(Aop));
+  std::stable_sort(Ni.begin(), Ni.end(), by(C));
+  auto c_set = [&C](const View &v) {
+    std::string s;
+    for (char c : v.idx)
+      if (C.find(c) >= 0) s += c;
+    std::sort(s.begin(), s.end());
+    return s;
+  };
+  std::string ms = Mi, ns = Ni;
+  std::sort(ms.begin(), ms.end());
+  std::sort(ns.begin(), ns.end());
+  CC4S_ASSERT((c_set(A2) == ms && c_set(B2) == ns) || (c_set(A2) == ns && c_set(B2) == ms),
+              "the two contractions split the result indices differently");
+  plan->swap2 = c_set(A2) != ms;
+  const View &A = plan->swap2 ? B2 : A2;
+  const View &B = plan->swap2 ? A2 : B2;
+  std::string Ki;
+  for (char c : A.idx) {
+    if (C.find(c) >= 0) continue;
+    CC4S_ASSERT(B.find(c) >= 0 && A.len(c) == B.len(c),
+                std::string("index '") + c + "' of the second contraction");
+    Ki += c;
+  }
+  CC4S_ASSERT(!Ki.empty(), "the second contraction has no summed index");
+  for (char c : Mi) CC4S_ASSERT(A.len(c) == C.len(c), "length mismatch (A2, C)");
+  for (char c : Ni) CC4S_ASSERT(B.len(c) == C.len(c), "length mismatch (B2, C)");
+  bool a_unit_k = false;
+  for (char c : Ki) a_unit_k |= A.stride(c) == 1 && A.len(c) > 1;
+  std::stable_sort(Ki.begin(), Ki.end(), a_unit_k ? by(A) : by(B));
+  auto rowA = offsets(A, Mi, C), colA = offsets(A, Ki, A);
+  auto rowB = offsets(B, Ki, A), colB = offsets(B, Ni, C);
+  CC4S_ASSERT(static_cast<int64_t>(rowA.size()) == plan->M &&
+                  static_cast<int64_t>(colB.size()) == plan->N,
+              "the two contractions have different result shapes");
+  plan->K1 = P1.K;
+  plan->K1p = (P1.K + kZ3BK - 1) / kZ3BK * kZ3BK;
+  plan->K2 = static_cast<int64_t>(colA.size());
+  plan->K = plan->K1p + plan->K2;
+  std::vector<int64_t> all;
+  all.insert(all.end(), rowA.begin(), rowA.end());
+  all.insert(all.end(), colA.begin(), colA.end());
+  all.insert(all.end(), rowB.begin(), rowB.end());
+  all.insert(all.end(), colB.begin(), colB.end());
+  plan->tables.allocate(all.size() * sizeof(int64_t));
+  CC4S_HIP(hipMemcpy(plan->tables.get(), all.data(), all.size() * sizeof(int64_t),
+                     hipMemcpyHostToDevice));
+  const int64_t *base = plan->tables.as<int64_t>();
+  plan->rowA2 = base; base += rowA.size();
+  plan->colA2 = base; base += colA.size();
+  plan->rowB2 = base; base += rowB.size();
+  plan->colB2 = base;
+  plan->rowA = P1.rowA; plan->colA = P1.colA; plan->rowB = P1.rowB; plan->colB = P1.colB;
+  plan->rowC = P1.rowC; plan->colC = P1.colC;
+  // split K when the tile grid cannot fill the CUs twice over (one
+  // workgroup per CU)
+  const int64_t tiles = ((plan->M + kZ3BM - 1) / kZ3BM) * ((plan->N + kZ3BN - 1) / kZ3BN);
+  const int64_t kstages = plan->K / kZ3BK + 1;
+  int64_t splits = 1;
+  if (tiles < 512) {
+    splits = std::min<int64_t>((1024 + tiles - 1) / tiles, kstages / 4);
+    while (splits > 1 && static_cast<size_t>(splits) * plan->M * plan->N * 16 > max_workspace)
+      --splits;
+    splits = std::max<int64_t>(1, std::min<int64_t>(splits, 256));
+  }
+  plan->splits = static_cast<int>(splits);
+  auto &ref = *plan;
+  plans_.emplace(key, std::move(plan));
+  return ref;
+}
+
+void ContractionEngine::contract2(double alpha, const View &A1, const View &B1,
+                                  double alpha2, const View &A2, const View &B2,
+                                  double beta, const View &C) {
+  CC4S_ASSERT(A1.field == C.field && B1.field == C.field && A2.field == C.field &&
+                  B2.field == C.field,
+              "mixed scalar fields");
+  if (C.field != Field::Complex || g_complex_generation != 3 || mixed_ || alpha == 0.0) {
+    contract(alpha, A1, B1, beta, C);
+    contract(alpha2, A2, B2, 1.0, C);
+    return;
+  }
+  CC4S_ASSERT(!C.conj, "conjugated result view");
+  GettPlan &P = plan_for2(A1, B1, A2, B2, C);
+  if (P.M == 0 || P.N == 0) return;
+  GettLaunch L;
+  L.field = C.field;
+  L.A = P.swap ? B1.ptr : A1.ptr;
+  L.B = P.swap ? A1.ptr : B1.ptr;
+  L.conjA = P.swap ? B1.conj : A1.conj;
+  L.conjB = P.swap ? A1.conj : B1.conj;
+  L.A2 = P.swap2 ? B2.ptr : A2.ptr;
+  L.B2 = P.swap2 ? A2.ptr : B2.ptr;
+  L.conjA2 = P.swap2 ? B2.conj : A2.conj;
+  L.conjB2 = P.swap2 ? A2.conj : B2.conj;
+  L.C = C.ptr;
+  L.rowA = P.rowA; L.colA = P.colA; L.rowB = P.rowB; L.colB = P.colB;
+  L.rowA2 = P.rowA2; L.colA2 = P.colA2; L.rowB2 = P.rowB2; L.colB2 = P.colB2;
+  L.rowC = P.rowC; L.colC = P.colC;
+  L.M = P.M; L.N = P.N; L.K = P.K;
+  L.K1 = P.K1; L.K1p = P.K1p; L.K2 = P.K2;
+  L.f2 = alpha2 / alpha;
+  L.splits = P.splits;
+  L.alpha = alpha; L.beta = beta;
+  L.a_kcontig = P.a_kc; L.b_kcontig = P.b_kc;
+  L.W = nullptr;
+  if (P.splits > 1) {
+    workspace_.reserve(static_cast<size_t>(P.splits) * P.M * P.N * 16);
+    L.W = workspace_.get();
+  }
+  const double flops = 8.0 * double(P.M) * double(P.N) * double(P.K1 + P.K2);
+  if (profiling_) {
+    Pending p{take_event(), take_event(), gett_variant(L), flops};
+    launch_gett(L, stream_, p.start, p.stop);
+    pending_.push_back(p);
+    if (pending_.size() > 4096) (void)timings();
+  } else {
+    launch_gett(L, stream_);
+  }
+  stats.flops += flops;
+  stats.launches += 1;
 }
 
 void ContractionEngine::contract(double alpha, const View &A, const View &B,

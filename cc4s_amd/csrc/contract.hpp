@@ -43,6 +43,12 @@ class ContractionEngine {
   // C = alpha * op(A) op(B) + beta * C, op = conjugation where the view says
   void contract(double alpha, const View &A, const View &B, double beta,
                 const View &C);
+  // C = alpha * op(A1) op(B1) + alpha2 * op(A2) op(B2) + beta * C for two
+  // contractions whose operands split C's indices the same way: one launch
+  // with the second contraction as a second K segment, so C is read and
+  // written once (complex FP64 passes; otherwise two launches)
+  void contract2(double alpha, const View &A1, const View &B1, double alpha2,
+                 const View &A2, const View &B2, double beta, const View &C);
   // C = alpha * op(A) + beta * C, A's indices are a permutation of C's
   void sum(double alpha, const View &A, double beta, const View &C);
   // C = beta * C
@@ -104,6 +110,8 @@ class ContractionEngine {
 
  private:
   GettPlan &plan_for(const View &A, const View &B, const View &C);
+  GettPlan &plan_for2(const View &A1, const View &B1, const View &A2, const View &B2,
+                      const View &C);
   void contract_mixed(GettPlan &P, double alpha, const View &A, const View &B,
                       double beta, const View &C, double flops);
   bool mixed_ = false;
@@ -141,6 +149,14 @@ struct GettLaunch {
   double alpha, beta;
   bool conjA, conjB;
   bool a_kcontig, b_kcontig;
+  // optional second K segment (complex generation-3 kernel): k in
+  // [K1p, K1p + K2) from A2, B2 and their tables, times f2; K = K1p + K2,
+  // K1p = K1 rounded up to whole K-stages (16)
+  const void *A2 = nullptr, *B2 = nullptr;
+  const int64_t *rowA2 = nullptr, *colA2 = nullptr, *rowB2 = nullptr, *colB2 = nullptr;
+  int64_t K1 = 0, K1p = 0, K2 = 0;
+  double f2 = 1.0;
+  bool conjA2 = false, conjB2 = false;
 };
 // `start`/`stop` (optional) bracket the main kernel, not the reduction
 void launch_gett(const GettLaunch &L, hipStream_t s, hipEvent_t start = nullptr,

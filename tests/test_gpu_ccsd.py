@@ -283,10 +283,14 @@ def test_ueg_energies_of_the_reference_test_suite():
 
 
 @pytest.mark.parametrize("field", ["complex", "real"])
-@pytest.mark.parametrize("shape", [(3, 7, 5), (4, 12, 30)], ids=["o3v7G5", "o4v12G30"])
+@pytest.mark.parametrize("shape", [(3, 7, 5), (4, 12, 30), (5, 33, 20)],
+                         ids=["o3v7G5", "o4v12G30", "o5v33G20"])
 def test_triples_parity(field, shape):
     """(T) of given amplitudes (PerturbativeTriplesReference.cxx:55-180)
-    against the oracle's restatement, 1e-12 relative."""
+    against the oracle's restatement, 1e-12 relative.  The complex field
+    pairs each particle term (K = Nv) with a hole term (K = No) in one
+    two-segment contraction; Nv = 7 and 33 pad the first segment to whole
+    K-stages of 16."""
     No, Nv, NG = shape
     print(f"triples {field} {shape}", flush=True)
     inp = syn.generate(No, Nv, NG, field, 23)
