@@ -14,14 +14,17 @@
 // 1/6 for three equal.  Bandwidth-bound: each element of S is formed once,
 // in LDS, and read there at its six orderings.  Each block keeps its running
 // sum across launches in `acc[block]` (one stream: a fixed order), so the
-// energy is reproducible.
+// energy is reproducible.  Blocks of 512 threads, as many as are resident at
+// once (the grid is one wave of blocks: a second, partial wave would leave
+// most CUs idle while it runs).
 #include "triples.hpp"
 
 namespace cc4s_amd {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 512;
+constexpr int kComplexBlocks = 768;
 
 template <bool CPLX>
 struct TS;
@@ -79,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void triples_energy_kernel(
   using T = typename S::T;
   // S at the six orderings of the canonical tile triple's index ranges
   __shared__ T tile[6][kTileElems];
-  __shared__ double sh[kBlock];
+  __shared__ double sh[kBlock / 64];
   const int64_t v = Nv, o = No;
   const int64_t nt = (v + kTile - 1) / kTile;
   const int64_t ntiles = nt * nt * nt * kc;
@@ -162,7 +165,7 @@ void triples_energy(hipStream_t s, Field f, const void *W, const void *Vpphh,
   CC4S_ASSERT(i >= 0 && i <= j && j < No && k0 >= j && kc > 0 && k0 + kc <= No,
               "triples: hole indices out of range (needs i <= j <= k)");
   if (f == Field::Complex)
-    triples_energy_kernel<true><<<kTriplesBlocks, kBlock, 0, s>>>(
+    triples_energy_kernel<true><<<kComplexBlocks, kBlock, 0, s>>>(
         static_cast<const double2 *>(W), static_cast<const double2 *>(Vpphh),
         static_cast<const double2 *>(T1), eps_h, eps_p, No, Nv, i, j, k0, kc, acc);
   else

@@ -7,7 +7,10 @@
 
 namespace cc4s_amd {
 
-// fixed grid of the energy kernel; `acc` holds one running sum per block
+// largest grid of the energy kernel (one running sum per block in `acc`);
+// the complex kernel uses 768 blocks (three per CU, bound by LDS), the real
+// one 1024 (four per CU, bound by threads): one resident wave of blocks, no
+// tail
 constexpr int kTriplesBlocks = 1024;
 
 // acc[block] += partial sums of the weighted Re(W Zs)/D over W[a,b,c,k - k0],
