@@ -910,7 +910,7 @@ void ContractionEngine::apply_env_overrides() {
     set_complex_gett_generation(std::atoi(e));
   if (const char *e = std::getenv("CC4S_MIXED_TERMS")) {
     const int v = std::atoi(e);
-    CC4S_ASSERT(v == 12 || v == 16, "CC4S_MIXED_TERMS must be 12 or 16");
+    CC4S_ASSERT(v == 9 || v == 12 || v == 16, "CC4S_MIXED_TERMS must be 9, 12 or 16");
     mixed_terms = v;
   }
   if (const char *e = std::getenv("CC4S_MIXED_MAX_K")) {
@@ -1297,7 +1297,11 @@ void ContractionEngine::contract(double alpha, const View &A, const View &B,
 void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
                                        const View &B, double beta, const View &C,
                                        double flops) {
-  const MixedTile T = mixed_tile();
+  // 9 products per complex MAC: half parts plus a third plane of Re + Im
+  const bool nine = mixed_half && mixed_terms == 9;
+  const int terms = mixed_terms == 9 && !mixed_half ? 12 : mixed_terms;
+  const size_t eb = nine ? 12 : 8;  // packed bytes per element and k
+  const MixedTile T = mixed_tile(terms);
   const int64_t Mp = (P.M + T.BM - 1) / T.BM * T.BM;
   const int64_t Np = (P.N + T.BN - 1) / T.BN * T.BN;
   const int64_t Kfull = (P.K + T.BK - 1) / T.BK * T.BK;
@@ -1327,36 +1331,39 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
   }
   if (cacheA || cacheB) {
     const int64_t rows = cacheA ? Mp : Np;
-    if (cache_plan_[slot] != &P || cache_is_a_[slot] != cacheA) {
-      cache_[slot].reserve(static_cast<size_t>(rows) * Kfull * 8);
+    if (cache_plan_[slot] != &P || cache_is_a_[slot] != cacheA || cache_eb_[slot] != eb) {
+      cache_[slot].reserve(static_cast<size_t>(rows) * Kfull * eb);
+      char *cs = nine ? cache_[slot].as<char>() + static_cast<size_t>(rows) * Kfull * 8
+                      : nullptr;
       if (cacheA) {
         if (mixed_half)
           launch_split_scale(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M,
-                             P.K, P.a_kc, scPart, scC, stream_);
+                             P.K, P.a_kc, nine, scPart, scC, stream_);
         launch_pack(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K, 0,
-                    Kfull, Mp, Aop.conj, P.a_kc, cache_[slot].get(), scC, stream_);
+                    Kfull, Mp, Aop.conj, P.a_kc, cache_[slot].get(), cs, scC, stream_);
       } else {
         if (mixed_half)
           launch_split_scale(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N,
-                             P.K, P.b_kc, scPart, scC, stream_);
+                             P.K, P.b_kc, nine, scPart, scC, stream_);
         launch_pack(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K, 0,
-                    Kfull, Np, Bop.conj, P.b_kc, cache_[slot].get(), scC, stream_);
+                    Kfull, Np, Bop.conj, P.b_kc, cache_[slot].get(), cs, scC, stream_);
       }
       cache_plan_[slot] = &P;
       cache_is_a_[slot] = cacheA;
+      cache_eb_[slot] = eb;
     }
   }
   if (mixed_half && !cacheA)
     launch_split_scale(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K,
-                       P.a_kc, scPart, scA, stream_);
+                       P.a_kc, nine, scPart, scA, stream_);
   if (mixed_half && !cacheB)
     launch_split_scale(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K,
-                       P.b_kc, scPart, scB, stream_);
+                       P.b_kc, nine, scPart, scB, stream_);
   const double *useScA = mixed_half ? (cacheA ? scC : scA) : nullptr;
   const double *useScB = mixed_half ? (cacheB ? scC : scB) : nullptr;
   const int64_t packed_rows = (cacheA ? 0 : Mp) + (cacheB ? 0 : Np);
   int64_t Kc = packed_rows > 0
-                   ? static_cast<int64_t>(pack_budget / (static_cast<size_t>(packed_rows) * 8))
+                   ? static_cast<int64_t>(pack_budget / (static_cast<size_t>(packed_rows) * eb))
                    : Kfull;
   Kc = std::max<int64_t>(T.BK, Kc / T.BK * T.BK);
   Kc = std::min(Kc, Kfull);
@@ -1371,9 +1378,16 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
   // no FP32 sum (one split of one chunk) runs over more than mixed_max_k
   Kc = std::min<int64_t>(Kc, std::max<int64_t>(T.BK, mixed_max_k * splits / T.BK * T.BK));
   const int64_t chunks = (P.K + Kc - 1) / Kc;
-  pack_.reserve(static_cast<size_t>(packed_rows) * Kc * 8);
+  pack_.reserve(static_cast<size_t>(packed_rows) * Kc * eb);
   char *Apk = pack_.as<char>();
   char *Bpk = Apk + (cacheA ? 0 : static_cast<size_t>(Mp) * Kc * 8);
+  // the Re + Im planes follow the main planes
+  char *Aps = nine ? Apk + static_cast<size_t>(packed_rows) * Kc * 8 : nullptr;
+  char *Bps = nine ? Aps + (cacheA ? 0 : static_cast<size_t>(Mp) * Kc * 4) : nullptr;
+  const char *cacheS =
+      nine && (cacheA || cacheB)
+          ? cache_[slot].as<char>() + static_cast<size_t>(cacheA ? Mp : Np) * Kfull * 8
+          : nullptr;
   double *W = nullptr;
   if (splits > 1) {
     workspace_.reserve(static_cast<size_t>(splits) * P.M * P.N * 16);
@@ -1394,15 +1408,21 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
     const void *Buse = cacheB ? static_cast<const void *>(cache_[slot].as<char>() +
                                                           static_cast<size_t>(k0) * Np * 8)
                               : Bpk;
+    const void *AuseS = !nine ? nullptr
+                        : cacheA ? static_cast<const void *>(cacheS + static_cast<size_t>(k0) * Mp * 4)
+                                 : Aps;
+    const void *BuseS = !nine ? nullptr
+                        : cacheB ? static_cast<const void *>(cacheS + static_cast<size_t>(k0) * Np * 4)
+                                 : Bps;
     if (!cacheA)
       launch_pack(static_cast<const double *>(Aop.ptr), P.rowA, P.colA, P.M, P.K, k0,
-                  kc, Mp, Aop.conj, P.a_kc, Apk, useScA, stream_);
+                  kc, Mp, Aop.conj, P.a_kc, Apk, Aps, useScA, stream_);
     if (!cacheB)
       launch_pack(static_cast<const double *>(Bop.ptr), P.colB, P.rowB, P.N, P.K, k0,
-                  kc, Np, Bop.conj, P.b_kc, Bpk, useScB, stream_);
+                  kc, Np, Bop.conj, P.b_kc, Bpk, Bps, useScB, stream_);
     const double bc = c == 0 ? beta : 1.0;
     launch_bgemm(Ause, Buse, static_cast<double *>(C.ptr), W, P.rowC, P.colC, P.M,
-                 P.N, Mp, Np, kc, splits, alpha, bc, useScA, useScB, mixed_terms,
+                 P.N, Mp, Np, kc, splits, alpha, bc, useScA, useScB, terms, AuseS, BuseS,
                  stream_);
     if (splits > 1)
       launch_splitk_reduce(Field::Complex, W, static_cast<double *>(C.ptr), P.rowC,

@@ -80,7 +80,9 @@ class ContractionEngine {
   bool mixed_half = false;
   // 16-bit products per complex multiply-add of the split kernel: 12 (the
   // hi*hi, hi*lo and lo*hi products of both real parts) or 16 (also lo*lo,
-  // which lies 2^-22 below the product, under the FP32 sums' rounding)
+  // which lies 2^-22 below the product, under the FP32 sums' rounding), or,
+  // with half parts, 9 (three real products per complex multiply-add: Re Re,
+  // Im Im and (Re + Im)(Re + Im), each split in three)
   int mixed_terms = 12;
   // CC4S_MIXED_SPLIT, CC4S_MIXED_MAX_K, CC4S_MIXED_TERMS and
   // CC4S_GETT_GENERATION, when set, replace the settings above and the
@@ -122,6 +124,7 @@ class ContractionEngine {
   std::string cache_idx_[kCacheSlots];
   const GettPlan *cache_plan_[kCacheSlots] = {nullptr, nullptr};
   bool cache_is_a_[kCacheSlots] = {true, true};
+  size_t cache_eb_[kCacheSlots] = {0, 0};  // packed bytes per element and k
   struct Pending {
     hipEvent_t start, stop;
     std::string name;
@@ -181,34 +184,37 @@ void launch_splitk_reduce(Field f, const double *W, double *C,
 struct MixedTile {
   int BM, BN, BK;
 };
-MixedTile mixed_tile();
 // multiply tile of the split kernel: 2 (default) 128 x 128 with 4 waves, two
 // workgroups per CU; 1: 128 x 256 with 8 waves, one workgroup per CU
-// (CC4S_MIXED_TILE overrides)
+// (CC4S_MIXED_TILE overrides); the 9-product kernel always uses 128 x 128
+MixedTile mixed_tile(int terms = 12);
 void set_mixed_tile(int tile);
 // packs k in [k0, k0+Kc) of an operand (FP64 complex, addressed by
 // offMN[mn] + offK[k]) into split words, zero-padded: bf16 parts, or
 // (scale != null) half parts of the operand times scale[0].  Layout
 // [Kc/2][MNp] of 16-byte pairs: pair (p, mn) holds the words of k = 2p and
 // 2p+1, word = (Re hi | Re lo << 16, Im hi | Im lo << 16); Kc is even.
+// out_s (half only, or null): [Kc/2][MNp] 8-byte pairs of the words of
+// Re + Im (of the conjugate when conj) of k = 2p and 2p+1.
 void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
                  int64_t MN, int64_t K, int64_t k0, int64_t Kc, int64_t MNp,
-                 bool conj, bool kcontig, void *out, const double *scale,
-                 hipStream_t s);
+                 bool conj, bool kcontig, void *out, void *out_s,
+                 const double *scale, hipStream_t s);
 // scale[0] = the power of two that brings max(|Re|, |Im|) of the operand
-// into [2^14, 2^15) (the half split's range); partials holds
-// split_scale_partials_bytes()
+// (max(|Re| + |Im|) with sumabs) into [2^14, 2^15) (the half split's range);
+// partials holds split_scale_partials_bytes()
 void launch_split_scale(const double *X, const int64_t *offMN, const int64_t *offK,
-                        int64_t MN, int64_t K, bool kcontig, double *partials,
-                        double *scale, hipStream_t s);
+                        int64_t MN, int64_t K, bool kcontig, bool sumabs,
+                        double *partials, double *scale, hipStream_t s);
 size_t split_scale_partials_bytes();
 // C (or split-K partials W) from packed operands ([Kc/2][Mp] and [Kc/2][Np]
 // pairs); scaleA/scaleB (both or neither): half parts, the sums are divided
-// by both; terms: 16-bit products per complex multiply-add (12 or 16)
+// by both; terms: 16-bit products per complex multiply-add (12 or 16, or 9:
+// half parts only, with the Re + Im planes Asum/Bsum of both operands)
 void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
                   const int64_t *rowC, const int64_t *colC, int64_t M, int64_t N,
                   int64_t Mp, int64_t Np, int64_t Kc, int splits, double alpha,
                   double beta, const double *scaleA, const double *scaleB,
-                  int terms, hipStream_t s);
+                  int terms, const void *Asum, const void *Bsum, hipStream_t s);
 
 }  // namespace cc4s_amd

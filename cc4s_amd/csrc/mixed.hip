@@ -74,7 +74,9 @@ __device__ __forceinline__ unsigned int split_part_f16(double x) {
 constexpr int kPackTile = 64;
 constexpr int kMaxBlocks = 1024;
 
-// per-block max(|re|, |im|) over the operand elements offMN[mn] + offK[k].
+// per-block max(|re|, |im|) (or max(|re| + |im|) with `sumabs`, the range of
+// re + im for the three-product split) over the operand elements
+// offMN[mn] + offK[k].
 // Work items are segments of 1024 consecutive elements of one line of the
 // contiguous index (k if kcontig, else mn); a wave takes one item, its lanes
 // read 16 elements each with coalesced loads.
@@ -82,7 +84,7 @@ constexpr int kSegment = 1024;
 
 __global__ __launch_bounds__(256) void maxabs_kernel(const double2 *X, const int64_t *offMN,
                                                      const int64_t *offK, int64_t MN,
-                                                     int64_t K, bool kcontig,
+                                                     int64_t K, bool kcontig, bool sumabs,
                                                      double *partials) {
   __shared__ double sh[256];
   double m = 0.0;
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(256) void maxabs_kernel(const double2 *X, const int
 #pragma unroll 4
     for (int64_t i = i0 + lane; i < i1; i += 64) {
       const double2 x = X[base + inOff[i]];
-      m = fmax(m, fmax(fabs(x.x), fabs(x.y)));
+      m = fmax(m, sumabs ? fabs(x.x) + fabs(x.y) : fmax(fabs(x.x), fabs(x.y)));
     }
   }
   m = wave_max(m);
@@ -139,14 +141,18 @@ struct PackArgs {
   double sign;           // -1 conjugates
   bool kcontig;          // source contiguous along k
   uint4 *out;            // [Kc/2][MNp] pairs of k
+  uint2 *out_s;          // or null: [Kc/2][MNp] pairs of k of the words of re + im
   const double *scale;   // FP16: the operand's power-of-two scale
 };
 
 // packs a 64(k) x 64(mn) tile: reads in the source's contiguous order,
-// transposes through LDS, writes pairs of k mn-fastest
-template <bool FP16>
+// transposes through LDS, writes pairs of k mn-fastest; SUMS also writes the
+// words of re + im (half parts only)
+template <bool FP16, bool SUMS>
 __global__ __launch_bounds__(256) void pack_kernel(PackArgs p) {
+  static_assert(FP16 || !SUMS, "the re + im plane is packed in half parts");
   __shared__ uint2 tile[kPackTile][kPackTile + 1];
+  __shared__ unsigned int tile_s[SUMS ? kPackTile : 1][kPackTile + 1];
   // the tile's offsets, read once (-1: outside the operand)
   __shared__ int64_t oK[kPackTile], oMN[kPackTile];
   const double sc = FP16 ? p.scale[0] : 1.0;
@@ -165,18 +171,21 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs p) {
     const int a = e % kPackTile, b = e / kPackTile;
     const int kk = p.kcontig ? a : b, mm = p.kcontig ? b : a;
     uint2 v = make_uint2(0u, 0u);
+    unsigned int vs = 0u;
     const int64_t ok = oK[kk], om = oMN[mm];
     if (ok >= 0 && om >= 0) {
       const double2 x = p.X[om + ok];
       if constexpr (FP16) {
         v.x = split_part_f16(sc * x.x);
         v.y = split_part_f16(sc * p.sign * x.y);
+        if constexpr (SUMS) vs = split_part_f16(sc * (x.x + p.sign * x.y));
       } else {
         v.x = split_part(x.x);
         v.y = split_part(p.sign * x.y);
       }
     }
     tile[kk][mm] = v;
+    if constexpr (SUMS) tile_s[kk][mm] = vs;
   }
   __syncthreads();
   // kb and Kc are even: a pair never straddles the chunk's end
@@ -186,6 +195,8 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs p) {
     if (krow < p.Kc && mn < p.MNp) {
       const uint2 w0 = tile[2 * kp][mm], w1 = tile[2 * kp + 1][mm];
       p.out[(krow >> 1) * p.MNp + mn] = make_uint4(w0.x, w0.y, w1.x, w1.y);
+      if constexpr (SUMS)
+        p.out_s[(krow >> 1) * p.MNp + mn] = make_uint2(tile_s[2 * kp][mm], tile_s[2 * kp + 1][mm]);
     }
   }
 }
@@ -193,6 +204,7 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs p) {
 struct BgArgs {
   const uint4 *A;  // [Kc/2][Mp] pairs of k
   const uint4 *B;  // [Kc/2][Np] pairs of k
+  const uint2 *As, *Bs;  // nine products: the re + im words, [Kc/2][Mp], [Kc/2][Np]
   double *C;
   double *W;
   const int64_t *rowC, *colC;
@@ -470,6 +482,219 @@ __global__ __launch_bounds__(NT, MINW) void bgemm_kernel(BgArgs p) {
   }
 }
 
+// --- 9 products per complex multiply-add (three-product complex form) -----
+//   P1 = sum ar br, P2 = sum ai bi, P3 = sum (ar + ai)(br + bi),
+//   Re = P1 - P2, Im = P3 - P1 - P2 (formed in FP64 from the FP32 sums),
+// each real product split as hi*hi + hi*lo + lo*hi: 9 half products per
+// complex MAC instead of 12.  s = ar + ai is split from its FP64 value (the
+// operand's scale keeps |ar| + |ai| inside half's range).  Per K-stage (16
+// complex k) a half-wave takes 8 k; one part's 8 words (x_k = h | l << 16)
+// become 12 operand words, 24 k-slots = 3 MFMAs:
+//   A: [ah ah al] per k:  (ah0 ah0) (al0 ah1) (ah1 al1)  per two k
+//   B: [bh bl bh] per k:  (bh0 bl0) (bh0 bh1) (bl1 bh1)  per two k
+struct Ops9 {
+  u32x4 v[3];
+};
+__device__ __forceinline__ Ops9 expand9_a(const unsigned int (&x)[8]) {
+  Ops9 o;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    put12(*reinterpret_cast<Ops12 *>(&o), 3 * e, lo_lo(x[2 * e]));
+    put12(*reinterpret_cast<Ops12 *>(&o), 3 * e + 1, hi_lo(x[2 * e + 1], x[2 * e]));
+    put12(*reinterpret_cast<Ops12 *>(&o), 3 * e + 2, x[2 * e + 1]);
+  }
+  return o;
+}
+__device__ __forceinline__ Ops9 expand9_b(const unsigned int (&y)[8]) {
+  Ops9 o;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    put12(*reinterpret_cast<Ops12 *>(&o), 3 * e, y[2 * e]);
+    put12(*reinterpret_cast<Ops12 *>(&o), 3 * e + 1, lo_pair(y[2 * e + 1], y[2 * e]));
+    put12(*reinterpret_cast<Ops12 *>(&o), 3 * e + 2, hi_lo(y[2 * e + 1], y[2 * e + 1]));
+  }
+  return o;
+}
+
+// 4 waves of 64 x 64 on a 128 x 128 tile, one workgroup per CU (three
+// accumulator sets and the raw words of two k-blocks fill the register file
+// of one wave per SIMD); LDS holds two K-stages of the (re, im) pairs and the
+// re + im pairs.
+template <int NT, int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__(NT, 1) void bgemm9_kernel(BgArgs p) {
+  constexpr int MB = WM / 32, NB = WN / 32;
+  constexpr int WAVES_N = BN / WN;
+  constexpr int BP = BK / 2;
+  static_assert((BM / WM) * (BN / WN) * 64 == NT, "one wave per WM x WN block");
+  static_assert(BK == 16 && MB == 2 && NB == 2, "one 16-k step per stage, 2 x 2 blocks");
+  constexpr int QA = BP * BM / NT, QB = BP * BN / NT;
+  static_assert(QA == 4 && QB == 4 && NT % BM == 0 && NT % BN == 0, "staging map");
+
+  __shared__ uint4 As[2][BP][BM];
+  __shared__ uint4 Bs[2][BP][BN];
+  __shared__ uint2 Av[2][BP][BM];
+  __shared__ uint2 Bv[2][BP][BN];
+
+  const int t = threadIdx.x;
+  const int ntiles = p.tilesM * p.tilesN;
+  int g = blockIdx.x;
+  if ((ntiles & 7) == 0) g = (g & 7) * (ntiles >> 3) + (g >> 3);
+  constexpr int GM = 8;
+  const int per_group = GM * p.tilesN;
+  const int first_m = (g / per_group) * GM;
+  const int gsize = min(p.tilesM - first_m, GM);
+  const int tm = first_m + (g % per_group) % gsize;
+  const int tn = (g % per_group) / gsize;
+  const int64_t m0 = static_cast<int64_t>(tm) * BM;
+  const int64_t n0 = static_cast<int64_t>(tn) * BN;
+  const int64_t kbeg = static_cast<int64_t>(blockIdx.y) * p.kChunk;
+  const int64_t kend = min(p.Kc, kbeg + p.kChunk);
+  const int nstages = kend > kbeg ? static_cast<int>((kend - kbeg) / BK) : 0;
+
+  // staging registers, named (a private array would be promoted to LDS)
+  uint4 sa0, sa1, sa2, sa3, sb0, sb1, sb2, sb3;
+  uint2 va0, va1, va2, va3, vb0, vb1, vb2, vb3;
+  constexpr int AR = NT / BM, BR = NT / BN;
+  const int ap = t / BM, am = t % BM;
+  const int bp = t / BN, bn = t % BN;
+  auto load = [&](int s) {
+    const int64_t pb = (kbeg + static_cast<int64_t>(s) * BK) >> 1;
+    const int64_t oa = (pb + ap) * p.Mp + m0 + am, ob = (pb + bp) * p.Np + n0 + bn;
+    const int64_t sa = AR * p.Mp, sbs = BR * p.Np;
+    sa0 = p.A[oa]; sa1 = p.A[oa + sa]; sa2 = p.A[oa + 2 * sa]; sa3 = p.A[oa + 3 * sa];
+    va0 = p.As[oa]; va1 = p.As[oa + sa]; va2 = p.As[oa + 2 * sa]; va3 = p.As[oa + 3 * sa];
+    sb0 = p.B[ob]; sb1 = p.B[ob + sbs]; sb2 = p.B[ob + 2 * sbs]; sb3 = p.B[ob + 3 * sbs];
+    vb0 = p.Bs[ob]; vb1 = p.Bs[ob + sbs]; vb2 = p.Bs[ob + 2 * sbs]; vb3 = p.Bs[ob + 3 * sbs];
+  };
+  auto store = [&](int buf) {
+    As[buf][ap][am] = sa0; As[buf][ap + AR][am] = sa1;
+    As[buf][ap + 2 * AR][am] = sa2; As[buf][ap + 3 * AR][am] = sa3;
+    Av[buf][ap][am] = va0; Av[buf][ap + AR][am] = va1;
+    Av[buf][ap + 2 * AR][am] = va2; Av[buf][ap + 3 * AR][am] = va3;
+    Bs[buf][bp][bn] = sb0; Bs[buf][bp + BR][bn] = sb1;
+    Bs[buf][bp + 2 * BR][bn] = sb2; Bs[buf][bp + 3 * BR][bn] = sb3;
+    Bv[buf][bp][bn] = vb0; Bv[buf][bp + BR][bn] = vb1;
+    Bv[buf][bp + 2 * BR][bn] = vb2; Bv[buf][bp + 3 * BR][bn] = vb3;
+  };
+
+  const int lane = t & 63;
+  const int w = t >> 6;
+  const int wm = w / WAVES_N, wn = w % WAVES_N;
+  // P1, P2, P3 of each 32x32 block
+  f32x16 acc1[MB][NB], acc2[MB][NB], acc3[MB][NB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        acc1[i][j][r] = 0.f;
+        acc2[i][j][r] = 0.f;
+        acc3[i][j][r] = 0.f;
+      }
+
+  if (nstages > 0) {
+    load(0);
+    store(0);
+    if (nstages > 1) load(1);
+    __syncthreads();
+  }
+  const int ra = wm * WM + (lane & 31), rb = wn * WN + (lane & 31);
+  const int pl = 4 * (lane >> 5);  // lanes 0-31: k 0-7 of the stage, 32-63: k 8-15
+  for (int s = 0; s < nstages; ++s) {
+    const int buf = s & 1;
+    // the stage's words of this half-wave (8 k) for both M-blocks and both
+    // N-blocks, read from LDS once: xa[part][block] / yb[part][block], part
+    // 0 re, 1 im, 2 re + im
+    // (the re + im words last, so that the first products need not wait for them)
+    unsigned int xa[3][MB][8], yb[3][NB][8];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+#pragma unroll
+      for (int i = 0; i < MB; ++i) {
+        const uint4 q = As[buf][pl + e][ra + 32 * i];
+        xa[0][i][2 * e] = q.x; xa[0][i][2 * e + 1] = q.z;
+        xa[1][i][2 * e] = q.y; xa[1][i][2 * e + 1] = q.w;
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const uint4 q = Bs[buf][pl + e][rb + 32 * j];
+        yb[0][j][2 * e] = q.x; yb[0][j][2 * e + 1] = q.z;
+        yb[1][j][2 * e] = q.y; yb[1][j][2 * e + 1] = q.w;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+#pragma unroll
+      for (int i = 0; i < MB; ++i) {
+        const uint2 v = Av[buf][pl + e][ra + 32 * i];
+        xa[2][i][2 * e] = v.x; xa[2][i][2 * e + 1] = v.y;
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const uint2 v = Bv[buf][pl + e][rb + 32 * j];
+        yb[2][j][2 * e] = v.x; yb[2][j][2 * e + 1] = v.y;
+      }
+    }
+    // stage s+1 goes to the other LDS buffer (read in stage s-1, released by
+    // its barrier) and stage s+2 is loaded, before the products
+    if (s + 1 < nstages) {
+      store(buf ^ 1);
+      if (s + 2 < nstages) load(s + 2);
+    }
+    auto product = [&](int part, f32x16 (&acc)[MB][NB]) {
+      const Ops9 a0 = expand9_a(xa[part][0]), a1 = expand9_a(xa[part][1]);
+      const Ops9 b0 = expand9_b(yb[part][0]), b1 = expand9_b(yb[part][1]);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        acc[0][0] = mfma_split<true>(a0.v[m], b0.v[m], acc[0][0]);
+        acc[0][1] = mfma_split<true>(a0.v[m], b1.v[m], acc[0][1]);
+        acc[1][0] = mfma_split<true>(a1.v[m], b0.v[m], acc[1][0]);
+        acc[1][1] = mfma_split<true>(a1.v[m], b1.v[m], acc[1][1]);
+      }
+    };
+    product(0, acc1);
+    product(1, acc2);
+    product(2, acc3);
+    __syncthreads();
+  }
+
+  // lane l, register r of a 32x32 block: row 8*(r/4) + 4*(l/32) + r%4, column l%32
+  const double unscale = 1.0 / (p.scaleA[0] * p.scaleB[0]);
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int64_t n = n0 + wn * WN + j * 32 + (lane & 31);
+    if (n >= p.N) continue;
+    const int64_t cc = p.W ? 0 : p.colC[n];
+#pragma unroll
+    for (int i = 0; i < MB; ++i) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t m = m0 + wm * WM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+        if (m >= p.M) continue;
+        const double p1 = static_cast<double>(acc1[i][j][r]);
+        const double p2 = static_cast<double>(acc2[i][j][r]);
+        const double p3 = static_cast<double>(acc3[i][j][r]);
+        const double vr = unscale * (p1 - p2);
+        const double vi = unscale * (p3 - (p1 + p2));
+        if (p.W) {
+          const int64_t o = (static_cast<int64_t>(blockIdx.y) * p.M + m) * p.N + n;
+          reinterpret_cast<double2 *>(p.W)[o] = make_double2(vr, vi);
+        } else {
+          double2 *c = reinterpret_cast<double2 *>(p.C) + p.rowC[m] + cc;
+          double2 out = make_double2(p.alpha * vr, p.alpha * vi);
+          if (p.beta != 0.0) {
+            const double2 old = *c;
+            out.x += p.beta * old.x;
+            out.y += p.beta * old.y;
+          }
+          *c = out;
+        }
+      }
+    }
+  }
+}
+
 // multiply tiles: 1 = 8 waves on 128 x 256 (one workgroup per CU, 96 KB of
 // LDS); 2 (default) = 4 waves on 128 x 128 (two workgroups per CU, 64 KB
 // each: one's LDS stores and barriers overlap the other's MFMAs; 3.3 %
@@ -493,18 +718,19 @@ void dispatch_bgemm(const BgArgs &a, dim3 grid, bool half, int terms, hipStream_
 
 void set_mixed_tile(int t) { g_mixed_tile = t == 1 ? 1 : 2; }
 
-MixedTile mixed_tile() {
-  return g_mixed_tile == 2 ? MixedTile{128, 128, kXBK} : MixedTile{128, 256, kXBK};
+MixedTile mixed_tile(int terms) {
+  return g_mixed_tile == 2 || terms == 9 ? MixedTile{128, 128, kXBK}
+                                         : MixedTile{128, 256, kXBK};
 }
 
 void launch_split_scale(const double *X, const int64_t *offMN, const int64_t *offK,
-                        int64_t MN, int64_t K, bool kcontig, double *partials,
-                        double *scale, hipStream_t s) {
+                        int64_t MN, int64_t K, bool kcontig, bool sumabs,
+                        double *partials, double *scale, hipStream_t s) {
   const int64_t n = MN * K;
   const int blocks = static_cast<int>(std::max<int64_t>(
       1, std::min<int64_t>(kMaxBlocks, (n + 255) / 256)));
   maxabs_kernel<<<blocks, 256, 0, s>>>(reinterpret_cast<const double2 *>(X), offMN, offK,
-                                       MN, K, kcontig, partials);
+                                       MN, K, kcontig, sumabs, partials);
   CC4S_HIP(hipGetLastError());
   scale_kernel<<<1, 256, 0, s>>>(partials, blocks, scale);
   CC4S_HIP(hipGetLastError());
@@ -514,8 +740,9 @@ size_t split_scale_partials_bytes() { return kMaxBlocks * sizeof(double); }
 
 void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
                  int64_t MN, int64_t K, int64_t k0, int64_t Kc, int64_t MNp,
-                 bool conj, bool kcontig, void *out, const double *scale,
-                 hipStream_t s) {
+                 bool conj, bool kcontig, void *out, void *out_s,
+                 const double *scale, hipStream_t s) {
+  CC4S_ASSERT(out_s == nullptr || scale != nullptr, "the re + im plane is half only");
   PackArgs a;
   a.X = reinterpret_cast<const double2 *>(X);
   a.offMN = offMN;
@@ -528,13 +755,16 @@ void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
   a.sign = conj ? -1.0 : 1.0;
   a.kcontig = kcontig;
   a.out = static_cast<uint4 *>(out);
+  a.out_s = static_cast<uint2 *>(out_s);
   a.scale = scale;
   dim3 grid(static_cast<unsigned>((MNp + kPackTile - 1) / kPackTile),
             static_cast<unsigned>((Kc + kPackTile - 1) / kPackTile));
-  if (scale)
-    pack_kernel<true><<<grid, 256, 0, s>>>(a);
+  if (out_s)
+    pack_kernel<true, true><<<grid, 256, 0, s>>>(a);
+  else if (scale)
+    pack_kernel<true, false><<<grid, 256, 0, s>>>(a);
   else
-    pack_kernel<false><<<grid, 256, 0, s>>>(a);
+    pack_kernel<false, false><<<grid, 256, 0, s>>>(a);
   CC4S_HIP(hipGetLastError());
 }
 
@@ -542,11 +772,14 @@ void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
                   const int64_t *rowC, const int64_t *colC, int64_t M, int64_t N,
                   int64_t Mp, int64_t Np, int64_t Kc, int splits, double alpha,
                   double beta, const double *scaleA, const double *scaleB,
-                  int terms, hipStream_t s) {
+                  int terms, const void *Asum, const void *Bsum, hipStream_t s) {
   CC4S_ASSERT((scaleA == nullptr) == (scaleB == nullptr),
               "split-half operands need both scales");
-  CC4S_ASSERT(terms == 12 || terms == 16, "split products per complex MAC: 12 or 16");
-  const MixedTile T = mixed_tile();
+  CC4S_ASSERT(terms == 9 || terms == 12 || terms == 16,
+              "split products per complex MAC: 9, 12 or 16");
+  CC4S_ASSERT(terms != 9 || (scaleA && Asum && Bsum),
+              "9 products need half parts and the re + im planes");
+  const MixedTile T = mixed_tile(terms);
   CC4S_ASSERT(Kc % T.BK == 0 && Mp % T.BM == 0 && Np % T.BN == 0,
               "packed operands are padded to whole tiles");
   BgArgs a;
@@ -554,6 +787,8 @@ void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
   a.scaleB = scaleB;
   a.A = static_cast<const uint4 *>(Apack);
   a.B = static_cast<const uint4 *>(Bpack);
+  a.As = static_cast<const uint2 *>(Asum);
+  a.Bs = static_cast<const uint2 *>(Bsum);
   a.C = C;
   a.W = splits > 1 ? W : nullptr;
   a.rowC = rowC;
@@ -566,7 +801,9 @@ void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
   a.tilesM = static_cast<int>(Mp / T.BM);
   a.tilesN = static_cast<int>(Np / T.BN);
   dim3 grid(a.tilesM * a.tilesN, splits);
-  if (g_mixed_tile == 2)
+  if (terms == 9)
+    bgemm9_kernel<256, 128, 128, kXBK, 64, 64><<<grid, 256, 0, s>>>(a);
+  else if (g_mixed_tile == 2)
     dispatch_bgemm<256, 128, 128, 2>(a, grid, scaleA != nullptr, terms, s);
   else
     dispatch_bgemm<512, 128, 256, 1>(a, grid, scaleA != nullptr, terms, s);

@@ -36,12 +36,16 @@ def test_mixed_contract_accuracy(case):
         assert rel_err(out, ref) < 5e-5, (case, alpha, beta, rel_err(out, ref))
 
 
+@pytest.mark.parametrize("terms", ["12", "9"])
 @pytest.mark.parametrize("case", CASES, ids=["%s,%s->%s" % c for c in CASES])
-def test_half_split_contract_accuracy(case, monkeypatch):
+def test_half_split_contract_accuracy(case, terms, monkeypatch):
     """Half-precision split parts with power-of-two operand scaling: about
     22 significant bits per part, the FP32 sums bound the error; operands
-    spanning many decades (scaled by 1e-6) keep it."""
+    spanning many decades (scaled by 1e-6) keep it.  With 9 products per
+    complex multiply-add (Re Re, Im Im and (Re + Im)(Re + Im), the scale
+    taken from max |Re| + |Im|) the same bar holds."""
     monkeypatch.setenv("CC4S_MIXED_SPLIT", "half")
+    monkeypatch.setenv("CC4S_MIXED_TERMS", terms)
     ia, ib, ic = case
     rng = np.random.default_rng(7)
     mk = lambda idx: (rng.standard_normal(tuple(LENS[c] for c in idx))
@@ -55,7 +59,10 @@ def test_half_split_contract_accuracy(case, monkeypatch):
         assert rel_err(out, ref) < 2e-6, (case, alpha, beta, rel_err(out, ref))
 
 
-def test_mixed_contract_conjugation_and_large_k():
+@pytest.mark.parametrize("split,terms", [("bf16", "12"), ("half", "9")])
+def test_mixed_contract_conjugation_and_large_k(split, terms, monkeypatch):
+    monkeypatch.setenv("CC4S_MIXED_SPLIT", split)
+    monkeypatch.setenv("CC4S_MIXED_TERMS", terms)
     rng = np.random.default_rng(3)
     M, N, K = 300, 200, 9000
     A = rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))
@@ -72,11 +79,13 @@ def test_mixed_rejects_real_field():
                      np.zeros((3, 2)), "mn", precision=api.PRECISION_MIXED)
 
 
-@pytest.mark.parametrize("split,bound", [("bf16", 1e-4), ("half", 1e-5)])
-def test_mixed_residuum_accuracy(split, bound, monkeypatch):
+@pytest.mark.parametrize("split,terms,bound", [("bf16", "12", 1e-4), ("half", "12", 1e-5),
+                                               ("half", "9", 1e-5)])
+def test_mixed_residuum_accuracy(split, terms, bound, monkeypatch):
     """At No=12, Nv=60, NG=300 the ring, Gamma and ladder contractions are
     above the mixed threshold (1e9 flops) and run split."""
     monkeypatch.setenv("CC4S_MIXED_SPLIT", split)
+    monkeypatch.setenv("CC4S_MIXED_TERMS", terms)
     No, Nv, NG = 12, 60, 300
     inp = syn.generate(No, Nv, NG, "complex", 5)
     T = syn.random_amplitudes(No, Nv, "complex", 9, t1_scale=0.02, t2_scale=0.02)
@@ -86,7 +95,8 @@ def test_mixed_residuum_accuracy(split, bound, monkeypatch):
             cc.set_inputs(inp.gamma, inp.eps)
             out[prec] = cc.residuum(T)
     e1, e2 = rel_err(out[1][0], out[0][0]), rel_err(out[1][1], out[0][1])
-    print(f"split {split}: residuum rel. error R1 {e1:.2e} R2 {e2:.2e}", flush=True)
+    print(f"split {split} ({terms} products): residuum rel. error R1 {e1:.2e} R2 {e2:.2e}",
+          flush=True)
     assert e2 < bound and e1 < bound
     assert e2 > 0.0  # the mixed path ran
 
