@@ -1390,7 +1390,7 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
           : nullptr;
   double *W = nullptr;
   if (splits > 1) {
-    workspace_.reserve(static_cast<size_t>(splits) * P.M * P.N * 16);
+    workspace_.reserve(static_cast<size_t>(splits) * P.M * P.N * (nine ? 16 : 8));
     W = workspace_.as<double>();
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1424,9 +1424,12 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
     launch_bgemm(Ause, Buse, static_cast<double *>(C.ptr), W, P.rowC, P.colC, P.M,
                  P.N, Mp, Np, kc, splits, alpha, bc, useScA, useScB, terms, AuseS, BuseS,
                  stream_);
-    if (splits > 1)
+    if (splits > 1 && nine)
       launch_splitk_reduce(Field::Complex, W, static_cast<double *>(C.ptr), P.rowC,
                            P.colC, P.M, P.N, splits, alpha, bc, stream_);
+    else if (splits > 1)
+      launch_split_reduce(W, static_cast<double *>(C.ptr), P.rowC, P.colC, P.M, P.N,
+                          splits, alpha, bc, useScA, useScB, stream_);
   }
   if (profiling_) {
     CC4S_HIP(hipEventRecord(e1, stream_));

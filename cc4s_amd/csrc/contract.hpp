@@ -207,11 +207,20 @@ void launch_split_scale(const double *X, const int64_t *offMN, const int64_t *of
                         int64_t MN, int64_t K, bool kcontig, bool sumabs,
                         double *partials, double *scale, hipStream_t s);
 size_t split_scale_partials_bytes();
+// C = alpha * sum_s W[s] / (scaleA[0] scaleB[0]) + beta * C over the FP32
+// partials of launch_bgemm (scales null: bf16 parts, no scaling)
+void launch_split_reduce(const void *W, double *C, const int64_t *rowC,
+                         const int64_t *colC, int64_t M, int64_t N, int splits,
+                         double alpha, double beta, const double *scaleA,
+                         const double *scaleB, hipStream_t s);
 // C (or split-K partials W) from packed operands ([Kc/2][Mp] and [Kc/2][Np]
 // pairs); scaleA/scaleB (both or neither): half parts, the sums are divided
 // by both; terms: 16-bit products per complex multiply-add (12 or 16, or 9:
-// half parts only, with the Re + Im planes Asum/Bsum of both operands)
-void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
+// half parts only, with the Re + Im planes Asum/Bsum of both operands).
+// Partials W [splits][M][N]: with 12 or 16 terms the unscaled FP32 sums
+// (8 bytes per element, reduced by launch_split_reduce), with 9 the scaled
+// FP64 values (16 bytes, reduced by launch_splitk_reduce).
+void launch_bgemm(const void *Apack, const void *Bpack, double *C, void *W,
                   const int64_t *rowC, const int64_t *colC, int64_t M, int64_t N,
                   int64_t Mp, int64_t Np, int64_t Kc, int splits, double alpha,
                   double beta, const double *scaleA, const double *scaleB,

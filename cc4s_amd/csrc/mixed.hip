@@ -206,7 +206,10 @@ struct BgArgs {
   const uint4 *B;  // [Kc/2][Np] pairs of k
   const uint2 *As, *Bs;  // nine products: the re + im words, [Kc/2][Mp], [Kc/2][Np]
   double *C;
-  double *W;
+  // split-K partials [splits][M][N]: the raw FP32 sums (float2) of
+  // bgemm_kernel, whose scale the reduction applies; FP64 (double2, scaled)
+  // from bgemm9_kernel
+  void *W;
   const int64_t *rowC, *colC;
   int64_t M, N, Mp, Np, Kc, kChunk;
   double alpha, beta;
@@ -462,12 +465,14 @@ __global__ __launch_bounds__(NT, MINW) void bgemm_kernel(BgArgs p) {
       for (int r = 0; r < 16; ++r) {
         const int64_t m = m0 + wm * WM + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
         if (m >= p.M) continue;
-        const double vr = unscale * static_cast<double>(accR[i][j][r]);
-        const double vi = unscale * static_cast<double>(accI[i][j][r]);
         if (p.W) {
+          // an FP32 sum is stored exactly as it is; the power-of-two unscale
+          // is applied to the FP64 sum of the splits
           const int64_t o = (static_cast<int64_t>(blockIdx.y) * p.M + m) * p.N + n;
-          reinterpret_cast<double2 *>(p.W)[o] = make_double2(vr, vi);
+          reinterpret_cast<float2 *>(p.W)[o] = make_float2(accR[i][j][r], accI[i][j][r]);
         } else {
+          const double vr = unscale * static_cast<double>(accR[i][j][r]);
+          const double vi = unscale * static_cast<double>(accI[i][j][r]);
           double2 *c = reinterpret_cast<double2 *>(p.C) + p.rowC[m] + cc;
           double2 out = make_double2(p.alpha * vr, p.alpha * vi);
           if (p.beta != 0.0) {
@@ -479,6 +484,38 @@ __global__ __launch_bounds__(NT, MINW) void bgemm_kernel(BgArgs p) {
         }
       }
     }
+  }
+}
+
+// C = alpha / (scaleA scaleB) * sum_s W[s] + beta * C for the FP32 split-K
+// partials of bgemm_kernel, summed in FP64 in split order (the same result,
+// bit for bit, as summing the scaled partials: the scales are powers of two).
+// Threads run along n (coalesced partials), blocks stride over m.
+__global__ __launch_bounds__(256) void split_reduce_kernel(
+    const float2 *W, double2 *C, const int64_t *rowC, const int64_t *colC, int64_t M,
+    int64_t N, int splits, double alpha, double beta, const double *scaleA,
+    const double *scaleB) {
+  const int64_t n = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (n >= N) return;
+  const double f = scaleA ? alpha * (1.0 / (scaleA[0] * scaleB[0])) : alpha;
+  const int64_t total = M * N;
+  const int64_t cn = colC[n];
+  for (int64_t m = blockIdx.y; m < M; m += gridDim.y) {
+    const int64_t e = m * N + n;
+    double sr = 0.0, si = 0.0;
+    for (int s = 0; s < splits; ++s) {
+      const float2 v = W[s * total + e];
+      sr += static_cast<double>(v.x);
+      si += static_cast<double>(v.y);
+    }
+    double2 *c = C + rowC[m] + cn;
+    double2 out = make_double2(f * sr, f * si);
+    if (beta != 0.0) {
+      const double2 old = *c;
+      out.x += beta * old.x;
+      out.y += beta * old.y;
+    }
+    *c = out;
   }
 }
 
@@ -738,6 +775,19 @@ void launch_split_scale(const double *X, const int64_t *offMN, const int64_t *of
 
 size_t split_scale_partials_bytes() { return kMaxBlocks * sizeof(double); }
 
+void launch_split_reduce(const void *W, double *C, const int64_t *rowC,
+                         const int64_t *colC, int64_t M, int64_t N, int splits,
+                         double alpha, double beta, const double *scaleA,
+                         const double *scaleB, hipStream_t s) {
+  if (M <= 0 || N <= 0) return;
+  const dim3 grid(static_cast<unsigned>((N + 255) / 256),
+                  static_cast<unsigned>(std::min<int64_t>(M, 4096)));
+  split_reduce_kernel<<<grid, 256, 0, s>>>(static_cast<const float2 *>(W),
+                                           reinterpret_cast<double2 *>(C), rowC, colC, M,
+                                           N, splits, alpha, beta, scaleA, scaleB);
+  CC4S_HIP(hipGetLastError());
+}
+
 void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
                  int64_t MN, int64_t K, int64_t k0, int64_t Kc, int64_t MNp,
                  bool conj, bool kcontig, void *out, void *out_s,
@@ -768,7 +818,7 @@ void launch_pack(const double *X, const int64_t *offMN, const int64_t *offK,
   CC4S_HIP(hipGetLastError());
 }
 
-void launch_bgemm(const void *Apack, const void *Bpack, double *C, double *W,
+void launch_bgemm(const void *Apack, const void *Bpack, double *C, void *W,
                   const int64_t *rowC, const int64_t *colC, int64_t M, int64_t N,
                   int64_t Mp, int64_t Np, int64_t Kc, int splits, double alpha,
                   double beta, const double *scaleA, const double *scaleB,
