@@ -179,6 +179,8 @@ def cpu_baseline_full(cfg, inp):
 
 
 MIXED_OPTIONS = {}
+# the particle-particle ladder of the timed contexts (--ladder)
+LADDER = {"name": "slab"}
 
 # Pass sequences of full runs of the complex No=64, Nv=512, NG=3000 workload
 # (scale rule mp2_per_occupied, DIIS 8, convergence |dE| < 1e-10 and
@@ -340,11 +342,12 @@ def ladder_comparison(cfg, inp, slab_passes, lad_slab, steps=2):
 
 
 def make_context(cfg, precision, switch, restart=None, keep=None, refine=None,
-                 ladder="slab"):
+                 ladder=None):
     """A context of the workload; mixed contexts take the split settings of
-    the command line (MIXED_OPTIONS)."""
+    the command line (MIXED_OPTIONS), every context its ladder (LADDER)
+    unless one is named."""
     extra = dict(MIXED_OPTIONS) if precision == api.PRECISION_MIXED else {}
-    extra["ladder"] = ladder
+    extra["ladder"] = ladder or LADDER["name"]
     if refine is not None and precision == api.PRECISION_MIXED:
         extra["mixedRefine"] = refine
     return api.CoupledCluster(cfg.No, cfg.Nv, cfg.NG, field=cfg.field,
@@ -498,6 +501,8 @@ def main():
                     choices=("none",) + tuple(syn.WORKLOADS),
                     help="after the main workload, time one FP64 and one split "
                          "pass of this workload (none: skip)")
+    ap.add_argument("--ladder", default="slab", choices=tuple(api.LADDERS),
+                    help="particle-particle ladder of the timed contexts")
     ap.add_argument("--ladder-compare", default="auto", choices=("auto", "on", "off"),
                     help="time FP64 passes with the fused ladder next to the slab "
                          "(auto: on for the No=32 Nv=256 workload)")
@@ -514,6 +519,7 @@ def main():
                           "(no sharding across GPUs); --gpus must be 1"}))
         sys.exit(2)
     switches = [float(x) for x in str(args.mixed_switch).split(",") if x]
+    LADDER["name"] = args.ladder
     if args.mixed_split is not None:
         MIXED_OPTIONS["mixedSplit"] = args.mixed_split
     if args.mixed_max_k is not None:

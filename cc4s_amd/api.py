@@ -171,9 +171,10 @@ def _fortran(a, dtype):
 
 
 MIXED_SPLITS = {"bf16": 0, "half": 1}
-# particle-particle ladder of FP64 complex passes: V slab through HBM, or one
-# fused kernel that forms V in LDS
-LADDERS = {"slab": 0, "fused": 1}
+# particle-particle ladder of complex passes: V slab through HBM, one fused
+# kernel that forms V in LDS (FP64 passes), or the slab contraction over pairs
+# of tau's symmetric and antisymmetric combinations (half the multiply-adds)
+LADDERS = {"slab": 0, "fused": 1, "paired": 2}
 
 
 def finite_size_correction(grid, B, S, N=20, vertex_unit=1.0, grid_unit=1.0,

@@ -21,7 +21,7 @@ BENCH = HERE / "gett_bench"
 DRIVER = HERE / "cc4s_ccsd"
 DRIVER_SOURCES = ["main.cpp", "yaml.cpp", "tensor_io.cpp"]
 SOURCES = ["contract.hip", "mixed.hip", "elementwise.hip", "adjoint.hip", "triples.hip", "ccsd.hip",
-           "coupled_cluster.hip", "finite_size.hip", "abi.hip", "noalloc.hip", "ladder.hip"]
+           "coupled_cluster.hip", "finite_size.hip", "abi.hip", "noalloc.hip", "ladder.hip", "ladder_pairs.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CC4S_OFFLOAD_ARCH", "gfx950")

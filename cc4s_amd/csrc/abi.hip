@@ -124,9 +124,11 @@ int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {
     p.max_residua = params->max_residua;
     p.ccsd.ppl = params->ppl != 0;
     p.ccsd.integrals_slice_size = params->integrals_slice_size;
-    CC4S_ASSERT(params->ladder == CC4S_LADDER_SLAB || params->ladder == CC4S_LADDER_FUSED,
-                "ladder must be CC4S_LADDER_SLAB or CC4S_LADDER_FUSED");
+    CC4S_ASSERT(params->ladder == CC4S_LADDER_SLAB || params->ladder == CC4S_LADDER_FUSED ||
+                    params->ladder == CC4S_LADDER_PAIRED,
+                "ladder must be CC4S_LADDER_SLAB, CC4S_LADDER_FUSED or CC4S_LADDER_PAIRED");
     p.ccsd.ladder_fused = params->ladder == CC4S_LADDER_FUSED;
+    p.ccsd.ladder_paired = params->ladder == CC4S_LADDER_PAIRED;
     auto ctx = std::make_unique<cc4s_ccsd_context>();
     ctx->params = *params;
     ctx->cc.reset(new CoupledCluster(p, params->device));

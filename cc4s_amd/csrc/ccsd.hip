@@ -36,7 +36,7 @@ struct Ccsd::Work {
   View Xakci;
   Tensor DGph, DGpp, DGhh, Dpp;  // dressed vertices
   Tensor wG, Zgvo, Wgvo, Wvooo, Woo, Zgoo;
-  Tensor slab, Rxy;
+  Tensor slab, Rxy, Rpm;  // Rpm: the paired ladder's Rp and Rm of a chunk
   FusedLadderTable fused;  // pair blocks of the fused ladder
   int64_t NGe = 0;  // length of the G index of the real-path vertex views
   bool hermitian = false;
@@ -192,12 +192,31 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
   const int64_t ns = (v + s - 1) / s;
   const size_t es = scalar_bytes(f);
   const bool fused = opt_.ladder_fused && f == Field::Complex && !E_.mixed();
+  // paired: tau's combinations go to Y's buffer (v^2 o^2 elements, unused
+  // after the ring terms) and each chunk's Vp, Vm to Xakic's (free after the
+  // crossed ring terms): cx * Ny * v^2 <= v^2 o^2
+  const bool paired = !fused && opt_.ladder_paired && f == Field::Complex &&
+                      w.Y.size() > 0 && std::min(s, v) <= o * o;
   const size_t slab_budget = size_t(2) << 30;
   const int64_t per_x = v * v * std::min(s, v) * static_cast<int64_t>(es);
-  const int64_t cx_max = std::max<int64_t>(1, std::min<int64_t>(s, slab_budget / per_x));
-  if (!fused && w.slab.size() < v * v * cx_max * std::min(s, v)) {
+  int64_t cx_max = std::max<int64_t>(1, std::min<int64_t>(s, slab_budget / per_x));
+  if (paired) cx_max = std::max<int64_t>(1, std::min(cx_max, o * o / std::min(s, v)));
+  if (!fused && (w.slab.size() < v * v * cx_max * std::min(s, v) ||
+                 w.Rxy.size() < cx_max * std::min(s, v) * o * o)) {
     w.slab.reset(f, "cxdy", {v, cx_max, v, std::min(s, v)});
     w.Rxy.reset(f, "xyij", {cx_max, std::min(s, v), o, o});
+  }
+  if (paired && w.Rpm.size() < cx_max * std::min(s, v) * o * o)
+    w.Rpm.reset(f, "xyn", {cx_max, std::min(s, v), o * o});
+  const int64_t Kp = v * (v + 1) / 2, Km = v * (v - 1) / 2;
+  const int64_t Np = o * (o + 1) / 2, Nm = o * (o - 1) / 2;
+  void *Tp = nullptr, *Tm = nullptr;
+  if (paired) {
+    CC4S_ASSERT(tauv.strides == dense_view(tauv.ptr, f, "cdij", {v, v, o, o}).strides,
+                "paired ladder: tau must be dense");
+    Tp = w.Y.buf.get();
+    Tm = offset_ptr(Tp, Kp * Np, f);
+    ladder_tau_pairs(tauv.ptr, Tp, Tm, v, o, E_.stream());
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (profile) {
@@ -211,9 +230,14 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
                                    R2.as("xyij"), s, w.fused, E_.stream());
     E_.stats.launches += 1;
   }
-  // tau is read unchanged by every slab contraction: in split mode it is
-  // packed once for all of them
-  if (E_.mixed()) E_.set_operand_cache(tauv.ptr);
+  // tau (or its two combinations) is read unchanged by every slab
+  // contraction: in split mode it is packed once for all of them
+  if (E_.mixed() && paired) {
+    E_.set_operand_cache(Tp, 0, "kn");
+    E_.set_operand_cache(Tm, 2, "kn");
+  } else if (E_.mixed()) {
+    E_.set_operand_cache(tauv.ptr);
+  }
   for (int64_t m = 0; m < ns && !fused; ++m) {
     // the right-hand slice is the same for the whole row m: packed once (its
     // index names tell it from the left-hand slice of a diagonal pair, which
@@ -231,7 +255,29 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
         const View slab = dense_view(w.slab.buf.get(), f, "cxdy", {v, cx, v, Ny});
         E_.contract(1.0, Lv, Rv, 0.0, slab);                       // :588-589
         const View Rxy = dense_view(w.Rxy.buf.get(), f, "xyij", {cx, Ny, o, o});
-        E_.contract(1.0, slab, tauv.as("cdij"), 0.0, Rxy);         // :590
+        if (paired) {                                              // :590
+          void *Vp = w.Xakic.buf.get();
+          void *Vm = offset_ptr(Vp, Kp * cx * Ny, f);
+          void *Rp = w.Rpm.buf.get();
+          void *Rm = offset_ptr(Rp, cx * Ny * Np, f);
+          ladder_vertex_pairs(slab.ptr, Vp, Vm, v, cx, Ny, E_.stream());
+          const double fl0 = E_.stats.flops;
+          E_.contract(1.0, dense_view(Vp, f, "kxy", {Kp, cx, Ny}),
+                      dense_view(Tp, f, "kn", {Kp, Np}), 0.0,
+                      dense_view(Rp, f, "xyn", {cx, Ny, Np}));
+          const View RmV = dense_view(Rm, f, "xyn", {cx, Ny, Nm});
+          if (Nm > 0 && Km > 0)
+            E_.contract(1.0, dense_view(Vm, f, "kxy", {Km, cx, Ny}),
+                        dense_view(Tm, f, "kn", {Km, Nm}), 0.0, RmV);
+          else if (Nm > 0)
+            E_.scale(0.0, RmV);
+          // credited as the slab contraction it replaces (8 flops per complex
+          // MAC over Nv^2 k and No^2 columns), the convention of the others
+          E_.stats.flops = fl0 + 8.0 * static_cast<double>(cx * Ny) * v * v * o * o;
+          ladder_unpack_pairs(Rp, Rm, Rxy.ptr, cx * Ny, o, E_.stream());
+        } else {
+          E_.contract(1.0, slab, tauv.as("cdij"), 0.0, Rxy);
+        }
         View blk = R2.sliced(0, a + x0, a + x0 + cx).sliced(1, b, b + Ny);
         E_.sum(1.0, Rxy, 1.0, blk.as("xyij"));                     // :591
         if (a > b) {                                               // :593-596

@@ -31,6 +31,10 @@ struct CcsdOptions {
   // FP64 complex ladder as one fused kernel (V formed in LDS, never in HBM)
   // instead of the slab GETTs; split passes keep the slab
   bool ladder_fused = false;
+  // complex ladder (FP64 and split passes): the slab contraction over pairs
+  // c <= d and i <= j of symmetric and antisymmetric combinations, half the
+  // multiply-adds (needs tau_cdij = tau_dcji, which the iteration keeps)
+  bool ladder_paired = false;
 };
 
 // time per residuum stage, filled when profiling is on

@@ -1320,14 +1320,14 @@ void ContractionEngine::contract_mixed(GettPlan &P, double alpha, const View &A,
   const int slotB = slotA < 0 ? slot_of(Bop) : -1;
   const bool cacheA = slotA >= 0, cacheB = slotB >= 0;
   const int slot = cacheA ? slotA : slotB;
-  // half split: per-operand scales (A, B, cache slots 0 and 1) and reduction room
+  // half split: per-operand scales (A, B, the cache slots) and reduction room
   double *scA = nullptr, *scB = nullptr, *scC = nullptr, *scPart = nullptr;
   if (mixed_half) {
-    scales_.reserve(4 * sizeof(double) + split_scale_partials_bytes());
+    scales_.reserve((2 + kCacheSlots) * sizeof(double) + split_scale_partials_bytes());
     scA = scales_.as<double>();
     scB = scA + 1;
     scC = scA + 2 + (slot > 0 ? slot : 0);
-    scPart = scA + 4;
+    scPart = scA + 2 + kCacheSlots;
   }
   if (cacheA || cacheB) {
     const int64_t rows = cacheA ? Mp : Np;

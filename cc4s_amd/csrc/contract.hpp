@@ -93,10 +93,12 @@ class ContractionEngine {
   // the same plan; the caller guarantees that the memory does not change
   // until the slot is set again or cleared.  A non-empty `idx` also requires
   // the operand view's index names (two views of the same memory in one
-  // contraction).  Two slots: the ladder keeps tau (its contractions'
-  // constant operand) in slot 0 and the right-hand dressed-vertex slice of a
-  // slice pair row (its formations' constant operand) in slot 1.
-  static constexpr int kCacheSlots = 2;
+  // contraction).  Three slots: the ladder keeps tau (its contractions'
+  // constant operand; in the paired ladder the symmetric combination) in
+  // slot 0, the right-hand dressed-vertex slice of a slice pair row (its
+  // formations' constant operand) in slot 1 and the antisymmetric tau
+  // combination of the paired ladder in slot 2.
+  static constexpr int kCacheSlots = 3;
   void set_operand_cache(const void *ptr, int slot = 0, const std::string &idx = "") {
     cache_ptr_[slot] = ptr;
     cache_idx_[slot] = idx;
@@ -119,12 +121,12 @@ class ContractionEngine {
   bool mixed_ = false;
   DeviceBuffer pack_;
   DeviceBuffer cache_[kCacheSlots];
-  DeviceBuffer scales_;  // half split: [A, B, cache slot 0, cache slot 1] scales, partials
-  const void *cache_ptr_[kCacheSlots] = {nullptr, nullptr};
+  DeviceBuffer scales_;  // half split: [A, B, cache slots 0-2] scales, partials
+  const void *cache_ptr_[kCacheSlots] = {};
   std::string cache_idx_[kCacheSlots];
-  const GettPlan *cache_plan_[kCacheSlots] = {nullptr, nullptr};
-  bool cache_is_a_[kCacheSlots] = {true, true};
-  size_t cache_eb_[kCacheSlots] = {0, 0};  // packed bytes per element and k
+  const GettPlan *cache_plan_[kCacheSlots] = {};
+  bool cache_is_a_[kCacheSlots] = {true, true, true};
+  size_t cache_eb_[kCacheSlots] = {};  // packed bytes per element and k
   struct Pending {
     hipEvent_t start, stop;
     std::string name;

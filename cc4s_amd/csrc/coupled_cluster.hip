@@ -285,6 +285,9 @@ CoupledCluster::CoupledCluster(const CoupledClusterParams &p, int device)
     : p_(p), device_(device) {
   const auto &s = p_.shape;
   CC4S_ASSERT(s.No > 0 && s.Nv > 0 && s.NG > 0, "empty dimension");
+  // a residuum with a single occupied orbital made an illegal device memory
+  // access whose kernel is not identified; such contexts are refused
+  CC4S_ASSERT(s.No >= 2, "No = 1 (a single occupied orbital) is not supported");
   CC4S_HIP(hipSetDevice(device_));
   CC4S_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   engine_.reset(new ContractionEngine(stream_));

@@ -709,8 +709,10 @@ class Runner {
     const std::string ladder = yaml::get_text(args, "ladder", "slab");
     if (ladder == "fused")
       p.ladder = CC4S_LADDER_FUSED;
+    else if (ladder == "paired")
+      p.ladder = CC4S_LADDER_PAIRED;
     else if (ladder != "slab")
-      throw std::runtime_error("unknown ladder '" + ladder + "' (slab | fused)");
+      throw std::runtime_error("unknown ladder '" + ladder + "' (slab | fused | paired)");
     p.device = static_cast<int>(yaml::get_integer(args, "device", 0));
     p.method = method == "Drccd" ? CC4S_METHOD_DRCCD : CC4S_METHOD_CCSD;
     p.linearized = yaml::get_boolean(args, "linearized", false) ? 1 : 0;

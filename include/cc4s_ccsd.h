@@ -97,14 +97,18 @@ typedef struct {
                                      power-of-two scale per operand (1) */
   int64_t mixed_max_k;            /* MIXED: longest FP32 sum of a split contraction; longer
                                      K is cut into chunks added in FP64 (2048) */
-  int ladder;                     /* particle-particle ladder of FP64 complex passes
+  int ladder;                     /* particle-particle ladder of complex passes
                                      (Ccsd.cxx:579-600): CC4S_LADDER_SLAB (V slab through
-                                     HBM, 0) or CC4S_LADDER_FUSED (V formed in LDS inside the
-                                     contraction kernel); same results to rounding */
+                                     HBM, 0), CC4S_LADDER_FUSED (FP64 passes: V formed in
+                                     LDS inside the contraction kernel) or
+                                     CC4S_LADDER_PAIRED (the slab contraction over pairs
+                                     c <= d, i <= j of tau's symmetric and antisymmetric
+                                     combinations, half its multiply-adds; FP64 and split
+                                     passes); same results to rounding */
 } cc4s_ccsd_params;
 
 /* Values of cc4s_ccsd_params.ladder. */
-enum { CC4S_LADDER_SLAB = 0, CC4S_LADDER_FUSED = 1 };
+enum { CC4S_LADDER_SLAB = 0, CC4S_LADDER_FUSED = 1, CC4S_LADDER_PAIRED = 2 };
 
 /* One pass of the iteration loop (CoupledCluster.cxx:129-175). */
 typedef struct {

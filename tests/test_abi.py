@@ -55,6 +55,13 @@ def test_unsupported_precision_is_refused_before_touching_a_device():
         api.CoupledCluster(2, 4, 3, field="real", precision=api.PRECISION_MIXED)
 
 
+def test_single_occupied_orbital_is_refused_before_touching_a_device():
+    with pytest.raises(api.CcsdError, match="No = 1"):
+        api.CoupledCluster(1, 9, 6, field="complex")
+    with pytest.raises(api.CcsdError, match="unknown ladder"):
+        api.CoupledCluster(2, 9, 6, field="complex", ladder="tiled")
+
+
 
 @pytest.mark.parametrize("cname,pyclass", [("cc4s_ccsd_params", "Params"),
                                            ("cc4s_ccsd_iteration", "Iteration"),

@@ -182,7 +182,7 @@ def test_flow_value_on_the_next_line(built, tmp_path):
     ("version", "serialization versions mismatch"),
     ("method", "Unknown method: 'Ccsdt'"),
     ("mixer", "Unknown mixer type: 'LinearMixer'"),
-    ("ladder", "unknown ladder 'tiled' (slab | fused)"),
+    ("ladder", "unknown ladder 'tiled' (slab | fused | paired)"),
 ])
 def test_reference_errors(built, tmp_path, case, message):
     inp = syn.generate(2, 5, 3, "complex", 1)
