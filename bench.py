@@ -180,7 +180,7 @@ def cpu_baseline_full(cfg, inp):
 
 MIXED_OPTIONS = {}
 # the particle-particle ladder of the timed contexts (--ladder)
-LADDER = {"name": "slab"}
+LADDER = {"name": "paired"}
 
 # Pass sequences of full runs of the complex No=64, Nv=512, NG=3000 workload
 # (scale rule mp2_per_occupied, DIIS 8, convergence |dE| < 1e-10 and
@@ -311,32 +311,44 @@ def large_workload_report(wname, scale, fused_ladder=False):
     return out
 
 
-def ladder_comparison(cfg, inp, slab_passes, lad_slab, steps=2):
-    """The particle-particle ladder of FP64 passes with the V slab (timed in
-    the main loop) and with the fused kernel (V formed in LDS, no slab): a
-    context with the fused ladder runs its first pass (R = V, no ladder)
-    untimed and `steps` passes with the ladder timed by HIP events around
-    it.  Its energies are compared with the slab context's passes of the same
-    index (same amplitudes up to rounding)."""
+def ladder_comparison(cfg, inp, main_passes, lad_main, steps=2):
+    """The particle-particle ladder of FP64 passes in each ladder mode: the
+    main loop's mode (timed there) and the others -- the V slab, the fused
+    kernel (V formed in LDS, no slab) and the paired contraction (over
+    c <= d, i <= j combinations of tau) -- each in a context that runs its
+    first pass (R = V, no ladder) untimed and `steps` passes with the ladder
+    timed by HIP events around it.  Their energies are compared with the
+    main context's passes of the same index (same amplitudes up to
+    rounding)."""
     def rate(ms, flops, calls):
         t = flops / (ms * 1e-3) / 1e12 if ms else None
         return {"ladder_ms_per_pass": ms / max(1, calls), "ladder_tflops_per_s": t,
                 "ladder_frac_fp64_mfma_peak": t / FP64_MFMA_PEAK_TFLOPS if t else None,
                 "flops_per_pass": flops / max(1, calls)}
-    out = {"slab": rate(*lad_slab)}
-    with make_context(cfg, api.PRECISION_FP64, 1e-5, ladder="fused") as cc:
-        cc.set_inputs(inp.gamma, inp.eps)
-        passes = [cc.iterate()]
-        cc.set_profiling(True)
-        for _ in range(steps):
-            passes.append(cc.iterate())
-        out["fused"] = rate(*cc.ladder_stats())
-        out["fused"]["seconds_per_iteration"] = float(np.mean([p["seconds"] for p in passes[1:]]))
-    n = min(len(passes), len(slab_passes))
-    out["max_abs_energy_difference_vs_slab"] = max(
-        abs(a["energy"] - b["energy"]) for a, b in zip(passes[:n], slab_passes[:n]))
-    out["fused_over_slab_time"] = (out["fused"]["ladder_ms_per_pass"]
-                                   / out["slab"]["ladder_ms_per_pass"])
+    main = LADDER["name"]
+    out = {main: rate(*lad_main)}
+    diff = 0.0
+    for other in api.LADDERS:
+        if other == main:
+            continue
+        with make_context(cfg, api.PRECISION_FP64, 1e-5, ladder=other) as cc:
+            cc.set_inputs(inp.gamma, inp.eps)
+            passes = [cc.iterate()]
+            cc.set_profiling(True)
+            for _ in range(steps):
+                passes.append(cc.iterate())
+            out[other] = rate(*cc.ladder_stats())
+            out[other]["seconds_per_iteration"] = float(
+                np.mean([p["seconds"] for p in passes[1:]]))
+        n = min(len(passes), len(main_passes))
+        diff = max(diff, max(abs(a["energy"] - b["energy"])
+                             for a, b in zip(passes[:n], main_passes[:n])))
+    out["main"] = main
+    out["max_abs_energy_difference_vs_main"] = diff
+    for other in api.LADDERS:
+        if other != main:
+            out[f"{other}_over_{main}_time"] = (out[other]["ladder_ms_per_pass"]
+                                               / out[main]["ladder_ms_per_pass"])
     log(f"ladder comparison: {json.dumps(out)}")
     return out
 
@@ -501,11 +513,11 @@ def main():
                     choices=("none",) + tuple(syn.WORKLOADS),
                     help="after the main workload, time one FP64 and one split "
                          "pass of this workload (none: skip)")
-    ap.add_argument("--ladder", default="slab", choices=tuple(api.LADDERS),
+    ap.add_argument("--ladder", default="paired", choices=tuple(api.LADDERS),
                     help="particle-particle ladder of the timed contexts")
     ap.add_argument("--ladder-compare", default="auto", choices=("auto", "on", "off"),
-                    help="time FP64 passes with the fused ladder next to the slab "
-                         "(auto: on for the No=32 Nv=256 workload)")
+                    help="time FP64 passes with the other ladder modes next to the "
+                         "main one (auto: on for the No=32 Nv=256 workload)")
     ap.add_argument("--large-fused-ladder", action="store_true",
                     help="also time an FP64 pass of the large workload with the fused ladder")
     ap.add_argument("--triples", action="store_true",

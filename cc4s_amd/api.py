@@ -207,7 +207,7 @@ class CoupledCluster:
                  integralsSliceSize=0, device=0, precision=PRECISION_FP64,
                  mixedSwitch=1e-5, method="Ccsd", linearized=False,
                  adjacentPairsExchange=False, mixedRestart=None, mixedRestartKeep=None,
-                 mixedRefine=None, mixedSplit=None, mixedMaxK=None, ladder="slab"):
+                 mixedRefine=None, mixedSplit=None, mixedMaxK=None, ladder="paired"):
         self.lib = load_library()
         p = default_params()
         p.field = FIELD_COMPLEX if field == "complex" else FIELD_REAL

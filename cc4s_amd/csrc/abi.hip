@@ -76,7 +76,7 @@ void cc4s_ccsd_default_params(cc4s_ccsd_params *p) {
   p->mixed_refine = 1;
   p->mixed_split = 1;
   p->mixed_max_k = 2048;
-  p->ladder = CC4S_LADDER_SLAB;
+  p->ladder = CC4S_LADDER_PAIRED;
 }
 
 int cc4s_ccsd_create(const cc4s_ccsd_params *params, cc4s_ccsd_context **out) {

@@ -706,7 +706,7 @@ class Runner {
     else if (precision != "fp64")
       throw std::runtime_error("unknown precision '" + precision + "' (fp64 | mixed)");
     p.mixed_switch = yaml::get_real(args, "mixedSwitch", p.mixed_switch);
-    const std::string ladder = yaml::get_text(args, "ladder", "slab");
+    const std::string ladder = yaml::get_text(args, "ladder", "paired");
     if (ladder == "fused")
       p.ladder = CC4S_LADDER_FUSED;
     else if (ladder == "paired")

@@ -99,12 +99,13 @@ typedef struct {
                                      K is cut into chunks added in FP64 (2048) */
   int ladder;                     /* particle-particle ladder of complex passes
                                      (Ccsd.cxx:579-600): CC4S_LADDER_SLAB (V slab through
-                                     HBM, 0), CC4S_LADDER_FUSED (FP64 passes: V formed in
-                                     LDS inside the contraction kernel) or
-                                     CC4S_LADDER_PAIRED (the slab contraction over pairs
-                                     c <= d, i <= j of tau's symmetric and antisymmetric
-                                     combinations, half its multiply-adds; FP64 and split
-                                     passes); same results to rounding */
+                                     HBM, as the reference), CC4S_LADDER_FUSED (FP64 passes:
+                                     V formed in LDS inside the contraction kernel) or
+                                     CC4S_LADDER_PAIRED (default: the slab contraction over
+                                     pairs c <= d, i <= j of tau's symmetric and
+                                     antisymmetric combinations, half its multiply-adds;
+                                     FP64 and split passes); same results to rounding.
+                                     The real field always uses the slab. */
 } cc4s_ccsd_params;
 
 /* Values of cc4s_ccsd_params.ladder. */

@@ -3,8 +3,8 @@ mp2_per_occupied rule) on the GPU against the oracle's committed fixtures
 (tests/golden/make_golden.py; the oracle needs about 1.5 hours of host time
 for the converged run, so the fixtures are made once, not in the test):
 
-* one residuum at seeded amplitudes (Ccsd.cxx:381-633), with the slab and
-  with the fused ladder: 256 sampled R1 and 2048 sampled R2 elements to
+* one residuum at seeded amplitudes (Ccsd.cxx:381-633), with the slab, the
+  fused and the paired ladder: 256 sampled R1 and 2048 sampled R2 elements to
   1e-12 of max |R|, checksums of R1 and R2, and the energy of the amplitudes
   (CoupledCluster.cxx:194-253) to 1e-12;
 * the converged FP64 run (CoupledCluster.cxx:129-175, DIIS 8): the energy of
@@ -66,7 +66,7 @@ def _compare_checksums(got, ref, n, tol):
         assert abs(got[key] - ref[key]) <= tol * scale, (key, got[key], ref[key])
 
 
-@pytest.mark.parametrize("ladder", ["slab", "fused"])
+@pytest.mark.parametrize("ladder", ["slab", "fused", "paired"])
 def test_residuum_at_seeded_amplitudes_against_the_oracle(ladder):
     fx = _fixture("complex_no32_nv256_ng1500_residuum.json")
     print(f"No=32 Nv=256 NG=1500 residuum ({ladder} ladder), scale {fx['scale']:.9e}",
