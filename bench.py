@@ -184,16 +184,17 @@ LADDER = {"name": "paired"}
 
 # Pass sequences of full runs of the complex No=64, Nv=512, NG=3000 workload
 # (scale rule mp2_per_occupied, DIIS 8, convergence |dE| < 1e-10 and
-# dR < 1e-8) to convergence on one MI355X: FP64 converges at pass 17; the
-# mixed run (half split, 2048-term FP32 sums, switch 1e-5, refinement 1)
-# does 8 split passes (the first is the R = V pass), the FP64 anchor, 7
-# corrected split passes and one FP64 pass.  F = FP64, S = split,
-# A = anchor (an FP64 and a split residuum), C = corrected split.
+# dR < 1e-8) to convergence on one MI355X with the paired ladder: FP64
+# converges at pass 17; the mixed run (half split, 2048-term FP32 sums,
+# switch 1e-5, refinement 1) does 8 split passes (the first is the R = V
+# pass), the FP64 anchor, 5 corrected split passes and one FP64 pass.
+# F = FP64, S = split, A = anchor (an FP64 and a split residuum),
+# C = corrected split.
 # dense half MFMA peak of MI355X and the algorithmic ceiling of a split
 # multiply with 12 half products per complex MAC (8 algorithmic flops)
 HALF_MFMA_PEAK_TFLOPS = 2500.0
 SPLIT12_CEILING_TFLOPS = HALF_MFMA_PEAK_TFLOPS * 8.0 / 24.0
-RECORDED_PASSES_NO64_NV512 = {"fp64": "F" * 17, "mixed": "S" * 8 + "A" + "C" * 7 + "F"}
+RECORDED_PASSES_NO64_NV512 = {"fp64": "F" * 17, "mixed": "S" * 8 + "A" + "C" * 5 + "F"}
 
 
 def dominant_kernel(stats):
