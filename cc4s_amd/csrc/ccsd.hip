@@ -216,7 +216,6 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
                 "paired ladder: tau must be dense");
     Tp = w.Y.buf.get();
     Tm = offset_ptr(Tp, Kp * Np, f);
-    ladder_tau_pairs(tauv.ptr, Tp, Tm, v, o, E_.stream());
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (profile) {
@@ -224,6 +223,7 @@ void Ccsd::ladder(const View &left, const View &right, const View &tauv,
     CC4S_HIP(hipEventCreate(&e1));
     CC4S_HIP(hipEventRecord(e0, E_.stream()));
   }
+  if (paired) ladder_tau_pairs(tauv.ptr, Tp, Tm, v, o, E_.stream());
   const double flops0 = E_.stats.flops;
   if (fused) {
     E_.stats.flops += ladder_fused(left.as("Gxc"), right.as("Gyd"), tauv.as("cdij"),
